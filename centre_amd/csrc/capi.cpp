@@ -1,0 +1,807 @@
+// extern "C" layer: argument checking, device scratch, plans, launches.
+#include <algorithm>
+#include <cstdio>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "centre_internal.h"
+
+namespace {
+
+int fail(centre_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                            \
+    do {                                                                     \
+        hipError_t e_ = (x);                                                 \
+        if (e_ != hipSuccess)                                                \
+            return fail(ctx, (int)e_, std::string(#x) + ": " +               \
+                                          hipGetErrorString(e_));            \
+    } while (0)
+
+struct DevBuf {
+    void* p = nullptr;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(size_t n) { return hipMalloc(&p, n ? n : 16); }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+int check_flag(centre_ctx* ctx) {
+    int f = 0;
+    HIPCHK(hipMemcpyAsync(&f, ctx->d_flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (f & 1) return fail(ctx, -4, "input code >= nbins");
+    if (f & 2) return fail(ctx, -4, "torsion value outside the gap histogram range");
+    return 0;
+}
+
+int check_ests(centre_ctx* ctx, const int* ests, int nest) {
+    if (!ests || nest <= 0 || nest > 4) return fail(ctx, -3, "need 1..4 estimators");
+    for (int e = 0; e < nest; ++e)
+        if (ests[e] < 0 || ests[e] > 3) return fail(ctx, -3, "unknown estimator id");
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int centre_init(int device, centre_ctx** out) {
+    if (!out) return -3;
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) return (int)e;
+    if (device < 0 || device >= n) return -3;
+    e = hipSetDevice(device);
+    if (e != hipSuccess) return (int)e;
+    centre_ctx* ctx = new centre_ctx();
+    ctx->device = device;
+    if ((e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipMalloc(&ctx->d_flag, sizeof(int))) != hipSuccess ||
+        (e = hipEventCreate(&ctx->ev0)) != hipSuccess ||
+        (e = hipEventCreate(&ctx->ev1)) != hipSuccess) {
+        centre_destroy(ctx);
+        return (int)e;
+    }
+    *out = ctx;
+    return 0;
+}
+
+void centre_destroy(centre_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->d_flag) (void)hipFree(ctx->d_flag);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* centre_last_error(const centre_ctx* ctx) {
+    return ctx ? ctx->err.c_str() : "null context";
+}
+
+int centre_sync(centre_ctx* ctx) {
+    if (!ctx) return -3;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+void* centre_stream(centre_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+float centre_last_pair_kernel_ms(const centre_ctx* ctx) {
+    return ctx ? ctx->last_pair_ms : 0.f;
+}
+
+// ---------------------------------------------------------------------------
+int centre_dev_discretize_hist(centre_ctx* ctx, const double* d_bat, uint64_t ld,
+                               uint32_t n_dof, uint64_t n_frame, int dof_type,
+                               const uint8_t* schemes, int n_schemes,
+                               int nbins_ref, int optimize, uint8_t* d_codes,
+                               double* d_extrema) {
+    if (!ctx) return -3;
+    if (!d_bat || !d_codes || !d_extrema || !schemes || n_schemes <= 0)
+        return fail(ctx, -3, "null buffer or no bin scheme");
+    if (dof_type < 1 || dof_type > 3) return fail(ctx, -3, "dof_type must be 1, 2 or 3");
+    if (ld < n_dof || n_frame == 0) return fail(ctx, -3, "ld < n_dof or no frames");
+    for (int s = 0; s < n_schemes; ++s)
+        if (schemes[s] == 0) return fail(ctx, -3, "zero bins in a scheme");
+    if (dof_type == 3 && optimize && nbins_ref <= 0)
+        return fail(ctx, -3, "nbins_ref required for torsion gap optimisation");
+    if (n_dof == 0) return 0;
+    HIPCHK(hipSetDevice(ctx->device));
+    const int nchunk = disc_chunks(n_frame);
+    DevBuf sch, part, gh, shift;
+    HIPCHK(sch.alloc(n_schemes));
+    HIPCHK(part.alloc(sizeof(double) * 8 * (size_t)nchunk * n_dof));
+    HIPCHK(gh.alloc(sizeof(uint32_t) * 180 * (size_t)n_dof));
+    HIPCHK(shift.alloc(sizeof(double) * n_dof));
+    HIPCHK(hipMemcpyAsync(sch.p, schemes, n_schemes, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->d_flag, 0, sizeof(int), ctx->stream));
+    DiscArgs a{};
+    a.bat = d_bat;
+    a.ld = ld;
+    a.n_dof = n_dof;
+    a.n_frame = n_frame;
+    a.dtype = dof_type;
+    a.schemes = sch.as<uint8_t>();
+    a.nsch = n_schemes;
+    a.nbins_ref = nbins_ref;
+    a.optimize = optimize;
+    a.codes = d_codes;
+    a.ext = d_extrema;
+    a.part = part.as<double>();
+    a.gaphist = gh.as<uint32_t>();
+    a.shift = shift.as<double>();
+    a.flag = ctx->d_flag;
+    HIPCHK(run_discretize_hist(a, nchunk, ctx->stream));
+    return check_flag(ctx);
+}
+
+int centre_discretize_hist(centre_ctx* ctx, const double* bat, uint64_t ld,
+                           uint32_t n_dof, uint64_t n_frame, int dof_type,
+                           const uint8_t* schemes, int n_schemes, int nbins_ref,
+                           int optimize, uint8_t* codes, double* extrema) {
+    if (!ctx) return -3;
+    if (!bat || !codes || !extrema) return fail(ctx, -3, "null buffer");
+    HIPCHK(hipSetDevice(ctx->device));
+    DevBuf b, c, e;
+    HIPCHK(b.alloc(sizeof(double) * ld * n_frame));
+    HIPCHK(c.alloc((size_t)n_dof * n_schemes * n_frame));
+    HIPCHK(e.alloc(sizeof(double) * 4 * n_dof));
+    HIPCHK(hipMemcpyAsync(b.p, bat, sizeof(double) * ld * n_frame, hipMemcpyHostToDevice, ctx->stream));
+    int r = centre_dev_discretize_hist(ctx, b.as<double>(), ld, n_dof, n_frame, dof_type,
+                                       schemes, n_schemes, nbins_ref, optimize,
+                                       c.as<uint8_t>(), e.as<double>());
+    if (r) return r;
+    HIPCHK(hipMemcpyAsync(codes, c.p, (size_t)n_dof * n_schemes * n_frame, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(extrema, e.p, sizeof(double) * 4 * n_dof, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+namespace {
+int run_dof1d(centre_ctx* ctx, const uint8_t* d_codes, uint64_t stride,
+              uint32_t n_dof, uint64_t n_frame, const uint8_t* d_types,
+              const double* d_ext, int nbins, int nsteps, const int* ests, int nest,
+              int jacobian, int kde, double* d_s1, double* d_L, double* d_w,
+              uint64_t* d_counts) {
+    DevBuf de;
+    HIPCHK(de.alloc(sizeof(int) * nest));
+    HIPCHK(hipMemcpyAsync(de.p, ests, sizeof(int) * nest, hipMemcpyHostToDevice, ctx->stream));
+    Dof1dArgs a{};
+    a.codes = d_codes;
+    a.stride = stride;
+    a.n_dof = (int)n_dof;
+    a.nb = nbins;
+    a.nsteps = nsteps;
+    a.ss = n_frame / nsteps;
+    a.types = d_types;
+    a.ext = d_ext;
+    a.jacobian = jacobian;
+    a.kde = kde;
+    a.ests = de.as<int>();
+    a.nest = nest;
+    a.s1 = d_s1;
+    a.L = d_L;
+    a.w = d_w;
+    a.counts = d_counts;
+    a.flag = ctx->d_flag;
+    HIPCHK(launch_dof1d(a, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));  // keeps `de` alive until done
+    return 0;
+}
+}  // namespace
+
+int centre_dev_dof_entropy(centre_ctx* ctx, const uint8_t* d_codes,
+                           uint64_t code_stride, uint32_t n_dof, uint64_t n_frame,
+                           const uint8_t* d_types, const double* d_extrema,
+                           int nbins, int nsteps, const int* ests, int nest,
+                           int jacobian, int kde, double* d_s1, uint64_t* d_counts) {
+    if (!ctx) return -3;
+    if (int r = check_ests(ctx, ests, nest)) return r;
+    if (!d_codes || !d_types || !d_extrema || !d_s1) return fail(ctx, -3, "null buffer");
+    if (nbins < 1 || nbins > 255 || nsteps < 1 || nsteps > 255)
+        return fail(ctx, -3, "nbins and nsteps must be in 1..255");
+    if (n_frame / nsteps == 0) return fail(ctx, -3, "fewer frames than steps");
+    if (code_stride < n_frame) return fail(ctx, -3, "code_stride < n_frame");
+    if (n_dof == 0) return 0;
+    HIPCHK(hipSetDevice(ctx->device));
+    DevBuf L, w;
+    HIPCHK(L.alloc(sizeof(double) * n_dof * nsteps));
+    HIPCHK(w.alloc(sizeof(double) * n_dof));
+    HIPCHK(hipMemsetAsync(ctx->d_flag, 0, sizeof(int), ctx->stream));
+    if (int r = run_dof1d(ctx, d_codes, code_stride, n_dof, n_frame, d_types, d_extrema,
+                          nbins, nsteps, ests, nest, jacobian, kde, d_s1,
+                          L.as<double>(), w.as<double>(), d_counts))
+        return r;
+    return check_flag(ctx);
+}
+
+// ---------------------------------------------------------------------------
+int centre_pair_plan_create(centre_ctx* ctx, uint32_t n_dof, int nbins,
+                            const uint32_t* pairs, uint64_t n_pair,
+                            centre_pair_plan** out) {
+    if (!ctx || !out) return -3;
+    *out = nullptr;
+    if (nbins < 1 || nbins > 255) return fail(ctx, -3, "nbins must be in 1..255");
+    if (n_dof < 2) return fail(ctx, -3, "need at least two DOFs");
+    HIPCHK(hipSetDevice(ctx->device));
+    auto* plan = new centre_pair_plan();
+    plan->n_dof = n_dof;
+    plan->nbins = (uint8_t)nbins;
+    const int R = dofs_per_tile(nbins);
+    plan->dofs_per_tile = R;
+    const int nblk = (int)((n_dof + R - 1) / R);
+    std::vector<int2> tiles;
+    if (!pairs) {
+        plan->all_pairs = true;
+        plan->n_canon = (uint64_t)n_dof * (n_dof - 1) / 2;
+        plan->n_rec = plan->n_canon;
+        for (int I = 0; I < nblk; ++I)
+            for (int J = I; J < nblk; ++J) tiles.push_back(make_int2(I, J));
+    } else {
+        plan->all_pairs = false;
+        plan->n_rec = n_pair;
+        std::vector<uint64_t> key(n_pair);
+        for (uint64_t p = 0; p < n_pair; ++p) {
+            uint32_t i = pairs[2 * p], j = pairs[2 * p + 1];
+            if (i >= n_dof || j >= n_dof || i == j) {
+                delete plan;
+                return fail(ctx, -3, "pair index out of range or i == j");
+            }
+            if (i > j) std::swap(i, j);
+            key[p] = ((uint64_t)i << 32) | j;
+        }
+        std::vector<uint64_t> uniq(key);
+        std::sort(uniq.begin(), uniq.end());
+        uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+        plan->n_canon = uniq.size();
+        std::vector<int> pm((size_t)n_dof * n_dof, -1);
+        std::vector<uint8_t> tmark((size_t)nblk * nblk, 0);
+        for (size_t c = 0; c < uniq.size(); ++c) {
+            const uint32_t i = (uint32_t)(uniq[c] >> 32), j = (uint32_t)uniq[c];
+            pm[(size_t)i * n_dof + j] = (int)c;
+            tmark[(size_t)(i / R) * nblk + j / R] = 1;
+        }
+        std::vector<uint64_t> r2c(n_pair);
+        for (uint64_t p = 0; p < n_pair; ++p) {
+            const uint32_t i = (uint32_t)(key[p] >> 32), j = (uint32_t)key[p];
+            r2c[p] = (uint64_t)pm[(size_t)i * n_dof + j];
+        }
+        for (int I = 0; I < nblk; ++I)
+            for (int J = I; J < nblk; ++J)
+                if (tmark[(size_t)I * nblk + J]) tiles.push_back(make_int2(I, J));
+        hipError_t e;
+        if ((e = hipMalloc(&plan->d_pairmap, sizeof(int) * pm.size())) != hipSuccess ||
+            (e = hipMemcpy(plan->d_pairmap, pm.data(), sizeof(int) * pm.size(),
+                           hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMalloc(&plan->d_rec2canon, sizeof(uint64_t) * (n_pair ? n_pair : 1))) != hipSuccess ||
+            (n_pair && (e = hipMemcpy(plan->d_rec2canon, r2c.data(), sizeof(uint64_t) * n_pair,
+                                      hipMemcpyHostToDevice)) != hipSuccess)) {
+            centre_pair_plan_destroy(plan);
+            return fail(ctx, (int)e, std::string("plan upload: ") + hipGetErrorString(e));
+        }
+    }
+    plan->n_tiles = (int)tiles.size();
+    hipError_t e = hipMalloc(&plan->d_tiles, sizeof(int2) * (tiles.size() ? tiles.size() : 1));
+    if (e == hipSuccess && !tiles.empty())
+        e = hipMemcpy(plan->d_tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        centre_pair_plan_destroy(plan);
+        return fail(ctx, (int)e, std::string("plan tiles: ") + hipGetErrorString(e));
+    }
+    *out = plan;
+    return 0;
+}
+
+void centre_pair_plan_destroy(centre_pair_plan* plan) {
+    if (!plan) return;
+    if (plan->d_tiles) (void)hipFree(plan->d_tiles);
+    if (plan->d_pairmap) (void)hipFree(plan->d_pairmap);
+    if (plan->d_rec2canon) (void)hipFree(plan->d_rec2canon);
+    delete plan;
+}
+
+uint64_t centre_pair_plan_records(const centre_pair_plan* plan) {
+    return plan ? plan->n_rec : 0;
+}
+
+int centre_pair_plan_tiles(const centre_pair_plan* plan) {
+    return plan ? plan->n_tiles : 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Canonical (i<j) DOF list of a plan, for the count-epilogue kernel.
+std::vector<uint32_t> canon_pairs(const centre_pair_plan* plan, const int* pm_host) {
+    std::vector<uint32_t> ij(2 * plan->n_canon);
+    if (plan->all_pairs) {
+        uint64_t p = 0;
+        for (uint32_t i = 0; i < plan->n_dof; ++i)
+            for (uint32_t j = i + 1; j < plan->n_dof; ++j) {
+                ij[2 * p] = i;
+                ij[2 * p + 1] = j;
+                ++p;
+            }
+    } else {
+        for (uint32_t i = 0; i < plan->n_dof; ++i)
+            for (uint32_t j = i + 1; j < plan->n_dof; ++j) {
+                const int c = pm_host[(size_t)i * plan->n_dof + j];
+                if (c >= 0) {
+                    ij[2 * (size_t)c] = i;
+                    ij[2 * (size_t)c + 1] = j;
+                }
+            }
+    }
+    return ij;
+}
+
+}  // namespace
+
+extern "C" int centre_dev_pair_entropy(centre_ctx* ctx, const centre_pair_plan* plan,
+                                       const uint8_t* d_codes, uint64_t code_stride,
+                                       uint64_t n_frame, const uint8_t* d_types,
+                                       const double* d_extrema, int nsteps,
+                                       const int* ests, int nest, int jacobian, int kde,
+                                       double* d_s1, double* d_s2, double* d_mi,
+                                       uint32_t* d_counts) {
+    if (!ctx) return -3;
+    if (!plan) return fail(ctx, -3, "null plan");
+    if (int r = check_ests(ctx, ests, nest)) return r;
+    if (!d_codes || !d_types || !d_extrema || !d_s2) return fail(ctx, -3, "null buffer");
+    if (nsteps < 1 || nsteps > 255) return fail(ctx, -3, "nsteps must be in 1..255");
+    if (n_frame / nsteps == 0) return fail(ctx, -3, "fewer frames than steps");
+    if (code_stride < n_frame) return fail(ctx, -3, "code_stride < n_frame");
+    HIPCHK(hipSetDevice(ctx->device));
+    const uint32_t D = plan->n_dof;
+    const int nb = plan->nbins;
+    const uint64_t ss = n_frame / nsteps;
+    // marginals of the same frames: S1 (all estimators), L, widths
+    DevBuf s1, L, w;
+    HIPCHK(s1.alloc(sizeof(double) * D * nest * nsteps));
+    HIPCHK(L.alloc(sizeof(double) * D * nsteps));
+    HIPCHK(w.alloc(sizeof(double) * D));
+    HIPCHK(hipMemsetAsync(ctx->d_flag, 0, sizeof(int), ctx->stream));
+    if (int r = run_dof1d(ctx, d_codes, code_stride, D, n_frame, d_types, d_extrema, nb,
+                          nsteps, ests, nest, jacobian, kde, s1.as<double>(),
+                          L.as<double>(), w.as<double>(), nullptr))
+        return r;
+    if (d_s1)
+        HIPCHK(hipMemcpyAsync(d_s1, s1.p, sizeof(double) * D * nest * nsteps,
+                              hipMemcpyDeviceToDevice, ctx->stream));
+
+    bool fused_only = true;
+    int slot_ml = -1, slot_mm = -1;
+    for (int e = 0; e < nest; ++e) {
+        if (ests[e] == 0 && slot_ml < 0) slot_ml = e;
+        else if (ests[e] == 1 && slot_mm < 0) slot_mm = e;
+        else fused_only = false;
+    }
+    const bool dump = !fused_only || d_counts != nullptr;
+    const bool all = plan->all_pairs;
+    const uint64_t P = plan->n_canon;
+
+    // outputs in canonical order (the caller's record order for all-pair plans)
+    DevBuf cs2, cmi, ccnt;
+    double* s2c = d_s2;
+    double* mic = d_mi;
+    uint32_t* cnt = d_counts;
+    if (!all) {
+        HIPCHK(cs2.alloc(sizeof(double) * P * nest * nsteps));
+        s2c = cs2.as<double>();
+        if (d_mi) {
+            HIPCHK(cmi.alloc(sizeof(double) * P * nsteps));
+            mic = cmi.as<double>();
+        }
+    }
+    if (dump && (!all || !d_counts)) {
+        const size_t bytes = sizeof(uint32_t) * P * nsteps * (size_t)nb * nb;
+        if (bytes > ((size_t)16 << 30))
+            return fail(ctx, -3, "materialised counts exceed 16 GiB; use ML/MM or fewer pairs");
+        HIPCHK(ccnt.alloc(bytes));
+        cnt = ccnt.as<uint32_t>();
+    }
+
+    PairKernelArgs a{};
+    a.codes = d_codes;
+    a.stride = code_stride;
+    a.n_dof = (int)D;
+    a.nb = nb;
+    a.R = plan->dofs_per_tile;
+    a.nsteps = nsteps;
+    a.ss = ss;
+    a.tiles = plan->d_tiles;
+    a.pairmap = plan->d_pairmap;
+    a.dof_w = w.as<double>();
+    a.dof_L = L.as<double>();
+    a.dof_S1 = s1.as<double>();
+    a.slot_ml = slot_ml;
+    a.slot_mm = slot_mm;
+    a.nest = nest;
+    a.s2 = s2c;
+    a.mi = mic;
+    a.mi_from_mm = (ests[0] == 1);
+    a.counts = cnt;
+    a.flag = ctx->d_flag;
+    // dof_S1 is [D][nest][S]; the kernel reads [D][S] of estimator 0.  With
+    // nest > 1 give it a compact copy.
+    DevBuf s1e0;
+    if (nest > 1 && d_mi) {
+        HIPCHK(s1e0.alloc(sizeof(double) * D * nsteps));
+        HIPCHK(hipMemcpy2DAsync(s1e0.p, sizeof(double) * nsteps, s1.p,
+                                sizeof(double) * nsteps * nest, sizeof(double) * nsteps, D,
+                                hipMemcpyDeviceToDevice, ctx->stream));
+        a.dof_S1 = s1e0.as<double>();
+    }
+    if (!d_mi) a.mi = nullptr;
+    if (d_mi && ests[0] > 1) a.mi = nullptr;  // MI for CS/JS computed below
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    HIPCHK(launch_pair_tiles(a, plan->n_tiles, dump, ctx->stream));
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+
+    if (!fused_only) {
+        // every estimator from the materialised counts
+        std::vector<int> pm_host;
+        if (!all) {
+            pm_host.resize((size_t)D * D);
+            HIPCHK(hipMemcpy(pm_host.data(), plan->d_pairmap, sizeof(int) * pm_host.size(),
+                             hipMemcpyDeviceToHost));
+        }
+        std::vector<uint32_t> ij = canon_pairs(plan, pm_host.data());
+        DevBuf dij, de;
+        HIPCHK(dij.alloc(sizeof(uint32_t) * ij.size()));
+        HIPCHK(de.alloc(sizeof(int) * nest));
+        HIPCHK(hipMemcpyAsync(dij.p, ij.data(), sizeof(uint32_t) * ij.size(),
+                              hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(de.p, ests, sizeof(int) * nest, hipMemcpyHostToDevice, ctx->stream));
+        CountEntropyArgs c{};
+        c.counts = cnt;
+        c.n_pair = P;
+        c.canon_ij = dij.as<uint32_t>();
+        c.nb = nb;
+        c.nsteps = nsteps;
+        c.ss = ss;
+        c.types = d_types;
+        c.ext = d_extrema;
+        c.jacobian = jacobian;
+        c.kde = kde;
+        c.ests = de.as<int>();
+        c.nest = nest;
+        c.s2 = s2c;
+        HIPCHK(launch_count_entropy(c, ctx->stream));
+        if (d_mi && ests[0] > 1) {
+            // MI = S1_i + S1_j - S2 for estimator 0 on the host side of the ABI
+            std::vector<double> hs1((size_t)D * nest * nsteps), hs2((size_t)P * nest * nsteps);
+            HIPCHK(hipMemcpyAsync(hs1.data(), s1.p, sizeof(double) * hs1.size(),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipMemcpyAsync(hs2.data(), s2c, sizeof(double) * hs2.size(),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            std::vector<double> hmi((size_t)P * nsteps);
+            for (uint64_t p = 0; p < P; ++p)
+                for (int s = 0; s < nsteps; ++s)
+                    hmi[p * nsteps + s] = hs1[((size_t)ij[2 * p] * nest) * nsteps + s] +
+                                          hs1[((size_t)ij[2 * p + 1] * nest) * nsteps + s] -
+                                          hs2[(p * nest) * nsteps + s];
+            HIPCHK(hipMemcpyAsync(mic, hmi.data(), sizeof(double) * hmi.size(),
+                                  hipMemcpyHostToDevice, ctx->stream));
+        }
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+
+    if (!all) {
+        HIPCHK(launch_gather_rec(s2c, plan->d_rec2canon, plan->n_rec, nest * nsteps, d_s2,
+                                 ctx->stream));
+        if (d_mi)
+            HIPCHK(launch_gather_rec(mic, plan->d_rec2canon, plan->n_rec, nsteps, d_mi,
+                                     ctx->stream));
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->last_pair_ms = ms;
+    return check_flag(ctx);
+}
+
+extern "C" int centre_pair_entropy(centre_ctx* ctx, const uint8_t* codes, uint32_t n_dof,
+                                   uint64_t n_frame, const uint8_t* types,
+                                   const double* extrema, int nbins, int nsteps,
+                                   const uint32_t* pairs, uint64_t n_pair, const int* ests,
+                                   int nest, int jacobian, int kde, double* s1, double* s2,
+                                   double* mi) {
+    if (!ctx) return -3;
+    if (!codes || !types || !extrema || !s2) return fail(ctx, -3, "null buffer");
+    if (int r = check_ests(ctx, ests, nest)) return r;
+    HIPCHK(hipSetDevice(ctx->device));
+    centre_pair_plan* plan = nullptr;
+    if (int r = centre_pair_plan_create(ctx, n_dof, nbins, pairs, n_pair, &plan)) return r;
+    struct PlanGuard {
+        centre_pair_plan* p;
+        ~PlanGuard() { centre_pair_plan_destroy(p); }
+    } guard{plan};
+    const uint64_t P = plan->n_rec;
+    DevBuf dc, dt, dx, ds1, ds2, dmi;
+    HIPCHK(dc.alloc((size_t)n_dof * n_frame));
+    HIPCHK(dt.alloc(n_dof));
+    HIPCHK(dx.alloc(sizeof(double) * 4 * n_dof));
+    HIPCHK(ds1.alloc(sizeof(double) * n_dof * nest * nsteps));
+    HIPCHK(ds2.alloc(sizeof(double) * P * nest * nsteps));
+    if (mi) HIPCHK(dmi.alloc(sizeof(double) * P * nsteps));
+    HIPCHK(hipMemcpyAsync(dc.p, codes, (size_t)n_dof * n_frame, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dt.p, types, n_dof, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dx.p, extrema, sizeof(double) * 4 * n_dof, hipMemcpyHostToDevice, ctx->stream));
+    if (int r = centre_dev_pair_entropy(ctx, plan, dc.as<uint8_t>(), n_frame, n_frame,
+                                        dt.as<uint8_t>(), dx.as<double>(), nsteps, ests, nest,
+                                        jacobian, kde, ds1.as<double>(), ds2.as<double>(),
+                                        mi ? dmi.as<double>() : nullptr, nullptr))
+        return r;
+    if (s1)
+        HIPCHK(hipMemcpyAsync(s1, ds1.p, sizeof(double) * n_dof * nest * nsteps,
+                              hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(s2, ds2.p, sizeof(double) * P * nest * nsteps, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    if (mi)
+        HIPCHK(hipMemcpyAsync(mi, dmi.p, sizeof(double) * P * nsteps, hipMemcpyDeviceToHost,
+                              ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Modified Bessel functions I0, I1 (Abramowitz & Stegun 9.8.1-9.8.4
+// polynomial approximations) and In, n >= 2, by Miller's downward recurrence
+// normalised with I0 -- the scheme Utils::modifiedBesselI uses
+// (Utils.cpp:60-168), including its iteration count 2*(n + int(sqrt(40n))).
+static double bessel_i0(double x) {
+    if (fabs(x) < 3.75) {
+        double y = x / 3.75;
+        y = y * y;
+        const double t = 1.2067429 + y * (0.2659732 + y * (0.360768e-1 + y * 0.45813e-2));
+        return 1.0 + y * (3.5156229 + y * (3.0899424 + y * t));
+    }
+    const double ax = fabs(x), y = 3.75 / ax;
+    const double bx = exp(ax) / sqrt(ax);
+    const double t = -0.2057706e-1 + y * (0.2635537e-1 + y * (-0.1647633e-1 + y * 0.392377e-2));
+    const double p = 0.39894228 + y * (0.1328592e-1 + y * (0.225319e-2 +
+                     y * (-0.157565e-2 + y * (0.916281e-2 + y * t))));
+    return p * bx;
+}
+
+static double bessel_i1(double x) {
+    if (fabs(x) < 3.75) {
+        double y = x / 3.75;
+        y = y * y;
+        const double t = 0.15084934 + y * (0.2658733e-1 + y * (0.301532e-2 + y * 0.32411e-3));
+        return x * (0.5 + y * (0.87890594 + y * (0.51498869 + y * t)));
+    }
+    const double ax = fabs(x), y = 3.75 / ax;
+    const double bx = exp(ax) / sqrt(ax);
+    const double t = 0.2282967e-1 + y * (-0.2895312e-1 + y * (0.1787654e-1 + y * -0.420059e-2));
+    const double p = 0.39894228 + y * (-0.3988024e-1 + y * (-0.362018e-2 +
+                     y * (0.163801e-2 + y * (-0.1031555e-1 + y * t))));
+    return p * bx;
+}
+
+double centre_bessel_i(int n, double x) {
+    if (n == 0) return bessel_i0(x);
+    if (n == 1) return bessel_i1(x);
+    if (fabs(x) < 1.e-10) return 0.0;
+    const double tox = 2.0 / x;
+    double bip = 0.0, bi = 1.0, res = 0.0;
+    const int iters = 2 * (n + (int)sqrtf((float)(40 * n)));
+    for (int i = iters; i >= 1; --i) {
+        const double bim = bip + (double)(i * tox * bi);
+        bip = bi;
+        bi = bim;
+        if (fabs(bi) > 1.e10) {
+            bi *= 1.e-10;
+            bip *= 1.e-10;
+            res *= 1.e-10;
+        }
+        if (i == n) res = bip;
+    }
+    return res * bessel_i0(x) / bi;
+}
+
+// Kernel bandwidth of getMinimaPositions (Real2Int.cpp:509-514).
+double centre_vm_bandwidth(uint64_t n, double kappa) {
+    const double i0 = centre_bessel_i(0, kappa);
+    return pow((3 * (double)n * (kappa * kappa) * centre_bessel_i(2, 2 * kappa) /
+                (4 * 1.7724538509055159 * pow(i0, 2.0))),
+               (2.0 / 5.0));
+}
+
+extern "C" int centre_dev_discretize_vmkde(centre_ctx* ctx, const double* d_bat, uint64_t ld,
+                                           uint32_t n_dof, uint64_t n_frame, int nmaxconf,
+                                           double kappa, uint32_t sdo_step, double sdo_conv,
+                                           uint32_t sdo_iter, uint32_t grid_intervals,
+                                           uint8_t* d_codes, double* d_minima,
+                                           uint8_t* d_nfound, double* d_grid) {
+    if (!ctx) return -3;
+    if (!d_bat || !d_codes || !d_minima || !d_nfound) return fail(ctx, -3, "null buffer");
+    if (nmaxconf < 1 || nmaxconf > 9) return fail(ctx, -3, "nmaxconf must be in 1..9");
+    if (grid_intervals < 2 || grid_intervals > 3600)
+        return fail(ctx, -3, "grid_intervals must be in 2..3600");
+    if (ld < n_dof || n_frame == 0) return fail(ctx, -3, "ld < n_dof or no frames");
+    if (n_dof == 0) return 0;
+    HIPCHK(hipSetDevice(ctx->device));
+    DevBuf grid, xt, trig;
+    double* g = d_grid;
+    if (!g) {
+        HIPCHK(grid.alloc(sizeof(double) * 2 * (grid_intervals + 1) * (size_t)n_dof));
+        g = grid.as<double>();
+    }
+    HIPCHK(xt.alloc(sizeof(float) * n_frame * (size_t)n_dof));
+    HIPCHK(trig.alloc(sizeof(double) * 2 * n_frame * (size_t)n_dof));
+    VmkdeArgs a{};
+    a.bat = d_bat;
+    a.ld = ld;
+    a.n_dof = n_dof;
+    a.n_frame = n_frame;
+    a.kappa = kappa;
+    a.nmaxconf = nmaxconf;
+    a.sdo_step = sdo_step;
+    a.sdo_conv = sdo_conv;
+    a.sdo_iter = sdo_iter;
+    a.grid = (int)grid_intervals;
+    a.gridbuf = g;
+    a.minima = d_minima;
+    a.nfound = d_nfound;
+    a.codes = d_codes;
+    a.xt = xt.as<float>();
+    a.trig = trig.as<double>();
+    HIPCHK(run_vmkde(a, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+extern "C" int centre_discretize_vmkde(centre_ctx* ctx, const double* bat, uint64_t ld,
+                                       uint32_t n_dof, uint64_t n_frame, int nmaxconf,
+                                       double kappa, uint32_t sdo_step, double sdo_conv,
+                                       uint32_t sdo_iter, uint32_t grid_intervals,
+                                       uint8_t* codes, double* minima, uint8_t* n_found,
+                                       double* grid) {
+    if (!ctx) return -3;
+    if (!bat || !codes || !minima || !n_found) return fail(ctx, -3, "null buffer");
+    HIPCHK(hipSetDevice(ctx->device));
+    DevBuf b, c, m, nf, gr;
+    HIPCHK(b.alloc(sizeof(double) * ld * n_frame));
+    HIPCHK(c.alloc((size_t)n_dof * n_frame));
+    HIPCHK(m.alloc(sizeof(double) * 9 * n_dof));
+    HIPCHK(nf.alloc(n_dof));
+    const size_t gbytes = sizeof(double) * 2 * (grid_intervals + 1) * (size_t)n_dof;
+    if (grid) HIPCHK(gr.alloc(gbytes));
+    HIPCHK(hipMemcpyAsync(b.p, bat, sizeof(double) * ld * n_frame, hipMemcpyHostToDevice,
+                          ctx->stream));
+    int r = centre_dev_discretize_vmkde(ctx, b.as<double>(), ld, n_dof, n_frame, nmaxconf, kappa,
+                                        sdo_step, sdo_conv, sdo_iter, grid_intervals,
+                                        c.as<uint8_t>(), m.as<double>(), nf.as<uint8_t>(),
+                                        grid ? gr.as<double>() : nullptr);
+    if (r) return r;
+    HIPCHK(hipMemcpyAsync(codes, c.p, (size_t)n_dof * n_frame, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(minima, m.p, sizeof(double) * 9 * n_dof, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(n_found, nf.p, n_dof, hipMemcpyDeviceToHost, ctx->stream));
+    if (grid) HIPCHK(hipMemcpyAsync(grid, gr.p, gbytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+extern "C" int centre_dev_mst(centre_ctx* ctx, const uint32_t* d_src, const uint32_t* d_dst,
+                              const double* d_w, uint64_t n_edge, uint32_t n_node,
+                              uint32_t* tree_edge_idx, uint32_t* n_tree) {
+    if (!ctx) return -3;
+    if (!n_tree || (!tree_edge_idx && n_node > 1)) return fail(ctx, -3, "null output");
+    if (n_edge && (!d_src || !d_dst || !d_w)) return fail(ctx, -3, "null edge buffer");
+    if (n_edge >= 0xFFFFFFFFull) return fail(ctx, -3, "too many edges (max 2^32-2)");
+    *n_tree = 0;
+    if (n_node == 0) return 0;
+    HIPCHK(hipSetDevice(ctx->device));
+    // endpoint range check on the host side of the boundary
+    {
+        std::vector<uint32_t> hs(n_edge), hd(n_edge);
+        if (n_edge) {
+            HIPCHK(hipMemcpy(hs.data(), d_src, sizeof(uint32_t) * n_edge, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(hd.data(), d_dst, sizeof(uint32_t) * n_edge, hipMemcpyDeviceToHost));
+        }
+        for (uint64_t e = 0; e < n_edge; ++e)
+            if (hs[e] >= n_node || hd[e] >= n_node) return fail(ctx, -3, "edge endpoint out of range");
+    }
+    DevBuf comp, bw, be, it, ch;
+    HIPCHK(comp.alloc(sizeof(uint32_t) * n_node));
+    HIPCHK(bw.alloc(sizeof(unsigned long long) * n_node));
+    HIPCHK(be.alloc(sizeof(unsigned int) * n_node));
+    HIPCHK(it.alloc(n_edge ? n_edge : 1));
+    HIPCHK(ch.alloc(sizeof(int)));
+    MstArgs a{};
+    a.src = d_src;
+    a.dst = d_dst;
+    a.w = d_w;
+    a.n_edge = n_edge;
+    a.n_node = n_node;
+    a.comp = comp.as<uint32_t>();
+    a.best_w = bw.as<unsigned long long>();
+    a.best_e = be.as<unsigned int>();
+    a.in_tree = it.as<uint8_t>();
+    a.changed = ch.as<int>();
+    HIPCHK(mst_init(a, ctx->stream));
+    for (uint32_t round = 0; round < 64; ++round) {
+        HIPCHK(run_mst_round(a, ctx->stream));
+        int changed = 0;
+        HIPCHK(hipMemcpyAsync(&changed, a.changed, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (!changed) break;
+    }
+    std::vector<uint8_t> flags(n_edge);
+    if (n_edge)
+        HIPCHK(hipMemcpy(flags.data(), a.in_tree, n_edge, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> idx;
+    for (uint64_t e = 0; e < n_edge; ++e)
+        if (flags[e]) idx.push_back((uint32_t)e);
+    if (idx.size() >= n_node) return fail(ctx, -3, "internal: forest larger than n_node - 1");
+    std::vector<double> wt(idx.size());
+    for (size_t k = 0; k < idx.size(); ++k)
+        HIPCHK(hipMemcpy(&wt[k], d_w + idx[k], sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<size_t> ord(idx.size());
+    for (size_t k = 0; k < ord.size(); ++k) ord[k] = k;
+    std::sort(ord.begin(), ord.end(), [&](size_t x, size_t y) {
+        if (wt[x] != wt[y]) return wt[x] > wt[y];
+        return idx[x] < idx[y];
+    });
+    for (size_t k = 0; k < ord.size(); ++k) tree_edge_idx[k] = idx[ord[k]];
+    *n_tree = (uint32_t)idx.size();
+    return 0;
+}
+
+extern "C" int centre_mst(centre_ctx* ctx, const uint32_t* src, const uint32_t* dst,
+                          const double* w, uint64_t n_edge, uint32_t n_node,
+                          uint32_t* tree_edge_idx, uint32_t* n_tree) {
+    if (!ctx) return -3;
+    if (n_edge && (!src || !dst || !w)) return fail(ctx, -3, "null edge buffer");
+    HIPCHK(hipSetDevice(ctx->device));
+    DevBuf s, d, ww;
+    HIPCHK(s.alloc(sizeof(uint32_t) * n_edge));
+    HIPCHK(d.alloc(sizeof(uint32_t) * n_edge));
+    HIPCHK(ww.alloc(sizeof(double) * n_edge));
+    if (n_edge) {
+        HIPCHK(hipMemcpy(s.p, src, sizeof(uint32_t) * n_edge, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d.p, dst, sizeof(uint32_t) * n_edge, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ww.p, w, sizeof(double) * n_edge, hipMemcpyHostToDevice));
+    }
+    return centre_dev_mst(ctx, s.as<uint32_t>(), d.as<uint32_t>(), ww.as<double>(), n_edge,
+                          n_node, tree_edge_idx, n_tree);
+}
+
+extern "C" int centre_dev_segment_sum(centre_ctx* ctx, const double* d_x,
+                                      const uint64_t* seg_begin, int n_seg, double* out) {
+    if (!ctx) return -3;
+    if (n_seg <= 0) return 0;
+    if (!d_x || !seg_begin || !out) return fail(ctx, -3, "null buffer");
+    for (int s = 0; s < n_seg; ++s)
+        if (seg_begin[s + 1] < seg_begin[s]) return fail(ctx, -3, "segments must be ordered");
+    HIPCHK(hipSetDevice(ctx->device));
+    DevBuf b, o;
+    HIPCHK(b.alloc(sizeof(uint64_t) * (n_seg + 1)));
+    HIPCHK(o.alloc(sizeof(double) * n_seg));
+    HIPCHK(hipMemcpyAsync(b.p, seg_begin, sizeof(uint64_t) * (n_seg + 1), hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPCHK(launch_segment_sum(d_x, b.as<uint64_t>(), n_seg, o.as<double>(), ctx->stream));
+    HIPCHK(hipMemcpyAsync(out, o.p, sizeof(double) * n_seg, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}

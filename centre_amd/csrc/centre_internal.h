@@ -1,0 +1,179 @@
+// Internal declarations shared by the HIP kernels and the C-ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/centre_hip.h"
+
+struct centre_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int* d_flag = nullptr;          // device error word (code validation etc.)
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_pair_ms = 0.f;       // duration of the last pair-kernel launch
+};
+
+struct centre_pair_plan {
+    uint32_t n_dof = 0;
+    uint8_t nbins = 0;
+    int dofs_per_tile = 0;          // R: DOFs per tile side
+    int n_tiles = 0;
+    int2* d_tiles = nullptr;        // (row block, col block)
+    int* d_pairmap = nullptr;       // [D][D] canonical index or -1; null = all i<j
+    uint64_t n_canon = 0;           // canonical (i<j) pairs computed
+    uint64_t n_rec = 0;             // records requested by the caller
+    uint64_t* d_rec2canon = nullptr;  // [n_rec] (explicit lists only)
+    bool all_pairs = true;
+};
+
+// Bins of one-hot rows per tile side (a multiple of 32, the MFMA block).
+constexpr int kTileRows = 256;
+// Frames per LDS chunk (the K extent staged per barrier).
+constexpr int kChunk = 64;
+// Padded LDS row stride (bytes) of the one-hot tiles: 16 extra bytes spread
+// the 16-byte fragment reads of one row block over all banks.
+constexpr int kLdsRow = kChunk + 16;
+
+// DOFs per tile side; capped at 60 so that the per-pair LDS slots of very
+// small bin counts still fit next to the one-hot buffers (160 KB LDS).
+inline int dofs_per_tile(int nb) {
+    int r = kTileRows / nb;
+    return r > 60 ? 60 : r;
+}
+
+struct PairKernelArgs {
+    const uint8_t* codes;   // [D] rows of `stride` bytes; frame f at +f
+    uint64_t stride;
+    int n_dof;
+    int nb;
+    int R;
+    int nsteps;
+    uint64_t ss;            // frames per convergence step
+    const int2* tiles;
+    const int* pairmap;     // null => all i<j, canonical index by triangle
+    const double* dof_w;    // [D] bin width (1.0 under kde)
+    const double* dof_L;    // [D][S] sum_b m_b(s) log J_b (0 if no Jacobian)
+    const double* dof_S1;   // [D][S] S1 of the MI estimator, or null
+    int slot_ml;            // output slot of ML in s2 (-1 = not requested)
+    int slot_mm;            // output slot of MM (-1 = not requested)
+    int nest;               // estimators in the s2 layout
+    double* s2;             // [P][nest][S]
+    double* mi;             // [P][S] or null
+    int mi_from_mm;         // MI uses the MM S2 (when MM is the first estimator)
+    uint32_t* counts;       // dump mode: [P][S][nb*nb] uint32 prefix counts
+    int* flag;              // error word: bit0 = code >= nbins
+};
+
+// Launchers (return hipError_t of the launch).
+hipError_t launch_pair_tiles(const PairKernelArgs& a, int n_tiles, bool dump,
+                             hipStream_t st);
+size_t pair_tile_lds_bytes(int R);
+
+struct Dof1dArgs {
+    const uint8_t* codes;   // [D] rows of `stride` bytes
+    uint64_t stride;
+    int n_dof;
+    int nb;
+    int nsteps;
+    uint64_t ss;
+    const uint8_t* types;   // [D] 1 bond, 2 angle, 3 torsion
+    const double* ext;      // [D][4] (min, max, ...)
+    int jacobian, kde;
+    const int* ests;        // [nest] estimator ids
+    int nest;
+    double* s1;             // [D][nest][S]
+    double* L;              // [D][S] sum_b m_b log J_b (for the pair epilogue)
+    double* w;              // [D] bin width used by the pair epilogue
+    uint64_t* counts;       // optional [D][S][nb] prefix counts
+    int* flag;
+};
+hipError_t launch_dof1d(const Dof1dArgs& a, hipStream_t st);
+
+// estimator epilogue on materialised counts (CS / JS / any estimator)
+struct CountEntropyArgs {
+    const uint32_t* counts;   // [P][S][nb*nb]
+    uint64_t n_pair;
+    const uint32_t* canon_ij; // [P][2] DOF indices
+    int nb, nsteps;
+    uint64_t ss;
+    const uint8_t* types;
+    const double* ext;
+    int jacobian, kde;
+    const int* ests;
+    int nest;
+    double* s2;               // [P][nest][S]
+};
+hipError_t launch_count_entropy(const CountEntropyArgs& a, hipStream_t st);
+
+// ---- discretizer -------------------------------------------------------------
+struct DiscArgs {
+    const double* bat;      // [F][ld] float64, DOF column j at bat[f*ld + j]
+    uint64_t ld;
+    uint32_t n_dof;
+    uint64_t n_frame;
+    int dtype;              // 1 bond, 2 angle, 3 torsion
+    const uint8_t* schemes; // device [nsch]
+    int nsch;
+    int nbins_ref;
+    int optimize;
+    uint8_t* codes;         // [D][nsch][F]
+    double* ext;            // [D][4]
+    // scratch
+    double* part;           // [nchunk][D][8] partial min/max/sum
+    uint32_t* gaphist;      // [D][180]
+    double* shift;          // [D]
+    int* flag;
+};
+hipError_t run_discretize_hist(const DiscArgs& a, int nchunk, hipStream_t st);
+int disc_chunks(uint64_t n_frame);
+
+// ---- vmKDE -------------------------------------------------------------------
+struct VmkdeArgs {
+    const double* bat;      // [F][ld]
+    uint64_t ld;
+    uint32_t n_dof;
+    uint64_t n_frame;
+    double kappa;
+    int nmaxconf;
+    uint32_t sdo_step;
+    double sdo_conv;
+    uint32_t sdo_iter;
+    int grid;               // grid intervals G (360 = reference)
+    double* gridbuf;        // [D][2][G+1] gradient, second derivative
+    double* minima;         // [D][9] radians
+    uint8_t* nfound;        // [D]
+    uint8_t* codes;         // [D][F]
+    float* xt;              // scratch [D][F] float32 transposed columns
+    double* trig;           // scratch [D][F][2] cos/sin of frames
+};
+hipError_t run_vmkde(const VmkdeArgs& a, hipStream_t st);
+// Host FP64 helpers shared with the vmKDE launcher (Utils.cpp:60-168).
+double centre_bessel_i(int n, double x);
+double centre_vm_bandwidth(uint64_t n, double kappa);
+
+// ---- MST -----------------------------------------------------------------------
+struct MstArgs {
+    const uint32_t* src;
+    const uint32_t* dst;
+    const double* w;
+    uint64_t n_edge;
+    uint32_t n_node;
+    uint32_t* comp;         // [V]
+    unsigned long long* best_w;   // [V] ordered weight key
+    unsigned int* best_e;   // [V]
+    uint8_t* in_tree;       // [E]
+    int* changed;           // device word
+};
+hipError_t run_mst_round(const MstArgs& a, hipStream_t st);
+hipError_t mst_init(const MstArgs& a, hipStream_t st);
+
+// ---- reductions ------------------------------------------------------------------
+hipError_t launch_segment_sum(const double* x, const uint64_t* d_beg, int n_seg,
+                              double* d_out, hipStream_t st);
+hipError_t launch_gather_rec(const double* src, const uint64_t* rec2canon,
+                             uint64_t n_rec, int width, double* dst,
+                             hipStream_t st);

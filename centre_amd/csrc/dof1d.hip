@@ -1,0 +1,221 @@
+// Marginal stage and the estimator epilogue on materialised counts.
+//
+// dof1d_kernel replaces binData1D + entropy1D (Estimators.cpp:56-118,
+// 217-389) for all DOFs at once: one workgroup per DOF histograms each step
+// chunk in LDS, keeps the prefix counts, and one lane evaluates every
+// requested estimator in the reference's loop order so S1 matches the
+// source's arithmetic term for term.  It also leaves, per DOF and step, the
+// marginal Jacobian sum L = sum_b m_b log J_b and the bin width that the
+// pair epilogue needs.
+//
+// count_entropy_kernel evaluates entropy2D (Estimators.cpp:391-651) for any
+// estimator from prefix counts written by the pair kernel's dump mode: one
+// thread per (pair, step, estimator), in the reference's row/column order,
+// including the Chao-Shen second Jacobian evaluated at mid2[j]
+// (Estimators.cpp:558,562) and the James-Stein target 1/nbins
+// (Estimators.cpp:510).
+#include "centre_internal.h"
+
+namespace {
+
+constexpr int kML = 0, kMM = 1, kCS = 2, kJS = 3;
+
+__device__ inline double jac(int type, double mid) {
+    if (type == 1) return mid * mid;        // pow(mid, 2)
+    if (type == 2) return sin(mid);
+    return 1.0;
+}
+
+// entropy1D for one step and one estimator.  fr: scratch [nb].
+__device__ double ent1d(int type, int est, int nb, const uint64_t* f, double n,
+                        double w, const double* mid, int jacobian, double* fr) {
+    for (int j = 0; j < nb; ++j) fr[j] = f[j] / n;
+    double sum = 0.0;
+    uint8_t blank = 0, single = 0;
+    if (est == kMM) {
+        for (int j = 0; j < nb; ++j)
+            if (f[j] == 0) ++blank;
+    } else if (est == kCS) {
+        for (int j = 0; j < nb; ++j)
+            if (f[j] == 1) ++single;
+        for (int j = 0; j < nb; ++j) fr[j] = (1 - (single / n)) * (f[j] / n);
+    } else if (est == kJS) {
+        const double tk = 1.0 / (double)nb;
+        double s2 = 0, tks = 0;
+        for (int j = 0; j < nb; ++j) {
+            s2 += fr[j] * fr[j];
+            tks += ((tk - fr[j]) * (tk - fr[j]));
+        }
+        const double lam = (1 - s2) / ((n - 1) * tks);
+        for (int j = 0; j < nb; ++j) fr[j] = (lam * tk) + ((1 - lam) * fr[j]);
+    }
+    const bool hasj = (type == 1 || type == 2);
+    for (int j = 0; j < nb; ++j) {
+        double pd = fr[j] / w;
+        if (pd > 0.0) {
+            const double denom = (est == kCS) ? 1 - pow((1 - fr[j]), n) : 1.0;
+            if (jacobian && hasj) {
+                const double jf = jac(type, mid[j]);
+                pd = pd / jf;
+                sum += (est == kCS) ? (jf * pd * log(pd) / denom) : (jf * pd * log(pd));
+            } else {
+                sum += (est == kCS) ? (pd * log(pd) / denom) : (pd * log(pd));
+            }
+        }
+    }
+    sum *= -w;
+    if (est == kMM) sum += ((nb - blank) - 1) / (2.0 * n);
+    return sum;
+}
+
+__global__ __launch_bounds__(256) void dof1d_kernel(Dof1dArgs a) {
+    __shared__ uint32_t h[256];
+    __shared__ uint64_t cum[256];
+    __shared__ double fr[256];
+    __shared__ double mid[256];
+    const int d = blockIdx.x, tid = threadIdx.x;
+    const int nb = a.nb, S = a.nsteps;
+    const uint8_t* row = a.codes + (uint64_t)d * a.stride;
+    const int type = a.types[d];
+    const double mn = a.ext[4 * d + 0], mx = a.ext[4 * d + 1];
+    const bool hasj = (type == 1 || type == 2);
+    double w = (mx - mn) / nb;
+    const double hw = w / 2;
+    if (a.kde) w = 1.0;
+    for (int b = tid; b < 256; b += 256) {
+        cum[b] = 0;
+        mid[b] = (hasj && b < nb) ? mn + b * w + hw : 0.0;
+    }
+    for (int s = 0; s < S; ++s) {
+        for (int b = tid; b < 256; b += 256) h[b] = 0;
+        __syncthreads();
+        const uint64_t f0 = (uint64_t)s * a.ss, f1 = f0 + a.ss;
+        for (uint64_t f = f0 + tid; f < f1; f += 256) {
+            const uint32_t c = row[f];
+            if (c < (uint32_t)nb) atomicAdd(&h[c], 1u);
+            else atomicOr(a.flag, 1);
+        }
+        __syncthreads();
+        if (tid < nb) {
+            cum[tid] += h[tid];
+            if (a.counts) a.counts[((uint64_t)d * S + s) * nb + tid] = cum[tid];
+        }
+        __syncthreads();
+        if (tid == 0) {
+            const double n = (s + 1.0) * a.ss;
+            for (int e = 0; e < a.nest; ++e)
+                a.s1[((uint64_t)d * a.nest + e) * S + s] =
+                    ent1d(type, a.ests[e], nb, cum, n, w, mid, a.jacobian, fr);
+            double L = 0.0;
+            if (a.jacobian && hasj)
+                for (int b = 0; b < nb; ++b)
+                    if (cum[b]) L += (double)cum[b] * log(jac(type, mid[b]));
+            a.L[(uint64_t)d * S + s] = L;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) a.w[d] = w;
+}
+
+__global__ __launch_bounds__(64) void count_entropy_kernel(CountEntropyArgs a) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int S = a.nsteps, nest = a.nest, nb = a.nb;
+    if (t >= a.n_pair * S * nest) return;
+    const int e = (int)(t % nest);
+    const int s = (int)((t / nest) % S);
+    const uint64_t p = t / ((uint64_t)nest * S);
+    const int est = a.ests[e];
+    const uint32_t i = a.canon_ij[2 * p], j = a.canon_ij[2 * p + 1];
+    const int t1 = a.types[i], t2 = a.types[j];
+    const double mn1 = a.ext[4 * i], mx1 = a.ext[4 * i + 1];
+    const double mn2 = a.ext[4 * j], mx2 = a.ext[4 * j + 1];
+    double w1 = (mx1 - mn1) / nb, w2 = (mx2 - mn2) / nb;
+    const double hw1 = w1 / 2, hw2 = w2 / 2;
+    if (a.kde) { w1 = 1.0; w2 = 1.0; }
+    const bool hj1 = (t1 == 1 || t1 == 2), hj2 = (t2 == 1 || t2 == 2);
+    const double area = w1 * w2;
+    const double n = (s + 1.0) * a.ss;
+    const uint32_t* f = a.counts + ((size_t)p * S + s) * nb * nb;
+    const unsigned nb2 = (unsigned)nb * nb;
+    unsigned blank = 0, single = 0;
+    double lam = 0.0, tk = 1.0 / (double)nb;
+    if (est == kMM) {
+        for (unsigned q = 0; q < nb2; ++q) if (f[q] == 0) ++blank;
+    } else if (est == kCS) {
+        for (unsigned q = 0; q < nb2; ++q) if (f[q] == 1) ++single;
+    } else if (est == kJS) {
+        double s2 = 0, tks = 0;
+        for (unsigned q = 0; q < nb2; ++q) {
+            const double fq = f[q] / n;
+            s2 += fq * fq;
+            tks += ((tk - fq) * (tk - fq));
+        }
+        lam = (1 - s2) / ((n - 1) * tks);
+    }
+    double sum = 0.0, jf1 = 1.0, jf2 = 1.0;
+    for (int r = 0; r < nb; ++r) {
+        if (hj1) jf1 = jac(t1, mn1 + (r * w1) + hw1);
+        double row = 0.0;
+        for (int c = 0; c < nb; ++c) {
+            double q = f[r * nb + c] / n;
+            if (est == kCS) q = (1 - (single / n)) * (f[r * nb + c] / n);
+            else if (est == kJS) q = (lam * tk) + ((1 - lam) * q);
+            double pd = q / area;
+            if (pd > 0.0) {
+                double denom = 1.0;
+                if (est == kCS) {
+                    denom = 1 - pow((1 - q), n);
+                    if (hj2) jf2 = jac(t2, mn2 + (r * w2) + hw2);
+                } else if (hj2) {
+                    jf2 = jac(t2, mn2 + (c * w2) + hw2);
+                }
+                if (a.jacobian) {
+                    const double jf = jf1 * jf2;
+                    pd = pd / jf;
+                    row += (est == kCS) ? (jf * pd * log(pd) / denom) : (jf * pd * log(pd));
+                } else {
+                    row += (est == kCS) ? (pd * log(pd) / denom) : (pd * log(pd));
+                }
+            }
+        }
+        row *= -area;
+        sum += row;
+    }
+    if (est == kMM) sum += ((nb2 - blank) - 1) / (2.0 * n);
+    a.s2[((size_t)p * nest + e) * S + s] = sum;
+}
+
+__global__ void gather_rec_kernel(const double* src, const uint64_t* rec2canon,
+                                  uint64_t n_rec, int width, double* dst) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_rec * width) return;
+    const uint64_t r = t / width;
+    const int k = (int)(t - r * width);
+    dst[t] = src[rec2canon[r] * width + k];
+}
+
+}  // namespace
+
+hipError_t launch_dof1d(const Dof1dArgs& a, hipStream_t st) {
+    if (a.n_dof == 0) return hipSuccess;
+    hipLaunchKernelGGL(dof1d_kernel, dim3(a.n_dof), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_count_entropy(const CountEntropyArgs& a, hipStream_t st) {
+    const uint64_t n = a.n_pair * a.nsteps * a.nest;
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(count_entropy_kernel, dim3((unsigned)((n + 63) / 64)),
+                       dim3(64), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_rec(const double* src, const uint64_t* rec2canon,
+                             uint64_t n_rec, int width, double* dst,
+                             hipStream_t st) {
+    const uint64_t n = n_rec * width;
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(gather_rec_kernel, dim3((unsigned)((n + 255) / 256)),
+                       dim3(256), 0, st, src, rec2canon, n_rec, width, dst);
+    return hipGetLastError();
+}

@@ -1,0 +1,349 @@
+// Pair stage: joint counts as one-hot int8 MFMA contractions over frames,
+// with the plogp entropy epilogue fused at every convergence-step boundary.
+//
+// What it replaces: binData2D + entropy2D (Estimators.cpp:130-205, 391-651),
+// called once per DOF pair by EntropyCalculator::run2d_xx / run2d_xy
+// (EntropyCalculator.cpp:530-564).
+//
+// Formulation.  For the DOFs of one tile, the one-hot matrix X has one row
+// per (DOF, bin) and one column per frame: X[d*nb + b][f] = (code[d][f] == b).
+// The joint counts of every DOF pair of the tile are the blocks of X_r X_c^T,
+// an int8 GEMM with exact int32 accumulation (v_mfma_i32_32x32x32_i8).
+// A tile is 256 one-hot rows x 256 one-hot columns: R = floor(256/nb) whole
+// DOFs per side (5 DOFs = 250 rows for nb = 50), so every pair lies inside
+// one tile and its counts are never written to memory.
+//
+// The frame axis is walked in chunks of kChunk frames aligned to the step
+// boundaries (step s = frames [s*ss, (s+1)*ss), remainder frames dropped as
+// Estimators.cpp:162 does).  Accumulators are never reset, so at the end of
+// step s they hold the prefix counts binData2D produces after its prefix sum
+// (Estimators.cpp:183-188).  The epilogue then adds c*log(c) over each pair's
+// bins and writes, per pair and step,
+//     S2 = log(n) + log(w_i w_j) - T/n + (L_i + L_j)/n,   T = sum c log c,
+// which is entropy2D's ML sum -sum (c/n) log(c / (n w_i w_j J_i J_j)) with
+// the Jacobian part folded into the marginal sums L_i = sum_b m_ib log J_ib
+// (m = 1-D counts of the same frames).  MM adds (nonzero bins - 1)/(2n).
+//
+// One-hot staging.  Codes of the tile's DOFs are loaded per chunk and the
+// one-hot rows are written into LDS by setting one byte per (DOF, frame) and
+// clearing the byte the same slot set two chunks earlier, so a chunk costs a
+// few byte writes per thread instead of rebuilding 32 KB of zeros.  Two
+// buffers alternate: chunk g+1 is staged while chunk g feeds the MFMAs.
+//
+// Waves: 4 per workgroup in a 2x2 layout, each owning a 128x128 quadrant
+// (4x4 MFMA 32x32 blocks = 256 int32 accumulators per lane).  Blocks with no
+// wanted pair (padding rows/cols, the lower triangle of diagonal tiles) are
+// skipped by a wave-uniform mask.
+#include "centre_internal.h"
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+namespace {
+
+constexpr int kOpBytes = kTileRows * kLdsRow;   // one operand, one buffer
+constexpr int kGroups = kChunk / 4;             // dword groups per DOF row
+
+struct Smem {
+    uint8_t* oh;        // [2 buf][2 operand][kTileRows * kLdsRow]
+    uint32_t* cl;       // [2 buf][2R DOFs][kGroups] codes staged per slot
+    uint8_t* row2dof;   // [256] local row DOF (0xFF = padding)
+    uint8_t* col2dof;   // [256]
+    double* slotT;      // [R*R] sum c log c per local pair
+    int* slotN;         // [R*R] nonzero bins per local pair
+    int* park;          // [4 waves][64 lanes][16] epilogue staging
+};
+
+__host__ __device__ inline size_t lds_layout(int R, size_t* off_cl,
+                                             size_t* off_map, size_t* off_T,
+                                             size_t* off_N, size_t* off_park) {
+    size_t off = (size_t)2 * 2 * kOpBytes;
+    *off_park = off;
+    off += (size_t)4 * 64 * 16 * 4;
+    *off_cl = off;
+    off += (size_t)2 * 2 * R * kGroups * 4;
+    *off_map = off;
+    off += 512;
+    off = (off + 15) & ~(size_t)15;
+    *off_T = off;
+    off += sizeof(double) * R * R;
+    *off_N = off;
+    off += sizeof(int) * R * R;
+    return off;
+}
+
+__device__ inline long long tri_index(long long i, long long j, long long D) {
+    return i * D - i * (i + 1) / 2 + (j - i - 1);
+}
+
+template <bool kDump>
+__global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
+    extern __shared__ __align__(16) uint8_t smem_raw[];
+    const int R = a.R, nb = a.nb, S = a.nsteps;
+    size_t ocl, omap, oT, oN, opark;
+    lds_layout(R, &ocl, &omap, &oT, &oN, &opark);
+    Smem sm;
+    sm.oh = smem_raw;
+    sm.cl = reinterpret_cast<uint32_t*>(smem_raw + ocl);
+    sm.row2dof = smem_raw + omap;
+    sm.col2dof = smem_raw + omap + 256;
+    sm.slotT = reinterpret_cast<double*>(smem_raw + oT);
+    sm.slotN = reinterpret_cast<int*>(smem_raw + oN);
+    sm.park = reinterpret_cast<int*>(smem_raw + opark);
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int2 tile = a.tiles[blockIdx.x];
+    const int r0 = tile.x * R, c0 = tile.y * R;
+    const int nr = min(R, a.n_dof - r0), nc = min(R, a.n_dof - c0);
+    const bool diag = tile.x == tile.y;
+    const int ngen = diag ? nr : nr + nc;   // diagonal tiles share operands
+    const long long D = a.n_dof;
+
+    // ---- LDS initialisation ------------------------------------------------
+    {
+        uint4* p = reinterpret_cast<uint4*>(sm.oh);
+        const uint4 z = make_uint4(0, 0, 0, 0);
+        for (int i = tid; i < 2 * 2 * kOpBytes / 16; i += 256) p[i] = z;
+        for (int i = tid; i < 2 * 2 * R * kGroups; i += 256) sm.cl[i] = 0xFFFFFFFFu;
+        for (int i = tid; i < 256; i += 256) {
+            int d = i / nb;
+            sm.row2dof[i] = (uint8_t)(d < nr ? d : 0xFF);
+            sm.col2dof[i] = (uint8_t)(d < nc ? d : 0xFF);
+        }
+        for (int i = tid; i < R * R; i += 256) {
+            sm.slotT[i] = 0.0;
+            sm.slotN[i] = 0;
+        }
+    }
+
+    // ---- wave-uniform mask of MFMA blocks holding at least one wanted pair --
+    uint32_t mask = 0;
+#pragma unroll
+    for (int bi = 0; bi < 4; ++bi) {
+#pragma unroll
+        for (int bj = 0; bj < 4; ++bj) {
+            const int rlo = 128 * wr + 32 * bi, clo = 128 * wc + 32 * bj;
+            const int drlo = rlo / nb, dclo = clo / nb;
+            const int dchi = min((clo + 31) / nb, nc - 1);
+            bool need = drlo < nr && dclo < nc;
+            if (diag) need = need && drlo < dchi;
+            if (need) mask |= 1u << (bi * 4 + bj);
+        }
+    }
+
+    const int nch = (int)((a.ss + kChunk - 1) / kChunk);
+    const int G = S * nch;
+    const bool aligned = ((a.stride & 3) == 0) && ((a.ss & 3) == 0) &&
+                         ((reinterpret_cast<uintptr_t>(a.codes) & 3) == 0);
+
+    // Stage the one-hot rows of chunk g into buffer g&1.
+    auto stage = [&](int g) {
+        const int s = g / nch, c = g - s * nch;
+        const uint64_t f0 = (uint64_t)s * a.ss + (uint64_t)c * kChunk;
+        const int nvalid = (int)min((uint64_t)kChunk, (uint64_t)(s + 1) * a.ss - f0);
+        const int b = g & 1;
+        for (int q = tid; q < ngen * kGroups; q += 256) {
+            const int ld = q / kGroups, k4 = (q - ld * kGroups) * 4;
+            const bool isA = ld < nr;
+            const int dof = isA ? r0 + ld : c0 + (ld - nr);
+            const int rbase = (isA ? ld : ld - nr) * nb;
+            uint8_t* op = sm.oh + (size_t)(b * 2 + (isA ? 0 : 1)) * kOpBytes;
+            const uint8_t* src = a.codes + (uint64_t)dof * a.stride + f0 + k4;
+            uint32_t cv[4];
+            if (aligned && k4 + 4 <= nvalid) {
+                const uint32_t w4 = *reinterpret_cast<const uint32_t*>(src);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) cv[e] = (w4 >> (8 * e)) & 0xFFu;
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) cv[e] = (k4 + e < nvalid) ? src[e] : 0xFFu;
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                if (cv[e] != 0xFFu && cv[e] >= (uint32_t)nb) {
+                    atomicOr(a.flag, 1);
+                    cv[e] = 0xFFu;
+                }
+            }
+            uint32_t* slot = sm.cl + ((size_t)b * 2 * R + ld) * kGroups + (k4 >> 2);
+            const uint32_t old = *slot;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const uint32_t ob = (old >> (8 * e)) & 0xFFu;
+                if (ob != 0xFFu) op[(rbase + ob) * kLdsRow + k4 + e] = 0;
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if (cv[e] != 0xFFu) op[(rbase + cv[e]) * kLdsRow + k4 + e] = 1;
+            *slot = cv[0] | (cv[1] << 8) | (cv[2] << 16) | (cv[3] << 24);
+        }
+    };
+
+    v16i acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = v16i{};
+
+    __syncthreads();
+    stage(0);
+    __syncthreads();
+
+    for (int g = 0; g < G; ++g) {
+        if (g + 1 < G) stage(g + 1);
+
+        const uint8_t* A = sm.oh + (size_t)((g & 1) * 2) * kOpBytes;
+        const uint8_t* B = diag ? A : A + kOpBytes;
+        if (mask) {
+#pragma unroll
+            for (int kk = 0; kk < kChunk / 32; ++kk) {
+                const int koff = kk * 32 + (lane >> 5) * 16;
+                v4i af[4], bf[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    af[i] = *reinterpret_cast<const v4i*>(
+                        A + (128 * wr + 32 * i + (lane & 31)) * kLdsRow + koff);
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    bf[j] = *reinterpret_cast<const v4i*>(
+                        B + (128 * wc + 32 * j + (lane & 31)) * kLdsRow + koff);
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        if (mask & (1u << (i * 4 + j)))
+                            acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
+                                af[i], bf[j], acc[i][j], 0, 0, 0);
+            }
+        }
+
+        if (g % nch == nch - 1) {
+            // ---- epilogue of step s: per-pair sum of c*log(c) -------------
+            const int s = g / nch;
+            // One 32x32 block: lane holds column (lane & 31) and rows
+            // 8*(r/4) + 4*(lane/32) + r%4 for r = 0..15.  Runs of rows of one
+            // DOF are summed in registers and flushed to the pair's LDS slot.
+            auto epi = [&](const v16i& blk, const int i, const int j) {
+                if (!(mask & (1u << (i * 4 + j)))) return;
+                const int gcol = 128 * wc + 32 * j + (lane & 31);
+                const int cd = sm.col2dof[gcol];
+                if (cd == 0xFF) return;
+                // park the lane's 16 counts in its own LDS slot so the row
+                // loop below stays rolled (no dynamic register indexing)
+                v4i* park = reinterpret_cast<v4i*>(sm.park) + (wave * 64 + lane) * 4;
+                park[0] = v4i{blk[0], blk[1], blk[2], blk[3]};
+                park[1] = v4i{blk[4], blk[5], blk[6], blk[7]};
+                park[2] = v4i{blk[8], blk[9], blk[10], blk[11]};
+                park[3] = v4i{blk[12], blk[13], blk[14], blk[15]};
+                const int* cnts = reinterpret_cast<const int*>(park);
+                int cur = -1, rn = 0;
+                double run = 0.0;
+#pragma unroll 1
+                for (int r = 0; r < 16; ++r) {
+                    const int grow = 128 * wr + 32 * i + 8 * (r >> 2) +
+                                     4 * (lane >> 5) + (r & 3);
+                    const int rd = sm.row2dof[grow];
+                    if (rd != cur) {
+                        if (cur >= 0 && cur != 0xFF && rn) {
+                            atomicAdd(&sm.slotT[cur * R + cd], run);
+                            atomicAdd(&sm.slotN[cur * R + cd], rn);
+                        }
+                        cur = rd;
+                        run = 0.0;
+                        rn = 0;
+                    }
+                    const int cnt = cnts[r];
+                    if (cnt > 0) {
+                        const double dc = (double)cnt;
+                        run += dc * log(dc);
+                        ++rn;
+                    }
+                    if constexpr (kDump) {
+                        if (rd != 0xFF) {
+                            const long long gi = r0 + rd, gj = c0 + cd;
+                            if (gi < gj) {
+                                const long long p = a.pairmap
+                                    ? (long long)a.pairmap[gi * D + gj]
+                                    : tri_index(gi, gj, D);
+                                if (p >= 0) {
+                                    const int bi_ = grow - rd * nb;
+                                    const int bj_ = gcol - cd * nb;
+                                    a.counts[((size_t)p * S + s) * nb * nb +
+                                             (size_t)bi_ * nb + bj_] = (uint32_t)cnt;
+                                }
+                            }
+                        }
+                    }
+                }
+                if (cur >= 0 && cur != 0xFF && rn) {
+                    atomicAdd(&sm.slotT[cur * R + cd], run);
+                    atomicAdd(&sm.slotN[cur * R + cd], rn);
+                }
+            };
+#define CENTRE_EPI(I, J) \
+    epi(acc[I][J], I, J);  \
+    __builtin_amdgcn_sched_barrier(0);
+            CENTRE_EPI(0, 0) CENTRE_EPI(0, 1) CENTRE_EPI(0, 2) CENTRE_EPI(0, 3)
+            CENTRE_EPI(1, 0) CENTRE_EPI(1, 1) CENTRE_EPI(1, 2) CENTRE_EPI(1, 3)
+            CENTRE_EPI(2, 0) CENTRE_EPI(2, 1) CENTRE_EPI(2, 2) CENTRE_EPI(2, 3)
+            CENTRE_EPI(3, 0) CENTRE_EPI(3, 1) CENTRE_EPI(3, 2) CENTRE_EPI(3, 3)
+#undef CENTRE_EPI
+            __syncthreads();
+            const double n = (double)(s + 1) * (double)a.ss;
+            const double logn = log(n);
+            for (int q = tid; q < nr * nc; q += 256) {
+                const int rd = q / nc, cd = q - rd * nc;
+                const long long gi = r0 + rd, gj = c0 + cd;
+                if (gi < gj) {
+                    const long long p = a.pairmap ? (long long)a.pairmap[gi * D + gj]
+                                                  : tri_index(gi, gj, D);
+                    if (p >= 0) {
+                        const double T = sm.slotT[rd * R + cd];
+                        const int nz = sm.slotN[rd * R + cd];
+                        const double L = a.dof_L[gi * S + s] + a.dof_L[gj * S + s];
+                        const double area = a.dof_w[gi] * a.dof_w[gj];
+                        const double ml = (logn + log(area)) - T / n + L / n;
+                        const double mm = ml + (double)(unsigned)(nz - 1) / (2.0 * n);
+                        if (a.slot_ml >= 0) a.s2[((size_t)p * a.nest + a.slot_ml) * S + s] = ml;
+                        if (a.slot_mm >= 0) a.s2[((size_t)p * a.nest + a.slot_mm) * S + s] = mm;
+                        if (a.mi) {
+                            const double s1 = a.dof_S1[gi * S + s] + a.dof_S1[gj * S + s];
+                            a.mi[(size_t)p * S + s] = s1 - (a.mi_from_mm ? mm : ml);
+                        }
+                    }
+                }
+                sm.slotT[rd * R + cd] = 0.0;
+                sm.slotN[rd * R + cd] = 0;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+size_t pair_tile_lds_bytes(int R) {
+    size_t a, b, c, d, e;
+    return lds_layout(R, &a, &b, &c, &d, &e);
+}
+
+hipError_t launch_pair_tiles(const PairKernelArgs& a, int n_tiles, bool dump,
+                             hipStream_t st) {
+    if (n_tiles <= 0) return hipSuccess;
+    const size_t lds = pair_tile_lds_bytes(a.R);
+    if (dump) {
+        hipError_t e = hipFuncSetAttribute(
+            reinterpret_cast<const void*>(&pair_tile_kernel<true>),
+            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(pair_tile_kernel<true>, dim3(n_tiles), dim3(256), lds, st, a);
+    } else {
+        hipError_t e = hipFuncSetAttribute(
+            reinterpret_cast<const void*>(&pair_tile_kernel<false>),
+            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(pair_tile_kernel<false>, dim3(n_tiles), dim3(256), lds, st, a);
+    }
+    return hipGetLastError();
+}

@@ -1,0 +1,278 @@
+"""ctypes binding of ``libcentre_hip.so`` (the C ABI in include/centre_hip.h).
+
+The library is the only compute path: if it is missing or the GPU is not
+usable, every entry point raises.  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcentre_hip.so")
+
+ML, MM, CS, JS = 0, 1, 2, 3
+BOND, ANGLE, DIHEDRAL = 1, 2, 3
+
+P = C.c_void_p
+U8, U32, U64, I32, F64 = C.c_uint8, C.c_uint32, C.c_uint64, C.c_int, C.c_double
+
+_SIGS = {
+    "centre_init": ([I32, P], I32),
+    "centre_destroy": ([P], None),
+    "centre_last_error": ([P], C.c_char_p),
+    "centre_sync": ([P], I32),
+    "centre_stream": ([P], P),
+    "centre_last_pair_kernel_ms": ([P], C.c_float),
+    "centre_dev_discretize_hist": ([P, P, U64, U32, U64, I32, P, I32, I32, I32, P, P], I32),
+    "centre_discretize_hist": ([P, P, U64, U32, U64, I32, P, I32, I32, I32, P, P], I32),
+    "centre_dev_discretize_vmkde": ([P, P, U64, U32, U64, I32, F64, U32, F64, U32, U32,
+                                     P, P, P, P], I32),
+    "centre_discretize_vmkde": ([P, P, U64, U32, U64, I32, F64, U32, F64, U32, U32,
+                                 P, P, P, P], I32),
+    "centre_dev_dof_entropy": ([P, P, U64, U32, U64, P, P, I32, I32, P, I32, I32, I32,
+                                P, P], I32),
+    "centre_pair_plan_create": ([P, U32, I32, P, U64, P], I32),
+    "centre_pair_plan_destroy": ([P], None),
+    "centre_pair_plan_records": ([P], U64),
+    "centre_pair_plan_tiles": ([P], I32),
+    "centre_dev_pair_entropy": ([P, P, P, U64, U64, P, P, I32, P, I32, I32, I32, P, P, P,
+                                 P], I32),
+    "centre_pair_entropy": ([P, P, U32, U64, P, P, I32, I32, P, U64, P, I32, I32, I32,
+                             P, P, P], I32),
+    "centre_dev_mst": ([P, P, P, P, U64, U32, P, P], I32),
+    "centre_mst": ([P, P, P, P, U64, U32, P, P], I32),
+    "centre_dev_segment_sum": ([P, P, P, I32, P], I32),
+    "centre_synth_bat": ([U64, U32, U32, U32, U64, P, I32], I32),
+}
+
+_lib = None
+
+
+def exported_symbols() -> Sequence[str]:
+    return list(_SIGS)
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with centre_amd/build.py "
+                f"(the HIP library is the only compute path)")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def _p(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def synth_bat(seed: int, n_bond: int, n_angle: int, n_dih: int,
+              n_frame: int, n_threads: int = 0) -> np.ndarray:
+    """Seeded synthetic BAT trajectory [n_frame, D] float64 (host)."""
+    out = np.empty((n_frame, n_bond + n_angle + n_dih), np.float64)
+    r = lib().centre_synth_bat(seed, n_bond, n_angle, n_dih, n_frame,
+                               _p(out), n_threads)
+    if r != 0:
+        raise CentreError(r, "centre_synth_bat failed")
+    return out
+
+
+class CentreError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"centre error {code}: {msg}")
+        self.code = code
+
+
+class Context:
+    """One GPU, one HIP stream (centre_init / centre_destroy)."""
+
+    def __init__(self, device: int = 0):
+        self._L = lib()
+        h = C.c_void_p()
+        r = self._L.centre_init(device, C.byref(h))
+        if r != 0:
+            raise CentreError(r, "centre_init failed (no usable GPU?)")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.centre_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def check(self, r: int):
+        if r != 0:
+            msg = self._L.centre_last_error(self.h)
+            raise CentreError(r, msg.decode() if msg else "")
+
+    @property
+    def stream(self) -> int:
+        return self._L.centre_stream(self.h) or 0
+
+    def sync(self):
+        self.check(self._L.centre_sync(self.h))
+
+    def last_pair_kernel_ms(self) -> float:
+        return float(self._L.centre_last_pair_kernel_ms(self.h))
+
+    # ---- host-buffer entry points ------------------------------------------
+    def discretize_hist(self, bat: np.ndarray, dof_type: int, schemes,
+                        nbins_ref: int = 0, optimize: bool = True):
+        """bat [F, D] float64 -> codes [D, nsch, F] uint8, extrema [D, 4]."""
+        bat = _c(bat, np.float64)
+        F, D = bat.shape
+        sch = _c(np.atleast_1d(schemes), np.uint8)
+        codes = np.zeros((D, len(sch), F), np.uint8)
+        ext = np.zeros((D, 4))
+        self.check(self._L.centre_discretize_hist(
+            self.h, _p(bat), D, D, F, dof_type, _p(sch), len(sch),
+            int(nbins_ref or sch[0]), int(optimize), _p(codes), _p(ext)))
+        return codes, ext
+
+    def discretize_vmkde(self, bat: np.ndarray, nmaxconf=5, kappa=1.0,
+                         sdo_step=5, sdo_conv=1e-4, sdo_iter=1000,
+                         grid_intervals=360, want_grid=False):
+        bat = _c(bat, np.float64)
+        F, D = bat.shape
+        codes = np.zeros((D, F), np.uint8)
+        mins = np.zeros((D, 9))
+        nf = np.zeros(D, np.uint8)
+        grid = np.zeros((D, 2, grid_intervals + 1)) if want_grid else None
+        self.check(self._L.centre_discretize_vmkde(
+            self.h, _p(bat), D, D, F, nmaxconf, kappa, sdo_step, sdo_conv,
+            sdo_iter, grid_intervals, _p(codes), _p(mins), _p(nf), _p(grid)))
+        return codes, mins, nf, grid
+
+    def pair_entropy(self, codes: np.ndarray, types, extrema, nbins: int,
+                     nsteps: int, pairs=None, ests=(ML,), jacobian=True,
+                     kde=False, want_s1=True, want_mi=True):
+        """codes [D, F] uint8 -> (S1 [D,nest,S], S2 [P,nest,S], MI [P,S])."""
+        codes = _c(codes, np.uint8)
+        D, F = codes.shape
+        types = _c(types, np.uint8)
+        ext = _c(extrema, np.float64)
+        ests = _c(np.atleast_1d(ests), np.int32)
+        nest = len(ests)
+        if pairs is None:
+            npair = D * (D - 1) // 2
+            pp = None
+        else:
+            pp = _c(pairs, np.uint32).reshape(-1, 2)
+            npair = len(pp)
+        s1 = np.zeros((D, nest, nsteps)) if want_s1 else None
+        s2 = np.zeros((npair, nest, nsteps))
+        mi = np.zeros((npair, nsteps)) if want_mi else None
+        self.check(self._L.centre_pair_entropy(
+            self.h, _p(codes), D, F, _p(types), _p(ext), nbins, nsteps, _p(pp),
+            npair, _p(ests), nest, int(jacobian), int(kde), _p(s1), _p(s2),
+            _p(mi)))
+        return s1, s2, mi
+
+    def mst(self, n_node: int, src, dst, w):
+        src = _c(src, np.uint32)
+        dst = _c(dst, np.uint32)
+        w = _c(w, np.float64)
+        out = np.zeros(max(n_node, 1), np.uint32)
+        nt = C.c_uint32(0)
+        self.check(self._L.centre_mst(self.h, _p(src), _p(dst), _p(w), len(w),
+                                      n_node, _p(out), C.byref(nt)))
+        return out[:nt.value]
+
+    # ---- device-pointer entry points (pointers as ints) ---------------------
+    def dev_discretize_hist(self, d_bat: int, ld: int, n_dof: int, n_frame: int,
+                            dof_type: int, schemes, nbins_ref: int,
+                            optimize: bool, d_codes: int, d_ext: int):
+        sch = _c(np.atleast_1d(schemes), np.uint8)
+        self.check(self._L.centre_dev_discretize_hist(
+            self.h, d_bat, ld, n_dof, n_frame, dof_type, _p(sch), len(sch),
+            nbins_ref, int(optimize), d_codes, d_ext))
+
+    def dev_dof_entropy(self, d_codes, stride, n_dof, n_frame, d_types, d_ext,
+                        nbins, nsteps, ests, jacobian, kde, d_s1, d_counts=None):
+        ests = _c(np.atleast_1d(ests), np.int32)
+        self.check(self._L.centre_dev_dof_entropy(
+            self.h, d_codes, stride, n_dof, n_frame, d_types, d_ext, nbins,
+            nsteps, _p(ests), len(ests), int(jacobian), int(kde), d_s1,
+            d_counts))
+
+    def dev_pair_entropy(self, plan: "PairPlan", d_codes, stride, n_frame,
+                         d_types, d_ext, nsteps, ests, jacobian, kde, d_s1,
+                         d_s2, d_mi, d_counts=None):
+        ests = _c(np.atleast_1d(ests), np.int32)
+        self.check(self._L.centre_dev_pair_entropy(
+            self.h, plan.h, d_codes, stride, n_frame, d_types, d_ext, nsteps,
+            _p(ests), len(ests), int(jacobian), int(kde), d_s1, d_s2, d_mi,
+            d_counts))
+
+    def dev_mst(self, d_src, d_dst, d_w, n_edge, n_node):
+        out = np.zeros(max(n_node, 1), np.uint32)
+        nt = C.c_uint32(0)
+        self.check(self._L.centre_dev_mst(self.h, d_src, d_dst, d_w, n_edge,
+                                          n_node, _p(out), C.byref(nt)))
+        return out[:nt.value]
+
+    def dev_segment_sum(self, d_x, seg_begin):
+        seg = _c(seg_begin, np.uint64)
+        out = np.zeros(len(seg) - 1)
+        self.check(self._L.centre_dev_segment_sum(self.h, d_x, _p(seg),
+                                                  len(seg) - 1, _p(out)))
+        return out
+
+
+class PairPlan:
+    """centre_pair_plan: the pair set and tiling of one pair-stage call."""
+
+    def __init__(self, ctx: Context, n_dof: int, nbins: int, pairs=None):
+        self.ctx = ctx
+        h = C.c_void_p()
+        pp = None if pairs is None else _c(pairs, np.uint32).reshape(-1, 2)
+        ctx.check(ctx._L.centre_pair_plan_create(
+            ctx.h, n_dof, nbins, _p(pp), 0 if pp is None else len(pp),
+            C.byref(h)))
+        self.h = h
+        self._keep = pp
+
+    @property
+    def records(self) -> int:
+        return int(self.ctx._L.centre_pair_plan_records(self.h))
+
+    @property
+    def tiles(self) -> int:
+        return int(self.ctx._L.centre_pair_plan_tiles(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx._L.centre_pair_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,177 @@
+/*
+ * centre_hip.h -- C ABI of the MI355X (gfx950) entropy hot path of CENTRE.
+ *
+ * Plain pointers and sizes only.  Functions named centre_dev_* take DEVICE
+ * pointers (memory the caller allocated on the context's GPU) and enqueue
+ * work on the context's stream; functions without _dev take HOST pointers,
+ * copy, run and copy back synchronously.
+ *
+ * Return value: 0 on success; -1 / -2 are the reference's size-mismatch
+ * codes (frequency / entropy buffer, Estimators.cpp:64,138,228-231,403-406);
+ * -3 = invalid argument; -4 = input code >= nbins; > 0 = a hipError_t.
+ * centre_last_error() returns a message for the last failure on a context.
+ *
+ * Layouts (one bin scheme per call; call once per scheme for several):
+ *   BAT columns   double  [n_frame][ld], DOF j of this type at column j
+ *                         (the [frame][dof] hyperslab Netcdf_BAT.cpp:247
+ *                         reads; rounded to float32 as nc_get_vara_float does)
+ *   codes         uint8   [n_dof][code_stride] (frames contiguous)
+ *   extrema       double  [n_dof][4] = {min, max, avg, phase}
+ *   S1            double  [n_dof][nest][nsteps]
+ *   S2            double  [n_pair][nest][nsteps]
+ *   MI            double  [n_pair][nsteps]   (estimator ests[0])
+ * DOF types: 1 bond, 2 angle, 3 torsion (BAT_t, Centre.h:42-47).
+ * Estimators: 0 ML, 1 MM, 2 CS, 3 JS (EntropyEstimators, Centre.h:89-94).
+ *
+ * Threading: one host thread per context; calls on one context serialise
+ * on its stream.  Contexts on the same GPU are independent.
+ */
+#ifndef CENTRE_HIP_H
+#define CENTRE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct centre_ctx centre_ctx;
+typedef struct centre_pair_plan centre_pair_plan;
+
+/* Context: owns a HIP stream and small device scratch on `device`.
+ * No reference counterpart: the reference calls host functions directly. */
+int centre_init(int device, centre_ctx **out);
+void centre_destroy(centre_ctx *ctx);
+const char *centre_last_error(const centre_ctx *ctx);
+int centre_sync(centre_ctx *ctx);
+/* The hipStream_t the context enqueues on (for the caller's event timing). */
+void *centre_stream(centre_ctx *ctx);
+/* Milliseconds of the last pair-stage kernel launch (HIP events). */
+float centre_last_pair_kernel_ms(const centre_ctx *ctx);
+
+/* ---- discretizer ----------------------------------------------------------
+ * Replaces Real2Int::bndReal2Int / angReal2Int / dihReal2Int
+ * (Real2Int.cpp:27-73, 131-184, 244-479) as called per DOF by
+ * Discretizer::run_dtype (Discretizer.cpp:138-216): float -> uint8 codes +
+ * extrema.  dof_type applies to all n_dof columns (one BAT type per call, as
+ * Discretizer::run calls run_dtype per type, Discretizer.cpp:314-361).
+ * codes: [n_dof][n_schemes][n_frame]. */
+int centre_dev_discretize_hist(centre_ctx *ctx, const double *d_bat,
+                               uint64_t ld, uint32_t n_dof, uint64_t n_frame,
+                               int dof_type, const uint8_t *schemes,
+                               int n_schemes, int nbins_ref, int optimize,
+                               uint8_t *d_codes, double *d_extrema);
+int centre_discretize_hist(centre_ctx *ctx, const double *bat, uint64_t ld,
+                           uint32_t n_dof, uint64_t n_frame, int dof_type,
+                           const uint8_t *schemes, int n_schemes,
+                           int nbins_ref, int optimize, uint8_t *codes,
+                           double *extrema);
+
+/* Replaces Real2Int::getMinimaPositions / vonMises and
+ * Utils::modifiedBesselI* (Real2Int.cpp:481-698, Utils.cpp:60-168) for
+ * torsions: von Mises KDE on a (grid_intervals+1)-point grid over 0..360
+ * degrees (360 = the reference's 1-degree grid), maxima, steepest-descent
+ * minima, and conformer-state codes [n_dof][n_frame].  minima: [n_dof][9]
+ * radians (first n_found[d] valid); grid_out (optional, may be NULL):
+ * [n_dof][2][grid_intervals+1] gradient and second derivative. */
+int centre_dev_discretize_vmkde(centre_ctx *ctx, const double *d_bat,
+                                uint64_t ld, uint32_t n_dof, uint64_t n_frame,
+                                int nmaxconf, double kappa, uint32_t sdo_step,
+                                double sdo_conv, uint32_t sdo_iter,
+                                uint32_t grid_intervals, uint8_t *d_codes,
+                                double *d_minima, uint8_t *d_nfound,
+                                double *d_grid);
+int centre_discretize_vmkde(centre_ctx *ctx, const double *bat, uint64_t ld,
+                            uint32_t n_dof, uint64_t n_frame, int nmaxconf,
+                            double kappa, uint32_t sdo_step, double sdo_conv,
+                            uint32_t sdo_iter, uint32_t grid_intervals,
+                            uint8_t *codes, double *minima, uint8_t *n_found,
+                            double *grid);
+
+/* ---- marginals -------------------------------------------------------------
+ * Replaces binData1D + entropy1D (Estimators.cpp:56-118, 217-389) for every
+ * DOF at once (EntropyCalculator::run1d, EntropyCalculator.cpp:6-279).
+ * Frames used: nsteps * (n_frame / nsteps) from the start of each row.
+ * counts (optional): [n_dof][nsteps][nbins] prefix counts (hist_*-1d). */
+int centre_dev_dof_entropy(centre_ctx *ctx, const uint8_t *d_codes,
+                           uint64_t code_stride, uint32_t n_dof,
+                           uint64_t n_frame, const uint8_t *d_types,
+                           const double *d_extrema, int nbins, int nsteps,
+                           const int *ests, int nest, int jacobian, int kde,
+                           double *d_s1, uint64_t *d_counts);
+
+/* ---- pair stage --------------------------------------------------------------
+ * A plan fixes the DOF count, bin count and the pair set: pairs == NULL
+ * means every i < j (row-major upper-triangle order, n_pair =
+ * n_dof*(n_dof-1)/2); otherwise an explicit list [n_pair][2] of DOF indices
+ * (any order, i != j; the reference's neighbour-list pairs,
+ * Centre.cpp:382-593).  Records keep the list order. */
+int centre_pair_plan_create(centre_ctx *ctx, uint32_t n_dof, int nbins,
+                            const uint32_t *pairs, uint64_t n_pair,
+                            centre_pair_plan **out);
+void centre_pair_plan_destroy(centre_pair_plan *plan);
+uint64_t centre_pair_plan_records(const centre_pair_plan *plan);
+int centre_pair_plan_tiles(const centre_pair_plan *plan);
+
+/* Replaces binData2D + entropy2D (Estimators.cpp:130-205, 391-651) over all
+ * pairs of the plan (EntropyCalculator::run2d_xx / run2d_xy,
+ * EntropyCalculator.cpp:281-1088), plus MI = S1_i + S1_j - S2_ij
+ * (getMaxMI2D, Convergance.cpp:183-194).  Joint counts are exact int32 MFMA
+ * contractions of one-hot codes; ML and MM are fused into the count pass,
+ * CS and JS go through materialised counts.  The marginals of the same
+ * frames are computed inside the call; d_s1 (optional) receives them as
+ * [n_dof][nest][nsteps].  d_counts (optional, NULL = never materialised;
+ * all-pair plans only): [n_pair][nsteps][nbins*nbins] uint32 prefix counts
+ * (hist_*-2d). */
+int centre_dev_pair_entropy(centre_ctx *ctx, const centre_pair_plan *plan,
+                            const uint8_t *d_codes, uint64_t code_stride,
+                            uint64_t n_frame, const uint8_t *d_types,
+                            const double *d_extrema, int nsteps,
+                            const int *ests, int nest, int jacobian, int kde,
+                            double *d_s1, double *d_s2, double *d_mi,
+                            uint32_t *d_counts);
+
+/* Host-buffer convenience: discretized codes in, S1/S2/MI out. */
+int centre_pair_entropy(centre_ctx *ctx, const uint8_t *codes,
+                        uint32_t n_dof, uint64_t n_frame,
+                        const uint8_t *types, const double *extrema,
+                        int nbins, int nsteps, const uint32_t *pairs,
+                        uint64_t n_pair, const int *ests, int nest,
+                        int jacobian, int kde, double *s1, double *s2,
+                        double *mi);
+
+/* ---- MIST --------------------------------------------------------------------
+ * Replaces boruvkaMST (Centre.cpp:5-143) as called by EntropyScorer::run
+ * (EntropyScorer.cpp:632-741): maximum spanning forest under the total order
+ * (weight descending, edge index ascending).  tree_edge_idx receives the
+ * forest's edge indices sorted by that order; *n_tree their count. */
+int centre_dev_mst(centre_ctx *ctx, const uint32_t *d_src,
+                   const uint32_t *d_dst, const double *d_w, uint64_t n_edge,
+                   uint32_t n_node, uint32_t *tree_edge_idx,
+                   uint32_t *n_tree);
+int centre_mst(centre_ctx *ctx, const uint32_t *src, const uint32_t *dst,
+               const double *w, uint64_t n_edge, uint32_t n_node,
+               uint32_t *tree_edge_idx, uint32_t *n_tree);
+
+/* Sum of each of n_seg contiguous segments of x (FP64), into out[n_seg]:
+ * the MIE totals (sumContrib1D/2D, Convergance.cpp:12-43).  seg_begin is a
+ * HOST array of n_seg + 1 ascending offsets into d_x. */
+int centre_dev_segment_sum(centre_ctx *ctx, const double *d_x,
+                           const uint64_t *seg_begin, int n_seg,
+                           double *out);
+
+/* ---- synthetic input ---------------------------------------------------------
+ * No reference counterpart (test and benchmark input).  Fills out[n_frame]
+ * [n_bond + n_angle + n_dih] (host, float64, [frame][dof]) with a seeded
+ * BAT trajectory: Gaussian bonds and angles and 3-component von Mises
+ * mixture torsions, all coupled through one shared 4-state Markov latent.
+ * The bytes depend only on the seed and sizes, not on n_threads. */
+int centre_synth_bat(uint64_t seed, uint32_t n_bond, uint32_t n_angle,
+                     uint32_t n_dih, uint64_t n_frame, double *out,
+                     int n_threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CENTRE_HIP_H */

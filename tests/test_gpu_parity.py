@@ -1,0 +1,250 @@
+"""HIP path vs the CPU oracle and the reference's stored outputs.
+
+Every test calls the C ABI (libcentre_hip.so) through ctypes; the oracle is
+only the checker.  Bars: codes, counts and MST edge sets bit-exact;
+entropies within 1e-10 relative (the pair kernel sums c*log(c) per pair in a
+different order than entropy2D, and the entropies are O(1), so the
+difference stays near 1e-14); MI, which can be close to zero, within 1e-10
+absolute.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from centre_amd import hip
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-10
+
+
+def rel_err(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1.0)))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = hip.Context(0)
+    yield c
+    c.close()
+
+
+def dev(x):
+    import torch
+    t = torch.from_numpy(np.ascontiguousarray(x)).cuda()
+    torch.cuda.synchronize()
+    return t
+
+
+# ---------------------------------------------------------------------------
+# reference goldens: example-1 codes -> entcontri S1/S2 (30 bins, 4 steps)
+TYPES = {"b": 1, "a": 2, "t": 3}
+
+
+def ex1_all(golden):
+    codes = np.concatenate([golden["ex1_codes_b"], golden["ex1_codes_a"],
+                            golden["ex1_codes_t"]])
+    ext = np.concatenate([golden["ex1_extreme_b"], golden["ex1_extreme_a"],
+                          golden["ex1_extreme_t"]])
+    nb_, na_ = len(golden["ex1_codes_b"]), len(golden["ex1_codes_a"])
+    types = np.array([1] * nb_ + [2] * na_ + [3] * len(golden["ex1_codes_t"]), np.uint8)
+    off = {"b": 0, "a": nb_, "t": nb_ + na_}
+    return codes, ext, types, off
+
+
+def test_example1_pair_entropies_vs_reference(ctx, golden):
+    codes, ext, types, off = ex1_all(golden)
+    kinds = {"bnd": "bb", "ang": "aa", "tor": "tt", "ba": "ba", "bd": "bt", "ad": "at"}
+    pairs, want = [], []
+    for k, (x, y) in kinds.items():
+        ids = golden[f"ex1_id_{k}2d"].astype(np.int64)
+        pairs.append(np.stack([ids[:, 0] + off[x], ids[:, 1] + off[y]], 1))
+        want.append(golden[f"ex1_S_{k}2d"])
+    pairs = np.concatenate(pairs)
+    want = np.concatenate(want)
+    print(f"example-1: {len(pairs)} pairs, {codes.shape} codes", flush=True)
+    s1, s2, mi = ctx.pair_entropy(codes, types, ext, 30, 4, pairs=pairs)
+    assert rel_err(s2[:, 0, :], want) < REL
+    for k, t in (("bnd", "b"), ("ang", "a"), ("tor", "t")):
+        ids = golden[f"ex1_id_{k}1d"][:, 0].astype(np.int64)
+        assert rel_err(s1[ids + off[t], 0, :], golden[f"ex1_S_{k}1d"]) < REL
+    mi_want = s1[pairs[:, 0], 0, :] + s1[pairs[:, 1], 0, :] - want
+    assert np.max(np.abs(mi - mi_want)) < 1e-10
+
+
+def test_example1_counts_bit_exact(ctx, golden):
+    import torch
+    codes, ext, types, off = ex1_all(golden)
+    D, F = codes.shape
+    plan = hip.PairPlan(ctx, D, 30)
+    P = plan.records
+    dc, dt, dx = dev(codes), dev(types), dev(ext)
+    s2 = torch.zeros((P, 1, 4), dtype=torch.float64, device="cuda")
+    cnt = torch.zeros((P, 4, 900), dtype=torch.int32, device="cuda")
+    ctx.dev_pair_entropy(plan, dc.data_ptr(), F, F, dt.data_ptr(), dx.data_ptr(),
+                         4, [0], True, False, 0, s2.data_ptr(), 0, cnt.data_ptr())
+    got = cnt.cpu().numpy().view(np.uint32)
+    iu, ju = np.triu_indices(D, 1)
+    pick = np.random.default_rng(0).choice(P, 300, replace=False)
+    want = O.pair_counts(codes, 30, np.stack([iu[pick], ju[pick]], 1), 4)
+    assert np.array_equal(got[pick].reshape(-1, 4, 30, 30), want.astype(np.uint32))
+
+
+# ---------------------------------------------------------------------------
+# synthetic configuration, 48 DOF (16 B / 16 A / 16 T) x 10,000 frames
+@pytest.fixture(scope="module")
+def cfg1():
+    bat = hip.synth_bat(0xCE27E + 0, 16, 16, 16, 10000)
+    return bat
+
+
+def oracle_discretize(bat, nb, nbins_ref, optimize=True):
+    F, D = bat.shape
+    codes = np.zeros((D, 1, F), np.uint8)
+    ext = np.zeros((D, 4))
+    types = np.array([1] * 16 + [2] * 16 + [3] * 16, np.uint8)
+    for d in range(D):
+        codes[d], ext[d] = O.real2int(bat[:, d].astype(np.float32), types[d], [nb],
+                                      nbins_ref, optimize)
+    return codes, ext, types
+
+
+@pytest.mark.parametrize("optimize", [True, False])
+def test_discretize_hist_vs_oracle(ctx, cfg1, optimize):
+    nb = 30
+    want_c, want_e, types = oracle_discretize(cfg1, nb, nb, optimize)
+    for t, sl in ((1, slice(0, 16)), (2, slice(16, 32)), (3, slice(32, 48))):
+        codes, ext = ctx.discretize_hist(cfg1[:, sl], t, [nb], nb, optimize)
+        assert np.array_equal(codes, want_c[sl])
+        assert np.array_equal(ext[:, [0, 1, 3]], want_e[sl][:, [0, 1, 3]])
+        assert np.max(np.abs(ext[:, 2] - want_e[sl][:, 2])) < 1e-12
+
+
+def test_discretize_multi_scheme(ctx, cfg1):
+    for t, sl in ((1, slice(0, 16)), (3, slice(32, 48))):
+        codes, ext = ctx.discretize_hist(cfg1[:, sl], t, [10, 30, 50], 30, True)
+        for d in range(16):
+            wc, we = O.real2int(cfg1[:, sl][:, d].astype(np.float32), t, [10, 30, 50], 30, True)
+            assert np.array_equal(codes[d], wc)
+            assert np.array_equal(ext[d, [0, 1, 3]], we[[0, 1, 3]])
+
+
+@pytest.mark.parametrize("ests", [(0,), (1,), (0, 1), (2,), (3,), (0, 1, 2, 3)])
+def test_cfg1_pair_and_marginal_entropies(ctx, cfg1, ests):
+    nb, S = 30, 5
+    codes, ext, types = oracle_discretize(cfg1, nb, nb)
+    codes = codes[:, 0, :]
+    D = codes.shape[0]
+    iu, ju = np.triu_indices(D, 1)
+    pairs = np.stack([iu, ju], 1)
+    s1, s2, mi = ctx.pair_entropy(codes, types, ext, nb, S, ests=ests)
+    w1 = O.dof_entropies(codes, types, ext, S, [nb], ests)
+    w2 = O.pair_entropies(codes, types, ext, pairs, S, [nb], ests)
+    assert rel_err(s1[:, :, :], w1[:, :, 0, :]) < REL
+    assert rel_err(s2, w2[:, :, 0, :]) < REL
+    mi_w = w1[iu, 0, 0, :] + w1[ju, 0, 0, :] - w2[:, 0, 0, :]
+    assert np.max(np.abs(mi - mi_w)) < 1e-10
+
+
+@pytest.mark.parametrize("nb,D,F,S", [(50, 7, 3001, 3), (5, 70, 2049, 2),
+                                      (3, 130, 999, 1), (255, 3, 1000, 4),
+                                      (64, 9, 50, 7), (31, 13, 4096, 16)])
+def test_ragged_shapes_vs_oracle(ctx, nb, D, F, S):
+    rng = np.random.default_rng(nb * 1000 + D)
+    codes = rng.integers(0, nb, size=(D, F), dtype=np.uint8)
+    # correlate some rows so counts are not uniform
+    codes[1::2] = (codes[0::2][: len(codes[1::2])] + rng.integers(0, 2, size=F)) % nb
+    types = rng.integers(1, 4, size=D).astype(np.uint8)
+    ext = np.zeros((D, 4))
+    ext[:, 0] = rng.uniform(0.5, 1.0, D)
+    ext[:, 1] = ext[:, 0] + rng.uniform(0.2, 1.5, D)
+    iu, ju = np.triu_indices(D, 1)
+    s1, s2, mi = ctx.pair_entropy(codes, types, ext, nb, S, ests=(0, 1))
+    w2 = O.pair_entropies(codes, types, ext, np.stack([iu, ju], 1), S, [nb], (0, 1))
+    assert rel_err(s2, w2[:, :, 0, :]) < REL
+
+
+def test_explicit_pairs_order_and_duplicates(ctx, cfg1):
+    nb, S = 30, 4
+    codes, ext, types = oracle_discretize(cfg1, nb, nb)
+    codes = codes[:, 0, :]
+    rng = np.random.default_rng(5)
+    pairs = rng.integers(0, 48, size=(200, 2))
+    pairs = pairs[pairs[:, 0] != pairs[:, 1]]
+    pairs = np.concatenate([pairs, pairs[:10, ::-1], pairs[:5]])
+    s1, s2, mi = ctx.pair_entropy(codes, types, ext, nb, S, pairs=pairs)
+    w2 = O.pair_entropies(codes, types, ext, pairs, S, [nb])
+    assert rel_err(s2[:, 0, :], w2[:, 0, 0, :]) < REL
+
+
+def test_code_out_of_range_is_an_error(ctx):
+    codes = np.zeros((4, 100), np.uint8)
+    codes[2, 50] = 30
+    with pytest.raises(hip.CentreError) as e:
+        ctx.pair_entropy(codes, np.ones(4, np.uint8), np.tile([1.0, 2.0, 0, 0], (4, 1)), 30, 2)
+    assert e.value.code == -4
+
+
+# ---------------------------------------------------------------------------
+def kruskal_set(n, src, dst, w):
+    order = sorted(range(len(w)), key=lambda e: (-w[e], e))
+    par = list(range(n))
+
+    def find(x):
+        while par[x] != x:
+            par[x] = par[par[x]]
+            x = par[x]
+        return x
+    out = []
+    for e in order:
+        a, b = find(src[e]), find(dst[e])
+        if a != b:
+            par[a] = b
+            out.append(e)
+    return out
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_mst_vs_oracle(ctx, seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(2, 300))
+    m = int(rng.integers(1, 3000))
+    src = rng.integers(0, n, m)
+    dst = rng.integers(0, n, m)
+    keep = src != dst
+    src, dst = src[keep], dst[keep]
+    w = rng.integers(0, 7, len(src)).astype(float) / 3 - 0.5
+    got = ctx.mst(n, src, dst, w)
+    want = O.boruvka(n, src, dst, w)
+    assert sorted(got.tolist()) == sorted(want.tolist())
+    assert got.tolist() == kruskal_set(n, src.tolist(), dst.tolist(), w.tolist())
+
+
+def test_mst_example2_trees(ctx, golden):
+    from test_oracle_scorer import score
+    for step in range(4):
+        _, _, _, src, dst, w, nt, nid = score(golden, "ex2", step)
+        got = ctx.mst(len(nt), src, dst, w)
+        want = O.boruvka(len(nt), src, dst, w)
+        assert sorted(got.tolist()) == sorted(want.tolist())
+        assert len(got) == 1036
+
+
+# ---------------------------------------------------------------------------
+def test_vmkde_grid_and_states(ctx):
+    bat = hip.synth_bat(0xCE27E + 2, 0, 0, 12, 3000)
+    for G, kappa, nmax in ((360, 1.0, 3), (64, 10.0, 5)):
+        codes, mins, nf, grid = ctx.discretize_vmkde(bat, nmax, kappa, 5, 1e-4, 1000, G,
+                                                     want_grid=True)
+        for d in range(bat.shape[1]):
+            x = bat[:, d].astype(np.float32)
+            wg = O.vmkde_grid(x, kappa, G)
+            g = grid[d].reshape(-1)
+            scale = np.max(np.abs(wg[: G + 1]))
+            assert np.max(np.abs(g[: G + 1] - wg[: G + 1])) <= 1e-11 * scale
+            # the state rule on the device grid is bit-exact with the oracle's
+            wc, wm, wn = O.vmkde_states(x, g, nmax, 5, 1e-4, 1000, G)
+            assert nf[d] == wn
+            assert np.array_equal(mins[d, :wn], wm)
+            assert np.array_equal(codes[d], wc)
