@@ -35,8 +35,8 @@ def parse():
     ap.add_argument("--frames", type=int, default=100000)
     ap.add_argument("--nbins", type=int, default=50)
     ap.add_argument("--nsteps", type=int, default=10)
-    ap.add_argument("--cpu-pairs", type=int, default=1500,
-                    help="pairs in the CPU-baseline sample")
+    ap.add_argument("--cpu-pairs", type=int, default=0,
+                    help="pairs in the CPU-baseline sample (0 = all pairs)")
     ap.add_argument("--no-cpu", action="store_true")
     return ap.parse_args()
 
@@ -49,7 +49,6 @@ def main():
               "path is built); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    import torch
     from centre_amd import hip
 
     nb, S, F = args.nbins, args.nsteps, args.frames
@@ -63,57 +62,56 @@ def main():
     t_gen = time.perf_counter() - t0
 
     ctx = hip.Context(0)
-    dev = torch.device("cuda:0")
-    d_bat = torch.from_numpy(bat).to(dev)
-    d_codes = torch.empty((D, F), dtype=torch.uint8, device=dev)
-    d_ext = torch.empty((D, 4), dtype=torch.float64, device=dev)
+    d_bat = ctx.to_device(bat)
+    d_codes = ctx.alloc((D, F), np.uint8)
+    d_ext = ctx.alloc((D, 4), np.float64)
     types = np.array([1] * nbd + [2] * nan + [3] * ndh, np.uint8)
-    d_types = torch.from_numpy(types).to(dev)
-    torch.cuda.synchronize()
+    d_types = ctx.to_device(types)
+    ctx.device_sync()
 
     # ---- discretisation (once; reported separately) ----------------------
     t0 = time.perf_counter()
     col = 0
     for t, n in ((1, nbd), (2, nan), (3, ndh)):
         if n:
-            ctx.dev_discretize_hist(d_bat.data_ptr() + 8 * col, D, n, F, t, [nb],
-                                    nb, True, d_codes.data_ptr() + col * F,
-                                    d_ext.data_ptr() + 32 * col)
+            ctx.dev_discretize_hist(d_bat.ptr + 8 * col, D, n, F, t, [nb],
+                                    nb, True, d_codes.ptr + col * F,
+                                    d_ext.ptr + 32 * col)
         col += n
     ctx.sync()
     t_disc = time.perf_counter() - t0
 
     plan = hip.PairPlan(ctx, D, nb)
-    d_s1 = torch.empty((D, 1, S), dtype=torch.float64, device=dev)
-    d_s2 = torch.empty((P, 1, S), dtype=torch.float64, device=dev)
-    d_mi = torch.empty((P, S), dtype=torch.float64, device=dev)
+    d_s1 = ctx.alloc((D, 1, S), np.float64)
+    d_s2 = ctx.alloc((P, 1, S), np.float64)
+    d_mi = ctx.alloc((P, S), np.float64)
 
     def step():
-        ctx.dev_pair_entropy(plan, d_codes.data_ptr(), F, F, d_types.data_ptr(),
-                             d_ext.data_ptr(), S, [0], True, False,
-                             d_s1.data_ptr(), d_s2.data_ptr(), d_mi.data_ptr())
+        ctx.dev_pair_entropy(plan, d_codes.ptr, F, F, d_types.ptr, d_ext.ptr, S,
+                             [0], True, False, d_s1.ptr, d_s2.ptr, d_mi.ptr)
 
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
+    ctx.device_sync()
     kms = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
         kms.append(ctx.last_pair_kernel_ms())
-    torch.cuda.synchronize()
+    ctx.device_sync()
     t_run = time.perf_counter() - t0
     ms_step = 1000.0 * t_run / args.steps
     value = P / (t_run / args.steps)
 
     # ---- MIST on the last step's MI (once; reported separately) ------------
     iu, ju = np.triu_indices(D, 1)
-    d_src = torch.from_numpy(iu.astype(np.uint32)).to(dev)
-    d_dst = torch.from_numpy(ju.astype(np.uint32)).to(dev)
-    w_last = d_mi[:, S - 1].contiguous()
-    torch.cuda.synchronize()
+    d_src = ctx.to_device(iu.astype(np.uint32))
+    d_dst = ctx.to_device(ju.astype(np.uint32))
+    mi_host = d_mi.numpy()
+    w_last = ctx.to_device(np.ascontiguousarray(mi_host[:, S - 1]))
+    ctx.device_sync()
     t0 = time.perf_counter()
-    tree = ctx.dev_mst(d_src.data_ptr(), d_dst.data_ptr(), w_last.data_ptr(), P, D)
+    tree = ctx.dev_mst(d_src.ptr, d_dst.ptr, w_last.ptr, P, D)
     t_mst = time.perf_counter() - t0
 
     kernel_ms = statistics.mean(kms)
@@ -123,11 +121,12 @@ def main():
 
     # ---- CPU baseline: the oracle's per-pair loop on a bounded sample -------
     cpu = None
+    parity = None
     if not args.no_cpu:
         import oracle as O
-        codes = d_codes.cpu().numpy()
-        ext = d_ext.cpu().numpy()
-        k = min(args.cpu_pairs, P)
+        codes = d_codes.numpy()
+        ext = d_ext.numpy()
+        k = min(args.cpu_pairs, P) if args.cpu_pairs > 0 else P
         pick = np.linspace(0, P - 1, k).astype(np.int64)
         pairs = np.stack([iu[pick], ju[pick]], 1)
         nthreads = min(16, os.cpu_count() or 1)
@@ -135,6 +134,14 @@ def main():
         t0 = time.perf_counter()
         O.pair_entropies(codes, types, ext, pairs, S, [nb], nthreads=nthreads)
         t_cpu = time.perf_counter() - t0
+        # spot check of the timed GPU output against the oracle on 64 pairs
+        chk = pick[np.linspace(0, k - 1, min(64, k)).astype(np.int64)]
+        want = O.pair_entropies(codes, types, ext, np.stack([iu[chk], ju[chk]], 1), S, [nb],
+                                nthreads=nthreads)[:, 0, 0, :]
+        got = d_s2.numpy()[chk, 0, :]
+        parity = {"pairs_checked": int(len(chk)),
+                  "max_rel_err_S2": float(np.max(np.abs(got - want) /
+                                                 np.maximum(np.abs(want), 1.0)))}
         cpu = {"value": k / t_cpu, "unit": "pair entropies/s", "cores": nthreads,
                "kind": "port",
                "sample": f"{k} of the {P} pairs (evenly strided), same codes, "
@@ -163,6 +170,7 @@ def main():
                      "kernel": "pair_tile_kernel", "kernel_ms": kernel_ms,
                      "ops_per_launch": ops},
         "cpu_baseline": cpu,
+        "parity_sample": parity,
         "stages_s": {"generate_host": t_gen, "discretize": t_disc,
                      "mst": t_mst, "mst_edges": int(len(tree))},
     }

@@ -438,6 +438,10 @@ extern "C" int centre_dev_pair_entropy(centre_ctx* ctx, const centre_pair_plan* 
     a.mi_from_mm = (ests[0] == 1);
     a.counts = cnt;
     a.flag = ctx->d_flag;
+    DevBuf tab;
+    HIPCHK(tab.alloc(sizeof(double) * (nsteps * ss + 1)));
+    HIPCHK(launch_clogc_table(tab.as<double>(), nsteps * ss + 1, ctx->stream));
+    a.clogc = tab.as<double>();
     // dof_S1 is [D][nest][S]; the kernel reads [D][S] of estimator 0.  With
     // nest > 1 give it a compact copy.
     DevBuf s1e0;
@@ -803,5 +807,48 @@ extern "C" int centre_dev_segment_sum(centre_ctx* ctx, const double* d_x,
     HIPCHK(launch_segment_sum(d_x, b.as<uint64_t>(), n_seg, o.as<double>(), ctx->stream));
     HIPCHK(hipMemcpyAsync(out, o.p, sizeof(double) * n_seg, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Device memory plumbing for callers without their own allocator.
+extern "C" int centre_dev_alloc(centre_ctx* ctx, size_t bytes, void** out) {
+    if (!ctx || !out) return -3;
+    *out = nullptr;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipMalloc(out, bytes ? bytes : 16));
+    return 0;
+}
+
+extern "C" int centre_dev_free(centre_ctx* ctx, void* p) {
+    if (!ctx) return -3;
+    if (!p) return 0;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipFree(p));
+    return 0;
+}
+
+extern "C" int centre_memcpy_h2d(centre_ctx* ctx, void* d_dst, const void* src, size_t bytes) {
+    if (!ctx) return -3;
+    if (!bytes) return 0;
+    if (!d_dst || !src) return fail(ctx, -3, "null pointer");
+    HIPCHK(hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+extern "C" int centre_memcpy_d2h(centre_ctx* ctx, void* dst, const void* d_src, size_t bytes) {
+    if (!ctx) return -3;
+    if (!bytes) return 0;
+    if (!dst || !d_src) return fail(ctx, -3, "null pointer");
+    HIPCHK(hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+extern "C" int centre_device_sync(centre_ctx* ctx) {
+    if (!ctx) return -3;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipDeviceSynchronize());
     return 0;
 }

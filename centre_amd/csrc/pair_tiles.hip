@@ -23,12 +23,15 @@
 // which is entropy2D's ML sum -sum (c/n) log(c / (n w_i w_j J_i J_j)) with
 // the Jacobian part folded into the marginal sums L_i = sum_b m_ib log J_ib
 // (m = 1-D counts of the same frames).  MM adds (nonzero bins - 1)/(2n).
+// c*log(c) comes from a table indexed by the count (c <= frames used), built
+// once per call, so the epilogue is a gather plus additions.
 //
-// One-hot staging.  Codes of the tile's DOFs are loaded per chunk and the
-// one-hot rows are written into LDS by setting one byte per (DOF, frame) and
-// clearing the byte the same slot set two chunks earlier, so a chunk costs a
-// few byte writes per thread instead of rebuilding 32 KB of zeros.  Two
-// buffers alternate: chunk g+1 is staged while chunk g feeds the MFMAs.
+// One-hot staging.  Codes of the tile's DOFs are prefetched into registers
+// two chunks ahead, and the one-hot rows are written into LDS by setting
+// one byte per (DOF, frame) and clearing the byte the same slot set two
+// chunks earlier, so a chunk costs a few byte writes per thread instead of
+// rebuilding 32 KB of zeros.  Two LDS buffers alternate: chunk g+1 is
+// staged while chunk g feeds the MFMAs.
 //
 // Waves: 4 per workgroup in a 2x2 layout, each owning a 128x128 quadrant
 // (4x4 MFMA 32x32 blocks = 256 int32 accumulators per lane).  Blocks with no
@@ -43,6 +46,7 @@ namespace {
 
 constexpr int kOpBytes = kTileRows * kLdsRow;   // one operand, one buffer
 constexpr int kGroups = kChunk / 4;             // dword groups per DOF row
+constexpr int kMaxPer = (2 * 60 * kGroups + 255) / 256;  // groups per thread
 
 struct Smem {
     uint8_t* oh;        // [2 buf][2 operand][kTileRows * kLdsRow]
@@ -51,15 +55,12 @@ struct Smem {
     uint8_t* col2dof;   // [256]
     double* slotT;      // [R*R] sum c log c per local pair
     int* slotN;         // [R*R] nonzero bins per local pair
-    int* park;          // [4 waves][64 lanes][16] epilogue staging
 };
 
 __host__ __device__ inline size_t lds_layout(int R, size_t* off_cl,
                                              size_t* off_map, size_t* off_T,
-                                             size_t* off_N, size_t* off_park) {
+                                             size_t* off_N) {
     size_t off = (size_t)2 * 2 * kOpBytes;
-    *off_park = off;
-    off += (size_t)4 * 64 * 16 * 4;
     *off_cl = off;
     off += (size_t)2 * 2 * R * kGroups * 4;
     *off_map = off;
@@ -80,8 +81,8 @@ template <bool kDump>
 __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
     extern __shared__ __align__(16) uint8_t smem_raw[];
     const int R = a.R, nb = a.nb, S = a.nsteps;
-    size_t ocl, omap, oT, oN, opark;
-    lds_layout(R, &ocl, &omap, &oT, &oN, &opark);
+    size_t ocl, omap, oT, oN;
+    lds_layout(R, &ocl, &omap, &oT, &oN);
     Smem sm;
     sm.oh = smem_raw;
     sm.cl = reinterpret_cast<uint32_t*>(smem_raw + ocl);
@@ -89,7 +90,6 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
     sm.col2dof = smem_raw + omap + 256;
     sm.slotT = reinterpret_cast<double*>(smem_raw + oT);
     sm.slotN = reinterpret_cast<int*>(smem_raw + oN);
-    sm.park = reinterpret_cast<int*>(smem_raw + opark);
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1;
@@ -98,6 +98,7 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
     const int nr = min(R, a.n_dof - r0), nc = min(R, a.n_dof - c0);
     const bool diag = tile.x == tile.y;
     const int ngen = diag ? nr : nr + nc;   // diagonal tiles share operands
+    const int nwork = ngen * kGroups;
     const long long D = a.n_dof;
 
     // ---- LDS initialisation ------------------------------------------------
@@ -137,46 +138,70 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
     const bool aligned = ((a.stride & 3) == 0) && ((a.ss & 3) == 0) &&
                          ((reinterpret_cast<uintptr_t>(a.codes) & 3) == 0);
 
-    // Stage the one-hot rows of chunk g into buffer g&1.
-    auto stage = [&](int g) {
+    // Codes of chunk g for this thread's dword groups; 0xFF marks a frame
+    // outside the chunk.
+    auto load_codes = [&](int g, uint32_t (&dst)[kMaxPer]) __attribute__((always_inline)) {
         const int s = g / nch, c = g - s * nch;
         const uint64_t f0 = (uint64_t)s * a.ss + (uint64_t)c * kChunk;
         const int nvalid = (int)min((uint64_t)kChunk, (uint64_t)(s + 1) * a.ss - f0);
-        const int b = g & 1;
-        for (int q = tid; q < ngen * kGroups; q += 256) {
-            const int ld = q / kGroups, k4 = (q - ld * kGroups) * 4;
-            const bool isA = ld < nr;
-            const int dof = isA ? r0 + ld : c0 + (ld - nr);
-            const int rbase = (isA ? ld : ld - nr) * nb;
-            uint8_t* op = sm.oh + (size_t)(b * 2 + (isA ? 0 : 1)) * kOpBytes;
-            const uint8_t* src = a.codes + (uint64_t)dof * a.stride + f0 + k4;
-            uint32_t cv[4];
-            if (aligned && k4 + 4 <= nvalid) {
-                const uint32_t w4 = *reinterpret_cast<const uint32_t*>(src);
 #pragma unroll
-                for (int e = 0; e < 4; ++e) cv[e] = (w4 >> (8 * e)) & 0xFFu;
-            } else {
-#pragma unroll
-                for (int e = 0; e < 4; ++e) cv[e] = (k4 + e < nvalid) ? src[e] : 0xFFu;
-            }
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                if (cv[e] != 0xFFu && cv[e] >= (uint32_t)nb) {
-                    atomicOr(a.flag, 1);
-                    cv[e] = 0xFFu;
+        for (int k = 0; k < kMaxPer; ++k) {
+            const int q = tid + 256 * k;
+            uint32_t w = 0xFFFFFFFFu;
+            if (q < nwork) {
+                const int ld = q / kGroups, k4 = (q - ld * kGroups) * 4;
+                const int dof = ld < nr ? r0 + ld : c0 + (ld - nr);
+                const uint8_t* src = a.codes + (uint64_t)dof * a.stride + f0 + k4;
+                if (aligned && k4 + 4 <= nvalid) {
+                    w = *reinterpret_cast<const uint32_t*>(src);
+                } else {
+                    w = 0;
+                    for (int e = 0; e < 4; ++e)
+                        w |= (k4 + e < nvalid ? (uint32_t)src[e] : 0xFFu) << (8 * e);
                 }
             }
-            uint32_t* slot = sm.cl + ((size_t)b * 2 * R + ld) * kGroups + (k4 >> 2);
-            const uint32_t old = *slot;
+            dst[k] = w;
+        }
+    };
+    // One-hot rows of chunk g into buffer g&1 from codes loaded earlier.
+    // A code >= nb (including a stored 0xFF on a valid frame) raises the
+    // error flag and is dropped.
+    auto write_onehot = [&](int g, const uint32_t (&cw)[kMaxPer]) __attribute__((always_inline)) {
+        const int b = g & 1;
+        const int s = g / nch, c = g - s * nch;
+        const uint64_t f0 = (uint64_t)s * a.ss + (uint64_t)c * kChunk;
+        const int nvalid = (int)min((uint64_t)kChunk, (uint64_t)(s + 1) * a.ss - f0);
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const uint32_t ob = (old >> (8 * e)) & 0xFFu;
-                if (ob != 0xFFu) op[(rbase + ob) * kLdsRow + k4 + e] = 0;
+        for (int k = 0; k < kMaxPer; ++k) {
+            const int q = tid + 256 * k;
+            if (q < nwork) {
+                const int ld = q / kGroups, k4 = (q - ld * kGroups) * 4;
+                const bool isA = ld < nr;
+                const int rbase = (isA ? ld : ld - nr) * nb;
+                uint8_t* op = sm.oh + (size_t)(b * 2 + (isA ? 0 : 1)) * kOpBytes;
+                uint32_t cv[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    cv[e] = (cw[k] >> (8 * e)) & 0xFFu;
+                    if (k4 + e >= nvalid) {
+                        cv[e] = 0xFFu;
+                    } else if (cv[e] >= (uint32_t)nb) {
+                        atomicOr(a.flag, 1);
+                        cv[e] = 0xFFu;
+                    }
+                }
+                uint32_t* slot = sm.cl + ((size_t)b * 2 * R + ld) * kGroups + (k4 >> 2);
+                const uint32_t old = *slot;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const uint32_t ob = (old >> (8 * e)) & 0xFFu;
+                    if (ob != 0xFFu) op[(rbase + ob) * kLdsRow + k4 + e] = 0;
+                }
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (cv[e] != 0xFFu) op[(rbase + cv[e]) * kLdsRow + k4 + e] = 1;
+                *slot = cv[0] | (cv[1] << 8) | (cv[2] << 16) | (cv[3] << 24);
             }
-#pragma unroll
-            for (int e = 0; e < 4; ++e)
-                if (cv[e] != 0xFFu) op[(rbase + cv[e]) * kLdsRow + k4 + e] = 1;
-            *slot = cv[0] | (cv[1] << 8) | (cv[2] << 16) | (cv[3] << 24);
         }
     };
 
@@ -186,89 +211,106 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = v16i{};
 
-    __syncthreads();
-    stage(0);
-    __syncthreads();
-
-    for (int g = 0; g < G; ++g) {
-        if (g + 1 < G) stage(g + 1);
-
+    auto mfma_chunk = [&](int g) __attribute__((always_inline)) {
+        if (!mask) return;
         const uint8_t* A = sm.oh + (size_t)((g & 1) * 2) * kOpBytes;
         const uint8_t* B = diag ? A : A + kOpBytes;
-        if (mask) {
 #pragma unroll
-            for (int kk = 0; kk < kChunk / 32; ++kk) {
-                const int koff = kk * 32 + (lane >> 5) * 16;
-                v4i af[4], bf[4];
+        for (int kk = 0; kk < kChunk / 32; ++kk) {
+            const int koff = kk * 32 + (lane >> 5) * 16;
+            v4i af[4], bf[4];
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
-                    af[i] = *reinterpret_cast<const v4i*>(
-                        A + (128 * wr + 32 * i + (lane & 31)) * kLdsRow + koff);
+            for (int i = 0; i < 4; ++i)
+                af[i] = *reinterpret_cast<const v4i*>(
+                    A + (128 * wr + 32 * i + (lane & 31)) * kLdsRow + koff);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                bf[j] = *reinterpret_cast<const v4i*>(
+                    B + (128 * wc + 32 * j + (lane & 31)) * kLdsRow + koff);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
-                    bf[j] = *reinterpret_cast<const v4i*>(
-                        B + (128 * wc + 32 * j + (lane & 31)) * kLdsRow + koff);
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-#pragma unroll
-                    for (int j = 0; j < 4; ++j)
-                        if (mask & (1u << (i * 4 + j)))
-                            acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
-                                af[i], bf[j], acc[i][j], 0, 0, 0);
-            }
+                    if (mask & (1u << (i * 4 + j)))
+                        acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
+                            af[i], bf[j], acc[i][j], 0, 0, 0);
         }
+    };
 
-        if (g % nch == nch - 1) {
-            // ---- epilogue of step s: per-pair sum of c*log(c) -------------
-            const int s = g / nch;
-            // One 32x32 block: lane holds column (lane & 31) and rows
-            // 8*(r/4) + 4*(lane/32) + r%4 for r = 0..15.  Runs of rows of one
-            // DOF are summed in registers and flushed to the pair's LDS slot.
-            auto epi = [&](const v16i& blk, const int i, const int j) {
-                if (!(mask & (1u << (i * 4 + j)))) return;
-                const int gcol = 128 * wc + 32 * j + (lane & 31);
-                const int cd = sm.col2dof[gcol];
-                if (cd == 0xFF) return;
-                // park the lane's 16 counts in its own LDS slot so the row
-                // loop below stays rolled (no dynamic register indexing)
-                v4i* park = reinterpret_cast<v4i*>(sm.park) + (wave * 64 + lane) * 4;
-                park[0] = v4i{blk[0], blk[1], blk[2], blk[3]};
-                park[1] = v4i{blk[4], blk[5], blk[6], blk[7]};
-                park[2] = v4i{blk[8], blk[9], blk[10], blk[11]};
-                park[3] = v4i{blk[12], blk[13], blk[14], blk[15]};
-                const int* cnts = reinterpret_cast<const int*>(park);
-                int cur = -1, rn = 0;
-                double run = 0.0;
-#pragma unroll 1
-                for (int r = 0; r < 16; ++r) {
-                    const int grow = 128 * wr + 32 * i + 8 * (r >> 2) +
-                                     4 * (lane >> 5) + (r & 3);
-                    const int rd = sm.row2dof[grow];
-                    if (rd != cur) {
-                        if (cur >= 0 && cur != 0xFF && rn) {
-                            atomicAdd(&sm.slotT[cur * R + cd], run);
-                            atomicAdd(&sm.slotN[cur * R + cd], rn);
-                        }
-                        cur = rd;
-                        run = 0.0;
-                        rn = 0;
+    // Step-boundary epilogue: per-pair sum of c*log(c) and nonzero bins.
+    auto epilogue = [&](int s, int g) __attribute__((always_inline)) {
+        // One 32x32 block: the lane holds column (lane & 31) and rows
+        // 8*(r/4) + 4*(lane/32) + r%4 for r = 0..15.
+        const bool two_dof = nb >= 28 && !kDump;
+        // the buffer chunk g was multiplied from is free until the next
+        // staging; each wave parks one block's counts there (4 KB) so the
+        // accumulators are read once, straight from AGPRs into LDS
+        int* park = reinterpret_cast<int*>(sm.oh + (size_t)((g & 1) * 2) * kOpBytes) +
+                    wave * 64 * 16;
+        auto epi = [&](const v16i& blk, const int i, const int j) __attribute__((always_inline)) {
+            if (!(mask & (1u << (i * 4 + j)))) return;
+            const int gcol = 128 * wc + 32 * j + (lane & 31);
+            const int cd = sm.col2dof[gcol];
+            if (cd == 0xFF) return;
+            v4i* pk = reinterpret_cast<v4i*>(park) + lane;
+            pk[0] = v4i{blk[0], blk[1], blk[2], blk[3]};
+            pk[64] = v4i{blk[4], blk[5], blk[6], blk[7]};
+            pk[128] = v4i{blk[8], blk[9], blk[10], blk[11]};
+            pk[192] = v4i{blk[12], blk[13], blk[14], blk[15]};
+            asm volatile("" ::: "memory");
+            const int row0 = 128 * wr + 32 * i + 4 * (lane >> 5);
+            if (two_dof) {
+                // nb >= 28: the lane's 16 rows (row0 .. row0+27) touch at most
+                // two DOFs; split them at the first row of the second one.
+                const int dlo = sm.row2dof[row0];
+                if (dlo == 0xFF) return;
+                const int bnd = (dlo + 1) * nb;
+                double slo = 0.0, shi = 0.0;
+                int nlo = 0, nhi = 0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const v4i c4 = pk[64 * q];
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        const int grow = row0 + 8 * q + t;
+                        const double v = a.clogc[c4[t]];
+                        const int nz = c4[t] > 0;
+                        if (grow >= bnd) { shi += v; nhi += nz; }
+                        else { slo += v; nlo += nz; }
                     }
-                    const int cnt = cnts[r];
-                    if (cnt > 0) {
-                        const double dc = (double)cnt;
-                        run += dc * log(dc);
-                        ++rn;
+                }
+                if (nlo) {
+                    atomicAdd(&sm.slotT[dlo * R + cd], slo);
+                    atomicAdd(&sm.slotN[dlo * R + cd], nlo);
+                }
+                const int dhi = sm.row2dof[row0 + 27];
+                if (nhi && dhi != 0xFF && dhi != dlo) {
+                    atomicAdd(&sm.slotT[dhi * R + cd], shi);
+                    atomicAdd(&sm.slotN[dhi * R + cd], nhi);
+                }
+                return;
+            }
+            // general: one LDS atomic per nonzero count
+#pragma unroll 1
+            for (int q = 0; q < 4; ++q) {
+                const v4i c4 = pk[64 * q];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int grow = row0 + 8 * q + t;
+                    const int rd = sm.row2dof[grow];
+                    const int cnt = c4[t];
+                    if (rd != 0xFF && cnt > 0) {
+                        atomicAdd(&sm.slotT[rd * R + cd], a.clogc[cnt]);
+                        atomicAdd(&sm.slotN[rd * R + cd], 1);
                     }
                     if constexpr (kDump) {
                         if (rd != 0xFF) {
                             const long long gi = r0 + rd, gj = c0 + cd;
                             if (gi < gj) {
-                                const long long p = a.pairmap
-                                    ? (long long)a.pairmap[gi * D + gj]
-                                    : tri_index(gi, gj, D);
+                                const long long p = a.pairmap ? (long long)a.pairmap[gi * D + gj]
+                                                              : tri_index(gi, gj, D);
                                 if (p >= 0) {
-                                    const int bi_ = grow - rd * nb;
-                                    const int bj_ = gcol - cd * nb;
+                                    const int bi_ = grow - rd * nb, bj_ = gcol - cd * nb;
                                     a.counts[((size_t)p * S + s) * nb * nb +
                                              (size_t)bi_ * nb + bj_] = (uint32_t)cnt;
                                 }
@@ -276,56 +318,83 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
                         }
                     }
                 }
-                if (cur >= 0 && cur != 0xFF && rn) {
-                    atomicAdd(&sm.slotT[cur * R + cd], run);
-                    atomicAdd(&sm.slotN[cur * R + cd], rn);
-                }
-            };
+            }
+        };
 #define CENTRE_EPI(I, J) \
     epi(acc[I][J], I, J);  \
     __builtin_amdgcn_sched_barrier(0);
-            CENTRE_EPI(0, 0) CENTRE_EPI(0, 1) CENTRE_EPI(0, 2) CENTRE_EPI(0, 3)
-            CENTRE_EPI(1, 0) CENTRE_EPI(1, 1) CENTRE_EPI(1, 2) CENTRE_EPI(1, 3)
-            CENTRE_EPI(2, 0) CENTRE_EPI(2, 1) CENTRE_EPI(2, 2) CENTRE_EPI(2, 3)
-            CENTRE_EPI(3, 0) CENTRE_EPI(3, 1) CENTRE_EPI(3, 2) CENTRE_EPI(3, 3)
+        CENTRE_EPI(0, 0) CENTRE_EPI(0, 1) CENTRE_EPI(0, 2) CENTRE_EPI(0, 3)
+        CENTRE_EPI(1, 0) CENTRE_EPI(1, 1) CENTRE_EPI(1, 2) CENTRE_EPI(1, 3)
+        CENTRE_EPI(2, 0) CENTRE_EPI(2, 1) CENTRE_EPI(2, 2) CENTRE_EPI(2, 3)
+        CENTRE_EPI(3, 0) CENTRE_EPI(3, 1) CENTRE_EPI(3, 2) CENTRE_EPI(3, 3)
 #undef CENTRE_EPI
-            __syncthreads();
-            const double n = (double)(s + 1) * (double)a.ss;
-            const double logn = log(n);
-            for (int q = tid; q < nr * nc; q += 256) {
-                const int rd = q / nc, cd = q - rd * nc;
-                const long long gi = r0 + rd, gj = c0 + cd;
-                if (gi < gj) {
-                    const long long p = a.pairmap ? (long long)a.pairmap[gi * D + gj]
-                                                  : tri_index(gi, gj, D);
-                    if (p >= 0) {
-                        const double T = sm.slotT[rd * R + cd];
-                        const int nz = sm.slotN[rd * R + cd];
-                        const double L = a.dof_L[gi * S + s] + a.dof_L[gj * S + s];
-                        const double area = a.dof_w[gi] * a.dof_w[gj];
-                        const double ml = (logn + log(area)) - T / n + L / n;
-                        const double mm = ml + (double)(unsigned)(nz - 1) / (2.0 * n);
-                        if (a.slot_ml >= 0) a.s2[((size_t)p * a.nest + a.slot_ml) * S + s] = ml;
-                        if (a.slot_mm >= 0) a.s2[((size_t)p * a.nest + a.slot_mm) * S + s] = mm;
-                        if (a.mi) {
-                            const double s1 = a.dof_S1[gi * S + s] + a.dof_S1[gj * S + s];
-                            a.mi[(size_t)p * S + s] = s1 - (a.mi_from_mm ? mm : ml);
-                        }
+        __syncthreads();
+        const double n = (double)(s + 1) * (double)a.ss;
+        const double logn = log(n);
+        for (int q = tid; q < nr * nc; q += 256) {
+            const int rd = q / nc, cd = q - rd * nc;
+            const long long gi = r0 + rd, gj = c0 + cd;
+            if (gi < gj) {
+                const long long p = a.pairmap ? (long long)a.pairmap[gi * D + gj]
+                                              : tri_index(gi, gj, D);
+                if (p >= 0) {
+                    const double T = sm.slotT[rd * R + cd];
+                    const int nz = sm.slotN[rd * R + cd];
+                    const double L = a.dof_L[gi * S + s] + a.dof_L[gj * S + s];
+                    const double area = a.dof_w[gi] * a.dof_w[gj];
+                    const double ml = (logn + log(area)) - T / n + L / n;
+                    const double mm = ml + (double)(unsigned)(nz - 1) / (2.0 * n);
+                    if (a.slot_ml >= 0) a.s2[((size_t)p * a.nest + a.slot_ml) * S + s] = ml;
+                    if (a.slot_mm >= 0) a.s2[((size_t)p * a.nest + a.slot_mm) * S + s] = mm;
+                    if (a.mi) {
+                        const double s1 = a.dof_S1[gi * S + s] + a.dof_S1[gj * S + s];
+                        a.mi[(size_t)p * S + s] = s1 - (a.mi_from_mm ? mm : ml);
                     }
                 }
-                sm.slotT[rd * R + cd] = 0.0;
-                sm.slotN[rd * R + cd] = 0;
             }
+            sm.slotT[rd * R + cd] = 0.0;
+            sm.slotN[rd * R + cd] = 0;
+        }
+        // the parked counts overwrote buffer g&1: clear it and its staged
+        // codes so the next staging into it starts from zeros
+        {
+            uint4* p = reinterpret_cast<uint4*>(sm.oh + (size_t)((g & 1) * 2) * kOpBytes);
+            const uint4 z = make_uint4(0, 0, 0, 0);
+            for (int i = tid; i < 2 * kOpBytes / 16; i += 256) p[i] = z;
+            for (int i = tid; i < 2 * R * kGroups; i += 256)
+                sm.cl[(size_t)(g & 1) * 2 * R * kGroups + i] = 0xFFFFFFFFu;
+        }
+    };
+
+    // ---- main loop ---------------------------------------------------------
+    // iteration g: stage chunk g+1 from codes loaded one iteration earlier,
+    // issue the loads of chunk g+2, multiply chunk g.  The loads complete
+    // while the MFMAs of chunk g drain; the copy at the end waits for them.
+    uint32_t cur[kMaxPer], nxt[kMaxPer];
+    __syncthreads();
+    load_codes(0, cur);
+    write_onehot(0, cur);
+    if (G > 1) load_codes(1, cur);
+    __syncthreads();
+    for (int g = 0; g < G; ++g) {
+        if (g + 1 < G) write_onehot(g + 1, cur);
+        if (g + 2 < G) load_codes(g + 2, nxt);
+        mfma_chunk(g);
+        if (g % nch == nch - 1) {
+            __syncthreads();                 // every wave is done with buffer g&1
+            epilogue(g / nch, g);
         }
         __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kMaxPer; ++k) cur[k] = nxt[k];
     }
 }
 
 }  // namespace
 
 size_t pair_tile_lds_bytes(int R) {
-    size_t a, b, c, d, e;
-    return lds_layout(R, &a, &b, &c, &d, &e);
+    size_t a, b, c, d;
+    return lds_layout(R, &a, &b, &c, &d);
 }
 
 hipError_t launch_pair_tiles(const PairKernelArgs& a, int n_tiles, bool dump,

@@ -47,6 +47,11 @@ _SIGS = {
     "centre_mst": ([P, P, P, P, U64, U32, P, P], I32),
     "centre_dev_segment_sum": ([P, P, P, I32, P], I32),
     "centre_synth_bat": ([U64, U32, U32, U32, U64, P, I32], I32),
+    "centre_dev_alloc": ([P, C.c_size_t, P], I32),
+    "centre_dev_free": ([P, P], I32),
+    "centre_memcpy_h2d": ([P, P, P, C.c_size_t], I32),
+    "centre_memcpy_d2h": ([P, P, P, C.c_size_t], I32),
+    "centre_device_sync": ([P], I32),
 }
 
 _lib = None
@@ -137,6 +142,19 @@ class Context:
 
     def sync(self):
         self.check(self._L.centre_sync(self.h))
+
+    def device_sync(self):
+        self.check(self._L.centre_device_sync(self.h))
+
+    # ---- device memory -------------------------------------------------------
+    def alloc(self, shape, dtype) -> "DevArray":
+        return DevArray(self, shape, dtype)
+
+    def to_device(self, x: np.ndarray) -> "DevArray":
+        x = np.ascontiguousarray(x)
+        d = DevArray(self, x.shape, x.dtype)
+        d.upload(x)
+        return d
 
     def last_pair_kernel_ms(self) -> float:
         return float(self._L.centre_last_pair_kernel_ms(self.h))
@@ -243,6 +261,44 @@ class Context:
         self.check(self._L.centre_dev_segment_sum(self.h, d_x, _p(seg),
                                                   len(seg) - 1, _p(out)))
         return out
+
+
+class DevArray:
+    """A device buffer owned through the C ABI (shape and dtype kept here)."""
+
+    def __init__(self, ctx: Context, shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(int(v) for v in np.atleast_1d(shape)) if np.ndim(shape) else (int(shape),)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        ctx.check(ctx._L.centre_dev_alloc(ctx.h, self.nbytes, C.byref(p)))
+        self.ptr = p.value or 0
+
+    def upload(self, x: np.ndarray):
+        x = np.ascontiguousarray(x, dtype=self.dtype)
+        if x.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        self.ctx.check(self.ctx._L.centre_memcpy_h2d(self.ctx.h, self.ptr, _p(x), x.nbytes))
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        self.ctx.check(self.ctx._L.centre_memcpy_d2h(self.ctx.h, _p(out), self.ptr, self.nbytes))
+        return out
+
+    def offset(self, nbytes: int) -> int:
+        return self.ptr + nbytes
+
+    def free(self):
+        if getattr(self, "ptr", 0) and self.ctx.h:
+            self.ctx._L.centre_dev_free(self.ctx.h, self.ptr)
+        self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class PairPlan:

@@ -161,6 +161,19 @@ int centre_dev_segment_sum(centre_ctx *ctx, const double *d_x,
                            const uint64_t *seg_begin, int n_seg,
                            double *out);
 
+/* ---- device memory plumbing --------------------------------------------------
+ * No reference counterpart: allocation and copies for callers that have no
+ * device allocator of their own (tests, the benchmark, the C++ host).
+ * Copies are synchronous on the context's stream. */
+int centre_dev_alloc(centre_ctx *ctx, size_t bytes, void **out);
+int centre_dev_free(centre_ctx *ctx, void *d_ptr);
+int centre_memcpy_h2d(centre_ctx *ctx, void *d_dst, const void *src,
+                      size_t bytes);
+int centre_memcpy_d2h(centre_ctx *ctx, void *dst, const void *d_src,
+                      size_t bytes);
+/* hipDeviceSynchronize on the context's device. */
+int centre_device_sync(centre_ctx *ctx);
+
 /* ---- synthetic input ---------------------------------------------------------
  * No reference counterpart (test and benchmark input).  Fills out[n_frame]
  * [n_bond + n_angle + n_dih] (host, float64, [frame][dof]) with a seeded

@@ -30,13 +30,6 @@ def ctx():
     c.close()
 
 
-def dev(x):
-    import torch
-    t = torch.from_numpy(np.ascontiguousarray(x)).cuda()
-    torch.cuda.synchronize()
-    return t
-
-
 # ---------------------------------------------------------------------------
 # reference goldens: example-1 codes -> entcontri S1/S2 (30 bins, 4 steps)
 TYPES = {"b": 1, "a": 2, "t": 3}
@@ -74,17 +67,16 @@ def test_example1_pair_entropies_vs_reference(ctx, golden):
 
 
 def test_example1_counts_bit_exact(ctx, golden):
-    import torch
     codes, ext, types, off = ex1_all(golden)
     D, F = codes.shape
     plan = hip.PairPlan(ctx, D, 30)
     P = plan.records
-    dc, dt, dx = dev(codes), dev(types), dev(ext)
-    s2 = torch.zeros((P, 1, 4), dtype=torch.float64, device="cuda")
-    cnt = torch.zeros((P, 4, 900), dtype=torch.int32, device="cuda")
-    ctx.dev_pair_entropy(plan, dc.data_ptr(), F, F, dt.data_ptr(), dx.data_ptr(),
-                         4, [0], True, False, 0, s2.data_ptr(), 0, cnt.data_ptr())
-    got = cnt.cpu().numpy().view(np.uint32)
+    dc, dt, dx = ctx.to_device(codes), ctx.to_device(types), ctx.to_device(ext)
+    s2 = ctx.alloc((P, 1, 4), np.float64)
+    cnt = ctx.alloc((P, 4, 900), np.uint32)
+    ctx.dev_pair_entropy(plan, dc.ptr, F, F, dt.ptr, dx.ptr, 4, [0], True, False,
+                         0, s2.ptr, 0, cnt.ptr)
+    got = cnt.numpy()
     iu, ju = np.triu_indices(D, 1)
     pick = np.random.default_rng(0).choice(P, 300, replace=False)
     want = O.pair_counts(codes, 30, np.stack([iu[pick], ju[pick]], 1), 4)
