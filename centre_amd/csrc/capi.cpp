@@ -438,10 +438,6 @@ extern "C" int centre_dev_pair_entropy(centre_ctx* ctx, const centre_pair_plan* 
     a.mi_from_mm = (ests[0] == 1);
     a.counts = cnt;
     a.flag = ctx->d_flag;
-    DevBuf tab;
-    HIPCHK(tab.alloc(sizeof(double) * (nsteps * ss + 1)));
-    HIPCHK(launch_clogc_table(tab.as<double>(), nsteps * ss + 1, ctx->stream));
-    a.clogc = tab.as<double>();
     // dof_S1 is [D][nest][S]; the kernel reads [D][S] of estimator 0.  With
     // nest > 1 give it a compact copy.
     DevBuf s1e0;

@@ -65,10 +65,9 @@ struct PairKernelArgs {
     double* mi;             // [P][S] or null
     int mi_from_mm;         // MI uses the MM S2 (when MM is the first estimator)
     uint32_t* counts;       // dump mode: [P][S][nb*nb] uint32 prefix counts
-    const double* clogc;    // [frames used + 1] c*log(c), c = 0 -> 0
     int* flag;              // error word: bit0 = code >= nbins
 };
-hipError_t launch_clogc_table(double* t, uint64_t n, hipStream_t st);
+
 
 // Launchers (return hipError_t of the launch).
 hipError_t launch_pair_tiles(const PairKernelArgs& a, int n_tiles, bool dump,

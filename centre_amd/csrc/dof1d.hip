@@ -194,19 +194,7 @@ __global__ void gather_rec_kernel(const double* src, const uint64_t* rec2canon,
     dst[t] = src[rec2canon[r] * width + k];
 }
 
-__global__ void clogc_kernel(double* t, uint64_t n) {
-    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
-    const double x = (double)c;
-    t[c] = c ? x * log(x) : 0.0;
-}
-
 }  // namespace
-
-hipError_t launch_clogc_table(double* t, uint64_t n, hipStream_t st) {
-    hipLaunchKernelGGL(clogc_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, t, n);
-    return hipGetLastError();
-}
 
 hipError_t launch_dof1d(const Dof1dArgs& a, hipStream_t st) {
     if (a.n_dof == 0) return hipSuccess;

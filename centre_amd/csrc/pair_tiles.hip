@@ -23,15 +23,14 @@
 // which is entropy2D's ML sum -sum (c/n) log(c / (n w_i w_j J_i J_j)) with
 // the Jacobian part folded into the marginal sums L_i = sum_b m_ib log J_ib
 // (m = 1-D counts of the same frames).  MM adds (nonzero bins - 1)/(2n).
-// c*log(c) comes from a table indexed by the count (c <= frames used), built
-// once per call, so the epilogue is a gather plus additions.
 //
-// One-hot staging.  Codes of the tile's DOFs are prefetched into registers
-// two chunks ahead, and the one-hot rows are written into LDS by setting
-// one byte per (DOF, frame) and clearing the byte the same slot set two
-// chunks earlier, so a chunk costs a few byte writes per thread instead of
-// rebuilding 32 KB of zeros.  Two LDS buffers alternate: chunk g+1 is
-// staged while chunk g feeds the MFMAs.
+// One-hot staging.  Codes of the tile's DOFs arrive in 256-frame
+// super-chunks with 16-byte loads, issued one super-chunk (four chunks of
+// MFMA work) ahead of use so their latency is hidden.  The one-hot rows are
+// written into LDS by setting one byte per (DOF, frame) and clearing the
+// byte the same slot set two chunks earlier, so a chunk costs a few byte
+// writes per thread instead of rebuilding 32 KB of zeros.  Two LDS buffers
+// alternate: chunk g+1 is staged while chunk g feeds the MFMAs.
 //
 // Waves: 4 per workgroup in a 2x2 layout, each owning a 128x128 quadrant
 // (4x4 MFMA 32x32 blocks = 256 int32 accumulators per lane).  Blocks with no
@@ -46,7 +45,7 @@ namespace {
 
 constexpr int kOpBytes = kTileRows * kLdsRow;   // one operand, one buffer
 constexpr int kGroups = kChunk / 4;             // dword groups per DOF row
-constexpr int kMaxPer = (2 * 60 * kGroups + 255) / 256;  // groups per thread
+constexpr int kSuper = 4;                       // chunks per super-chunk
 
 struct Smem {
     uint8_t* oh;        // [2 buf][2 operand][kTileRows * kLdsRow]
@@ -55,11 +54,13 @@ struct Smem {
     uint8_t* col2dof;   // [256]
     double* slotT;      // [R*R] sum c log c per local pair
     int* slotN;         // [R*R] nonzero bins per local pair
+    double* logt;       // [256] log(1 + j/256)
+    double* invt;       // [256] 1 / (1 + j/256)
 };
 
 __host__ __device__ inline size_t lds_layout(int R, size_t* off_cl,
                                              size_t* off_map, size_t* off_T,
-                                             size_t* off_N) {
+                                             size_t* off_N, size_t* off_log) {
     size_t off = (size_t)2 * 2 * kOpBytes;
     *off_cl = off;
     off += (size_t)2 * 2 * R * kGroups * 4;
@@ -70,19 +71,36 @@ __host__ __device__ inline size_t lds_layout(int R, size_t* off_cl,
     off += sizeof(double) * R * R;
     *off_N = off;
     off += sizeof(int) * R * R;
+    off = (off + 15) & ~(size_t)15;
+    *off_log = off;
+    off += 2 * 256 * sizeof(double);
     return off;
+}
+
+// log(c) for an integer count c >= 1: c = 2^(e-1) * m, m in [1, 2),
+// m = t_j (1 + r) with t_j = 1 + j/256 from a table, log(1 + r) by its Taylor
+// series (|r| < 2^-8, truncation < 2e-18).  About 15 instructions instead of
+// the ~100 of the general FP64 log; accurate to a few ulp.
+__device__ inline double log_count(int c, const double* logt, const double* invt) {
+    int e;
+    const double m2 = 2.0 * frexp((double)c, &e);          // [1, 2), exact
+    const double f = m2 - 1.0;                              // exact
+    const int j = (int)(f * 256.0);
+    const double r = (f - j * (1.0 / 256.0)) * invt[j];
+    const double p = r * (1.0 + r * (-0.5 + r * (1.0 / 3.0 + r * (-0.25 + r * (0.2 + r * (-1.0 / 6.0))))));
+    return (double)(e - 1) * 0.6931471805599453 + logt[j] + p;
 }
 
 __device__ inline long long tri_index(long long i, long long j, long long D) {
     return i * D - i * (i + 1) / 2 + (j - i - 1);
 }
 
-template <bool kDump>
+template <bool kDump, int kSlots>
 __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
     extern __shared__ __align__(16) uint8_t smem_raw[];
     const int R = a.R, nb = a.nb, S = a.nsteps;
-    size_t ocl, omap, oT, oN;
-    lds_layout(R, &ocl, &omap, &oT, &oN);
+    size_t ocl, omap, oT, oN, olog;
+    lds_layout(R, &ocl, &omap, &oT, &oN, &olog);
     Smem sm;
     sm.oh = smem_raw;
     sm.cl = reinterpret_cast<uint32_t*>(smem_raw + ocl);
@@ -90,6 +108,8 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
     sm.col2dof = smem_raw + omap + 256;
     sm.slotT = reinterpret_cast<double*>(smem_raw + oT);
     sm.slotN = reinterpret_cast<int*>(smem_raw + oN);
+    sm.logt = reinterpret_cast<double*>(smem_raw + olog);
+    sm.invt = sm.logt + 256;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1;
@@ -98,7 +118,6 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
     const int nr = min(R, a.n_dof - r0), nc = min(R, a.n_dof - c0);
     const bool diag = tile.x == tile.y;
     const int ngen = diag ? nr : nr + nc;   // diagonal tiles share operands
-    const int nwork = ngen * kGroups;
     const long long D = a.n_dof;
 
     // ---- LDS initialisation ------------------------------------------------
@@ -115,6 +134,11 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
         for (int i = tid; i < R * R; i += 256) {
             sm.slotT[i] = 0.0;
             sm.slotN[i] = 0;
+        }
+        {
+            const double t = 1.0 + tid * (1.0 / 256.0);
+            sm.logt[tid] = log(t);
+            sm.invt[tid] = 1.0 / t;
         }
     }
 
@@ -138,69 +162,85 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
     const bool aligned = ((a.stride & 3) == 0) && ((a.ss & 3) == 0) &&
                          ((reinterpret_cast<uintptr_t>(a.codes) & 3) == 0);
 
-    // Codes of chunk g for this thread's dword groups; 0xFF marks a frame
-    // outside the chunk.
-    auto load_codes = [&](int g, uint32_t (&dst)[kMaxPer]) __attribute__((always_inline)) {
-        const int s = g / nch, c = g - s * nch;
-        const uint64_t f0 = (uint64_t)s * a.ss + (uint64_t)c * kChunk;
-        const int nvalid = (int)min((uint64_t)kChunk, (uint64_t)(s + 1) * a.ss - f0);
+    // ---- code staging --------------------------------------------------------
+    // A staging slot is 4 consecutive frames of one DOF within a chunk: slot
+    // q -> DOF q / kGroups, frames 4 * (q % kGroups) .. +3.  Codes arrive in
+    // super-chunks of kSuper chunks: one dword load per slot and chunk, all
+    // issued kSuper chunks (of MFMA work) before they are needed.  0xFF marks
+    // a frame outside its chunk or beyond the last chunk.
+    auto load_super = [&](int j, uint32_t (&dst)[kSlots][kSuper]) __attribute__((always_inline)) {
 #pragma unroll
-        for (int k = 0; k < kMaxPer; ++k) {
-            const int q = tid + 256 * k;
-            uint32_t w = 0xFFFFFFFFu;
-            if (q < nwork) {
-                const int ld = q / kGroups, k4 = (q - ld * kGroups) * 4;
-                const int dof = ld < nr ? r0 + ld : c0 + (ld - nr);
-                const uint8_t* src = a.codes + (uint64_t)dof * a.stride + f0 + k4;
-                if (aligned && k4 + 4 <= nvalid) {
-                    w = *reinterpret_cast<const uint32_t*>(src);
-                } else {
-                    w = 0;
-                    for (int e = 0; e < 4; ++e)
-                        w |= (k4 + e < nvalid ? (uint32_t)src[e] : 0xFFu) << (8 * e);
-                }
+        for (int c = 0; c < kSuper; ++c) {
+            const int g = j * kSuper + c;
+            int nvalid = 0;
+            uint64_t f0 = 0;
+            if (g < G) {
+                const int s = g / nch, cc = g - s * nch;
+                f0 = (uint64_t)s * a.ss + (uint64_t)cc * kChunk;
+                nvalid = (int)min((uint64_t)kChunk, (uint64_t)(s + 1) * a.ss - f0);
             }
-            dst[k] = w;
-        }
-    };
-    // One-hot rows of chunk g into buffer g&1 from codes loaded earlier.
-    // A code >= nb (including a stored 0xFF on a valid frame) raises the
-    // error flag and is dropped.
-    auto write_onehot = [&](int g, const uint32_t (&cw)[kMaxPer]) __attribute__((always_inline)) {
-        const int b = g & 1;
-        const int s = g / nch, c = g - s * nch;
-        const uint64_t f0 = (uint64_t)s * a.ss + (uint64_t)c * kChunk;
-        const int nvalid = (int)min((uint64_t)kChunk, (uint64_t)(s + 1) * a.ss - f0);
 #pragma unroll
-        for (int k = 0; k < kMaxPer; ++k) {
-            const int q = tid + 256 * k;
-            if (q < nwork) {
+            for (int k = 0; k < kSlots; ++k) {
+                const int q = tid + 256 * k;
                 const int ld = q / kGroups, k4 = (q - ld * kGroups) * 4;
-                const bool isA = ld < nr;
-                const int rbase = (isA ? ld : ld - nr) * nb;
-                uint8_t* op = sm.oh + (size_t)(b * 2 + (isA ? 0 : 1)) * kOpBytes;
-                uint32_t cv[4];
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    cv[e] = (cw[k] >> (8 * e)) & 0xFFu;
-                    if (k4 + e >= nvalid) {
-                        cv[e] = 0xFFu;
-                    } else if (cv[e] >= (uint32_t)nb) {
-                        atomicOr(a.flag, 1);
-                        cv[e] = 0xFFu;
+                uint32_t w = 0xFFFFFFFFu;
+                if (ld < ngen && k4 < nvalid) {
+                    const int dof = ld < nr ? r0 + ld : c0 + (ld - nr);
+                    const uint8_t* src = a.codes + (uint64_t)dof * a.stride + f0 + k4;
+                    if (aligned && k4 + 4 <= nvalid) {
+                        w = *reinterpret_cast<const uint32_t*>(src);
+                    } else {
+                        w = 0;
+                        for (int e = 0; e < 4; ++e)
+                            w |= (k4 + e < nvalid ? (uint32_t)src[e] : 0xFFu) << (8 * e);
                     }
                 }
+                dst[k][c] = w;
+            }
+        }
+    };
+    // One-hot rows of chunk g into buffer g&1 from the dwords of its
+    // super-chunk.  A code >= nb raises the error flag and is dropped.
+    auto write_onehot = [&](int g, const uint32_t (&cw)[kSlots][kSuper]) __attribute__((always_inline)) {
+        const int b = g & 1;
+        const int cis = g - (g / kSuper) * kSuper;   // chunk within super-chunk
+        const int sg = g / nch, cg = g - sg * nch;
+        const int nvalid = (int)min((uint64_t)kChunk,
+                                    (uint64_t)(sg + 1) * a.ss - ((uint64_t)sg * a.ss + (uint64_t)cg * kChunk));
+#pragma unroll
+        for (int k = 0; k < kSlots; ++k) {
+            const int q = tid + 256 * k;
+            const int ld = q / kGroups, k4 = (q - ld * kGroups) * 4;
+            if (ld < ngen) {
+                // select dword cis with masks (a select chain on a runtime
+                // index would be turned into an indexed scratch access)
+                uint32_t wv = 0;
+#pragma unroll
+                for (int c = 0; c < kSuper; ++c) wv |= cw[k][c] & (0u - (uint32_t)(cis == c));
+                const bool isA = ld < nr;
+                const int rbase = (isA ? ld : ld - nr) * nb;
+                uint8_t* op = sm.oh + (size_t)(b * 2 + (isA ? 0 : 1)) * kOpBytes + k4;
                 uint32_t* slot = sm.cl + ((size_t)b * 2 * R + ld) * kGroups + (k4 >> 2);
                 const uint32_t old = *slot;
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const uint32_t ob = (old >> (8 * e)) & 0xFFu;
-                    if (ob != 0xFFu) op[(rbase + ob) * kLdsRow + k4 + e] = 0;
+                    if (ob != 0xFFu) op[(rbase + ob) * kLdsRow + e] = 0;
                 }
+                bool bad = false;
 #pragma unroll
-                for (int e = 0; e < 4; ++e)
-                    if (cv[e] != 0xFFu) op[(rbase + cv[e]) * kLdsRow + k4 + e] = 1;
-                *slot = cv[0] | (cv[1] << 8) | (cv[2] << 16) | (cv[3] << 24);
+                for (int e = 0; e < 4; ++e) {
+                    const uint32_t cv = (wv >> (8 * e)) & 0xFFu;
+                    if (k4 + e >= nvalid) continue;          // frame outside the chunk
+                    if (cv >= (uint32_t)nb) {                // includes a stored 0xFF
+                        bad = true;
+                        wv |= 0xFFu << (8 * e);
+                    } else {
+                        op[(rbase + cv) * kLdsRow + e] = 1;
+                    }
+                }
+                if (bad) atomicOr(a.flag, 1);
+                *slot = wv;
             }
         }
     };
@@ -211,30 +251,28 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = v16i{};
 
-    auto mfma_chunk = [&](int g) __attribute__((always_inline)) {
+    // MFMA half kk (32 frames) of chunk g from buffer g&1.
+    auto mfma_half = [&](int g, int kk) __attribute__((always_inline)) {
         if (!mask) return;
         const uint8_t* A = sm.oh + (size_t)((g & 1) * 2) * kOpBytes;
         const uint8_t* B = diag ? A : A + kOpBytes;
+        const int koff = kk * 32 + (lane >> 5) * 16;
+        v4i af[4], bf[4];
 #pragma unroll
-        for (int kk = 0; kk < kChunk / 32; ++kk) {
-            const int koff = kk * 32 + (lane >> 5) * 16;
-            v4i af[4], bf[4];
+        for (int i = 0; i < 4; ++i)
+            af[i] = *reinterpret_cast<const v4i*>(
+                A + (128 * wr + 32 * i + (lane & 31)) * kLdsRow + koff);
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-                af[i] = *reinterpret_cast<const v4i*>(
-                    A + (128 * wr + 32 * i + (lane & 31)) * kLdsRow + koff);
+        for (int j = 0; j < 4; ++j)
+            bf[j] = *reinterpret_cast<const v4i*>(
+                B + (128 * wc + 32 * j + (lane & 31)) * kLdsRow + koff);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int j = 0; j < 4; ++j)
-                bf[j] = *reinterpret_cast<const v4i*>(
-                    B + (128 * wc + 32 * j + (lane & 31)) * kLdsRow + koff);
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    if (mask & (1u << (i * 4 + j)))
-                        acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
-                            af[i], bf[j], acc[i][j], 0, 0, 0);
-        }
+                if (mask & (1u << (i * 4 + j)))
+                    acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
+                        af[i], bf[j], acc[i][j], 0, 0, 0);
     };
 
     // Step-boundary epilogue: per-pair sum of c*log(c) and nonzero bins.
@@ -273,7 +311,8 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
 #pragma unroll
                     for (int t = 0; t < 4; ++t) {
                         const int grow = row0 + 8 * q + t;
-                        const double v = a.clogc[c4[t]];
+                        const int cn = c4[t];
+                        const double v = (double)cn * log_count(max(cn, 1), sm.logt, sm.invt);
                         const int nz = c4[t] > 0;
                         if (grow >= bnd) { shi += v; nhi += nz; }
                         else { slo += v; nlo += nz; }
@@ -300,7 +339,8 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
                     const int rd = sm.row2dof[grow];
                     const int cnt = c4[t];
                     if (rd != 0xFF && cnt > 0) {
-                        atomicAdd(&sm.slotT[rd * R + cd], a.clogc[cnt]);
+                        atomicAdd(&sm.slotT[rd * R + cd],
+                                  (double)cnt * log_count(cnt, sm.logt, sm.invt));
                         atomicAdd(&sm.slotN[rd * R + cd], 1);
                     }
                     if constexpr (kDump) {
@@ -367,52 +407,70 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
     };
 
     // ---- main loop ---------------------------------------------------------
-    // iteration g: stage chunk g+1 from codes loaded one iteration earlier,
-    // issue the loads of chunk g+2, multiply chunk g.  The loads complete
-    // while the MFMAs of chunk g drain; the copy at the end waits for them.
-    uint32_t cur[kMaxPer], nxt[kMaxPer];
+    // iteration g: multiply the first half of chunk g, stage chunk g+1 while
+    // those MFMAs run, multiply the second half.  The super-chunk holding
+    // chunk g+1 was loaded kSuper iterations earlier; the loads of the next
+    // super-chunk are issued as soon as the current one is taken over.
+    uint32_t cur[kSlots][kSuper], nxt[kSlots][kSuper];
     __syncthreads();
-    load_codes(0, cur);
+    load_super(0, cur);
+    if (G > kSuper) load_super(1, nxt);
     write_onehot(0, cur);
-    if (G > 1) load_codes(1, cur);
     __syncthreads();
     for (int g = 0; g < G; ++g) {
-        if (g + 1 < G) write_onehot(g + 1, cur);
-        if (g + 2 < G) load_codes(g + 2, nxt);
-        mfma_chunk(g);
+        mfma_half(g, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        if (g + 1 < G) {
+            if ((g + 1) % kSuper == 0) {
+#pragma unroll
+                for (int k = 0; k < kSlots; ++k)
+#pragma unroll
+                    for (int c = 0; c < kSuper; ++c) cur[k][c] = nxt[k][c];
+                const int jn = (g + 1) / kSuper + 1;
+                if (jn * kSuper < G) load_super(jn, nxt);
+            }
+            write_onehot(g + 1, cur);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        mfma_half(g, 1);
         if (g % nch == nch - 1) {
             __syncthreads();                 // every wave is done with buffer g&1
             epilogue(g / nch, g);
         }
         __syncthreads();
-#pragma unroll
-        for (int k = 0; k < kMaxPer; ++k) cur[k] = nxt[k];
     }
 }
 
 }  // namespace
 
 size_t pair_tile_lds_bytes(int R) {
-    size_t a, b, c, d;
-    return lds_layout(R, &a, &b, &c, &d);
+    size_t a, b, c, d, e;
+    return lds_layout(R, &a, &b, &c, &d, &e);
+}
+
+template <bool kDump, int kSlots>
+static hipError_t launch_t(const PairKernelArgs& a, int n_tiles, size_t lds, hipStream_t st) {
+    hipError_t e = hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&pair_tile_kernel<kDump, kSlots>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((pair_tile_kernel<kDump, kSlots>), dim3(n_tiles), dim3(256), lds, st, a);
+    return hipGetLastError();
+}
+
+template <bool kDump>
+static hipError_t launch_d(const PairKernelArgs& a, int n_tiles, size_t lds, hipStream_t st) {
+    // staging slots per thread: 2R DOFs x kGroups slots over 256 threads
+    const int need = (2 * a.R * kGroups + 255) / 256;
+    if (need <= 1) return launch_t<kDump, 1>(a, n_tiles, lds, st);
+    if (need <= 2) return launch_t<kDump, 2>(a, n_tiles, lds, st);
+    if (need <= 4) return launch_t<kDump, 4>(a, n_tiles, lds, st);
+    return launch_t<kDump, 8>(a, n_tiles, lds, st);
 }
 
 hipError_t launch_pair_tiles(const PairKernelArgs& a, int n_tiles, bool dump,
                              hipStream_t st) {
     if (n_tiles <= 0) return hipSuccess;
     const size_t lds = pair_tile_lds_bytes(a.R);
-    if (dump) {
-        hipError_t e = hipFuncSetAttribute(
-            reinterpret_cast<const void*>(&pair_tile_kernel<true>),
-            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(pair_tile_kernel<true>, dim3(n_tiles), dim3(256), lds, st, a);
-    } else {
-        hipError_t e = hipFuncSetAttribute(
-            reinterpret_cast<const void*>(&pair_tile_kernel<false>),
-            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(pair_tile_kernel<false>, dim3(n_tiles), dim3(256), lds, st, a);
-    }
-    return hipGetLastError();
+    return dump ? launch_d<true>(a, n_tiles, lds, st) : launch_d<false>(a, n_tiles, lds, st);
 }

@@ -170,9 +170,10 @@ def test_explicit_pairs_order_and_duplicates(ctx, cfg1):
     assert rel_err(s2[:, 0, :], w2[:, 0, 0, :]) < REL
 
 
-def test_code_out_of_range_is_an_error(ctx):
-    codes = np.zeros((4, 100), np.uint8)
-    codes[2, 50] = 30
+@pytest.mark.parametrize("bad,F", [(30, 100), (255, 100), (255, 4096), (31, 4096)])
+def test_code_out_of_range_is_an_error(ctx, bad, F):
+    codes = np.zeros((4, F), np.uint8)
+    codes[2, F // 2] = bad
     with pytest.raises(hip.CentreError) as e:
         ctx.pair_entropy(codes, np.ones(4, np.uint8), np.tile([1.0, 2.0, 0, 0], (4, 1)), 30, 2)
     assert e.value.code == -4
