@@ -65,7 +65,7 @@ __host__ __device__ inline size_t lds_layout(int R, size_t* off_cl,
     *off_cl = off;
     off += (size_t)2 * 2 * R * kGroups * 4;
     *off_map = off;
-    off += 512;
+    off += 512 + 16;   // row/col DOF maps, then a 16-byte trash word
     off = (off + 15) & ~(size_t)15;
     *off_T = off;
     off += sizeof(double) * R * R;
@@ -111,7 +111,8 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
     sm.logt = reinterpret_cast<double*>(smem_raw + olog);
     sm.invt = sm.logt + 256;
 
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform
     const int wr = wave >> 1, wc = wave & 1;
     const int2 tile = a.tiles[blockIdx.x];
     const int r0 = tile.x * R, c0 = tile.y * R;
@@ -406,7 +407,169 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
         }
     };
 
-    // ---- main loop ---------------------------------------------------------
+    // ---- fast path -----------------------------------------------------------
+    // Off-diagonal tiles with R DOFs on both sides and 4-byte aligned codes
+    // (the bulk of the work) run a branch-free loop: all 16 MFMA blocks of a
+    // wave unconditionally (blocks of padding rows multiply zeros and are
+    // skipped by the epilogue), byte writes that have nothing to do go to a
+    // trash word instead of being predicated, and the codes a slot staged
+    // two chunks ago stay in registers.  The loop body is then one basic
+    // block, so the scheduler can spread the staging of chunk g+1 over the
+    // issue slots between the MFMAs of chunk g.
+    bool done = false;
+    uint32_t bad_acc = 0;
+    if constexpr (!kDump && kSlots <= 4) {
+        const bool fast = !diag && nr == R && nc == R && aligned;
+        if (fast) {
+            done = true;
+            uint8_t* const trash = smem_raw + omap + 512;
+            // slot k of this thread: DOF ld = tid/16 + 16k, frames 4*(tid%16)..+3
+            const int k4 = (tid & 15) * 4;
+            const uint8_t* src[kSlots];
+            uint32_t dst[kSlots];
+            bool live[kSlots];
+#pragma unroll
+            for (int k = 0; k < kSlots; ++k) {
+                const int ld = (tid >> 4) + 16 * k;
+                live[k] = ld < 2 * R;
+                const bool isA = ld < R;
+                const int dof = isA ? r0 + ld : c0 + (ld - R);
+                src[k] = a.codes + (live[k] ? (uint64_t)dof * a.stride + k4 : 0);
+                dst[k] = (uint32_t)((isA ? 0 : kOpBytes) + (isA ? ld : ld - R) * nb * kLdsRow + k4);
+            }
+            uint32_t cur[kSlots][kSuper], nxt[kSlots][kSuper], last0[kSlots], last1[kSlots];
+#pragma unroll
+            for (int k = 0; k < kSlots; ++k) last0[k] = last1[k] = 0xFFFFFFFFu;
+            auto advance = [](int& s, int& c, int n) __attribute__((always_inline)) {
+                if (++c == n) { c = 0; ++s; }
+            };
+            auto nvalid_of = [&](int s, int c) __attribute__((always_inline)) -> int {
+                return s < S ? (int)min((uint64_t)kChunk, a.ss - (uint64_t)c * kChunk) : 0;
+            };
+            int ls = 0, lc = 0;   // next chunk to load
+            auto load4 = [&](uint32_t (&d)[kSlots][kSuper]) __attribute__((always_inline)) {
+#pragma unroll
+                for (int c = 0; c < kSuper; ++c) {
+                    const uint64_t f0 = (uint64_t)ls * a.ss + (uint64_t)lc * kChunk;
+                    const int nv = nvalid_of(ls, lc);
+#pragma unroll
+                    for (int k = 0; k < kSlots; ++k) {
+                        // frames outside the chunk read a harmless in-bounds word;
+                        // staging ignores them by position
+                        const bool ok = live[k] & (k4 < nv);
+                        d[k][c] = *reinterpret_cast<const uint32_t*>(ok ? src[k] + f0 : a.codes);
+                    }
+                    advance(ls, lc, nch);
+                }
+            };
+            // chunk with nv valid frames from dword cis of cw into buffer b
+            auto stage = [&](int b, int cis, int nv, const uint32_t (&cw)[kSlots][kSuper])
+                __attribute__((always_inline)) {
+                uint8_t* const buf = sm.oh + (size_t)b * 2 * kOpBytes;
+#pragma unroll
+                for (int k = 0; k < kSlots; ++k) {
+                    uint32_t wv = 0;
+#pragma unroll
+                    for (int c = 0; c < kSuper; ++c) wv |= cw[k][c] & (0u - (uint32_t)(cis == c));
+                    const uint32_t old = b ? last1[k] : last0[k];
+                    uint8_t* const row = buf + dst[k];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const uint32_t ob = (old >> (8 * e)) & 0xFFu;
+                        uint8_t* const p = ob != 0xFFu ? row + ob * kLdsRow : trash;
+                        p[e] = 0;
+                    }
+                    uint32_t stored = 0;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        // bitwise ops on purpose: short-circuit forms become branches
+                        const uint32_t cv = (wv >> (8 * e)) & 0xFFu;
+                        const uint32_t in = (uint32_t)live[k] & (uint32_t)(k4 + e < nv);
+                        const uint32_t ok = in & (uint32_t)(cv < (uint32_t)nb);
+                        bad_acc |= in & ~ok;
+                        uint8_t* const p = ok ? row + cv * kLdsRow : trash;
+                        p[e] = 1;
+                        stored |= (ok ? cv : 0xFFu) << (8 * e);
+                    }
+                    last1[k] = b ? stored : last1[k];
+                    last0[k] = b ? last0[k] : stored;
+                }
+            };
+            auto frags = [&](const uint8_t* base, int kk, v4i (&af)[4], v4i (&bf)[4])
+                __attribute__((always_inline)) {
+                const int koff = kk * 32 + (lane >> 5) * 16;
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    af[i] = *reinterpret_cast<const v4i*>(
+                        base + (128 * wr + 32 * i + (lane & 31)) * kLdsRow + koff);
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    bf[j] = *reinterpret_cast<const v4i*>(
+                        base + kOpBytes + (128 * wc + 32 * j + (lane & 31)) * kLdsRow + koff);
+            };
+            auto mfma16 = [&](const v4i (&af)[4], const v4i (&bf)[4]) __attribute__((always_inline)) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
+            };
+
+            __syncthreads();
+            load4(cur);
+            load4(nxt);
+            stage(0, 0, nvalid_of(0, 0), cur);
+            __syncthreads();
+            int ms = 0, mc = 0;   // chunk g = ms * nch + mc
+            for (int g = 0; g < G; ++g) {
+                const int b = g & 1;
+                if (((g + 1) & (kSuper - 1)) == 0) {
+#pragma unroll
+                    for (int k = 0; k < kSlots; ++k)
+#pragma unroll
+                        for (int c = 0; c < kSuper; ++c) cur[k][c] = nxt[k][c];
+                    load4(nxt);
+                }
+                int s1 = ms, c1 = mc;
+                advance(s1, c1, nch);
+                const int nv1 = nvalid_of(s1, c1);
+                const uint8_t* base = sm.oh + (size_t)b * 2 * kOpBytes;
+                v4i a0[4], b0[4], a1[4], b1[4];
+                frags(base, 0, a0, b0);
+                frags(base, 1, a1, b1);
+                mfma16(a0, b0);
+                stage(b ^ 1, (g + 1) & (kSuper - 1), nv1, cur);
+                mfma16(a1, b1);
+                // fragment reads first, then one MFMA at a time with a share
+                // of the staging VALU work and one LDS write in its shadow
+                __builtin_amdgcn_sched_group_barrier(0x100, 16, 0);
+#pragma unroll
+                for (int i = 0; i < 32; ++i) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                    __builtin_amdgcn_sched_group_barrier(0x002, 2 * kSlots, 0);
+                    __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
+                }
+                if (mc == nch - 1) {
+                    __syncthreads();            // every wave is done with buffer b
+                    epilogue(ms, g);
+#pragma unroll
+                    for (int k = 0; k < kSlots; ++k) {
+                        last1[k] = b ? 0xFFFFFFFFu : last1[k];
+                        last0[k] = b ? last0[k] : 0xFFFFFFFFu;
+                    }
+                }
+                ms = s1;
+                mc = c1;
+                __syncthreads();
+            }
+        }
+    }
+    if (done) {
+        if (bad_acc) atomicOr(a.flag, 1);
+        return;
+    }
+
+    // ---- general path ------------------------------------------------------
     // iteration g: multiply the first half of chunk g, stage chunk g+1 while
     // those MFMAs run, multiply the second half.  The super-chunk holding
     // chunk g+1 was loaded kSuper iterations earlier; the loads of the next
