@@ -52,29 +52,47 @@ struct Smem {
     uint32_t* cl;       // [2 buf][2R DOFs][kGroups] codes staged per slot
     uint8_t* row2dof;   // [256] local row DOF (0xFF = padding)
     uint8_t* col2dof;   // [256]
-    double* slotT;      // [R*R] sum c log c per local pair
-    int* slotN;         // [R*R] nonzero bins per local pair
+    double* slotT;      // [R*R][nstripe] sum c log c per local pair
+    int* slotN;         // [R*R][nstripe] nonzero bins per local pair
     double* logt;       // [256] log(1 + j/256)
     double* invt;       // [256] 1 / (1 + j/256)
+    double* ctab;       // [ntab] c log c for small counts
 };
 
-__host__ __device__ inline size_t lds_layout(int R, size_t* off_cl,
-                                             size_t* off_map, size_t* off_T,
-                                             size_t* off_N, size_t* off_log) {
+constexpr int kTab = 4096;                 // c log c table entries when LDS allows
+constexpr size_t kLdsCap = 160 * 1024;
+
+struct LdsLayout {
+    size_t cl, map, T, N, log, park, tab, total;
+    int nstripe;   // copies of each pair slot, picked by lane % nstripe, so
+                   // that the lanes of one wave rarely add to the same word
+    int ntab;      // 0 (no table) or kTab
+};
+
+__host__ __device__ inline LdsLayout lds_layout(int R) {
+    LdsLayout L;
     size_t off = (size_t)2 * 2 * kOpBytes;
-    *off_cl = off;
+    L.cl = off;
     off += (size_t)2 * 2 * R * kGroups * 4;
-    *off_map = off;
-    off += 512 + 16;   // row/col DOF maps, then a 16-byte trash word
+    L.map = off;
+    off += 512 + kLdsRow;   // row/col DOF maps, then a trash row
     off = (off + 15) & ~(size_t)15;
-    *off_T = off;
-    off += sizeof(double) * R * R;
-    *off_N = off;
-    off += sizeof(int) * R * R;
+    L.nstripe = R <= 8 ? 16 : (R <= 16 ? 4 : 1);
+    L.T = off;
+    off += sizeof(double) * R * R * L.nstripe;
+    L.N = off;
+    off += sizeof(int) * R * R * L.nstripe;
     off = (off + 15) & ~(size_t)15;
-    *off_log = off;
+    L.log = off;
     off += 2 * 256 * sizeof(double);
-    return off;
+    // epilogue parking area of the fast path (R <= 32, i.e. <= 4 staging slots)
+    L.park = off;
+    if (R <= 32) off += (size_t)4 * 64 * 16 * sizeof(int);
+    L.tab = off;
+    L.ntab = off + sizeof(double) * kTab <= kLdsCap ? kTab : 0;
+    off += sizeof(double) * L.ntab;
+    L.total = off;
+    return L;
 }
 
 // log(c) for an integer count c >= 1: c = 2^(e-1) * m, m in [1, 2),
@@ -99,17 +117,19 @@ template <bool kDump, int kSlots>
 __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
     extern __shared__ __align__(16) uint8_t smem_raw[];
     const int R = a.R, nb = a.nb, S = a.nsteps;
-    size_t ocl, omap, oT, oN, olog;
-    lds_layout(R, &ocl, &omap, &oT, &oN, &olog);
+    const LdsLayout lay = lds_layout(R);
+    const size_t omap = lay.map;
+    const int nstr = lay.nstripe, ntab = lay.ntab;
     Smem sm;
     sm.oh = smem_raw;
-    sm.cl = reinterpret_cast<uint32_t*>(smem_raw + ocl);
+    sm.cl = reinterpret_cast<uint32_t*>(smem_raw + lay.cl);
     sm.row2dof = smem_raw + omap;
     sm.col2dof = smem_raw + omap + 256;
-    sm.slotT = reinterpret_cast<double*>(smem_raw + oT);
-    sm.slotN = reinterpret_cast<int*>(smem_raw + oN);
-    sm.logt = reinterpret_cast<double*>(smem_raw + olog);
+    sm.slotT = reinterpret_cast<double*>(smem_raw + lay.T);
+    sm.slotN = reinterpret_cast<int*>(smem_raw + lay.N);
+    sm.logt = reinterpret_cast<double*>(smem_raw + lay.log);
     sm.invt = sm.logt + 256;
+    sm.ctab = reinterpret_cast<double*>(smem_raw + lay.tab);
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform
@@ -132,7 +152,7 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
             sm.row2dof[i] = (uint8_t)(d < nr ? d : 0xFF);
             sm.col2dof[i] = (uint8_t)(d < nc ? d : 0xFF);
         }
-        for (int i = tid; i < R * R; i += 256) {
+        for (int i = tid; i < R * R * nstr; i += 256) {
             sm.slotT[i] = 0.0;
             sm.slotN[i] = 0;
         }
@@ -141,6 +161,7 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
             sm.logt[tid] = log(t);
             sm.invt[tid] = 1.0 / t;
         }
+        for (int c = tid; c < ntab; c += 256) sm.ctab[c] = c > 1 ? (double)c * log((double)c) : 0.0;
     }
 
     // ---- wave-uniform mask of MFMA blocks holding at least one wanted pair --
@@ -277,59 +298,81 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
     };
 
     // Step-boundary epilogue: per-pair sum of c*log(c) and nonzero bins.
-    auto epilogue = [&](int s, int g) __attribute__((always_inline)) {
+    // park: 4 KB of LDS per wave where one block's counts are parked so the
+    // accumulators are read once, straight from AGPRs; with rezero_g >= 0 the
+    // park area lies in buffer rezero_g & 1, which is cleared afterwards.
+    auto epilogue = [&](int s, int* park, int rezero_g) __attribute__((always_inline)) {
         // One 32x32 block: the lane holds column (lane & 31) and rows
         // 8*(r/4) + 4*(lane/32) + r%4 for r = 0..15.
         const bool two_dof = nb >= 28 && !kDump;
-        // the buffer chunk g was multiplied from is free until the next
-        // staging; each wave parks one block's counts there (4 KB) so the
-        // accumulators are read once, straight from AGPRs into LDS
-        int* park = reinterpret_cast<int*>(sm.oh + (size_t)((g & 1) * 2) * kOpBytes) +
-                    wave * 64 * 16;
         auto epi = [&](const v16i& blk, const int i, const int j) __attribute__((always_inline)) {
-            if (!(mask & (1u << (i * 4 + j)))) return;
+            if (!(mask & (1u << (i * 4 + j)))) return;   // wave-uniform
             const int gcol = 128 * wc + 32 * j + (lane & 31);
             const int cd = sm.col2dof[gcol];
-            if (cd == 0xFF) return;
+            const int row0 = 128 * wr + 32 * i + 4 * (lane >> 5);
+            // park the block's counts (4 KB per wave) and read them back: the
+            // accumulators leave the AGPRs in four 16-byte stores per lane
             v4i* pk = reinterpret_cast<v4i*>(park) + lane;
             pk[0] = v4i{blk[0], blk[1], blk[2], blk[3]};
             pk[64] = v4i{blk[4], blk[5], blk[6], blk[7]};
             pk[128] = v4i{blk[8], blk[9], blk[10], blk[11]};
             pk[192] = v4i{blk[12], blk[13], blk[14], blk[15]};
             asm volatile("" ::: "memory");
-            const int row0 = 128 * wr + 32 * i + 4 * (lane >> 5);
             if (two_dof) {
                 // nb >= 28: the lane's 16 rows (row0 .. row0+27) touch at most
                 // two DOFs; split them at the first row of the second one.
                 const int dlo = sm.row2dof[row0];
-                if (dlo == 0xFF) return;
-                const int bnd = (dlo + 1) * nb;
-                double slo = 0.0, shi = 0.0;
-                int nlo = 0, nhi = 0;
+                const bool ok = (cd != 0xFF) & (dlo != 0xFF);
+                const int kk = (dlo + 1) * nb - row0;   // offsets >= kk: second DOF
+                int c[16];
+                int mx = 0;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const v4i c4 = pk[64 * q];
 #pragma unroll
                     for (int t = 0; t < 4; ++t) {
-                        const int grow = row0 + 8 * q + t;
-                        const int cn = c4[t];
-                        const double v = (double)cn * log_count(max(cn, 1), sm.logt, sm.invt);
-                        const int nz = c4[t] > 0;
-                        if (grow >= bnd) { shi += v; nhi += nz; }
-                        else { slo += v; nlo += nz; }
+                        c[4 * q + t] = ok ? c4[t] : 0;
+                        mx = max(mx, c[4 * q + t]);
                     }
                 }
-                if (nlo) {
-                    atomicAdd(&sm.slotT[dlo * R + cd], slo);
-                    atomicAdd(&sm.slotN[dlo * R + cd], nlo);
+                double tot = 0.0, hi = 0.0;
+                int ntot = 0, nhi = 0;
+                auto add = [&](int r, double v) __attribute__((always_inline)) {
+                    const bool h = 8 * (r >> 2) + (r & 3) >= kk;
+                    const int nz = min(c[r], 1);
+                    tot += v;
+                    hi += h ? v : 0.0;
+                    ntot += nz;
+                    nhi += h ? nz : 0;
+                };
+                // counts below the table size (the usual case) are one LDS
+                // read each, all issued before use; a wave holding a larger
+                // count computes its 16 terms by the series
+                if (__builtin_amdgcn_ballot_w64(mx >= ntab) == 0) {
+                    double tv[16];
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) tv[r] = sm.ctab[c[r]];
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) add(r, tv[r]);
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 16; ++r)
+                        add(r, (double)c[r] * log_count(max(c[r], 1), sm.logt, sm.invt));
+                }
+                if (!ok) return;
+                const int stripe = lane % nstr;
+                if (ntot > nhi) {
+                    atomicAdd(&sm.slotT[(dlo * R + cd) * nstr + stripe], tot - hi);
+                    atomicAdd(&sm.slotN[(dlo * R + cd) * nstr + stripe], ntot - nhi);
                 }
                 const int dhi = sm.row2dof[row0 + 27];
                 if (nhi && dhi != 0xFF && dhi != dlo) {
-                    atomicAdd(&sm.slotT[dhi * R + cd], shi);
-                    atomicAdd(&sm.slotN[dhi * R + cd], nhi);
+                    atomicAdd(&sm.slotT[(dhi * R + cd) * nstr + stripe], hi);
+                    atomicAdd(&sm.slotN[(dhi * R + cd) * nstr + stripe], nhi);
                 }
                 return;
             }
+            if (cd == 0xFF) return;
             // general: one LDS atomic per nonzero count
 #pragma unroll 1
             for (int q = 0; q < 4; ++q) {
@@ -340,9 +383,10 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
                     const int rd = sm.row2dof[grow];
                     const int cnt = c4[t];
                     if (rd != 0xFF && cnt > 0) {
-                        atomicAdd(&sm.slotT[rd * R + cd],
-                                  (double)cnt * log_count(cnt, sm.logt, sm.invt));
-                        atomicAdd(&sm.slotN[rd * R + cd], 1);
+                        const double v = cnt < ntab ? sm.ctab[cnt]
+                                                    : (double)cnt * log_count(cnt, sm.logt, sm.invt);
+                        atomicAdd(&sm.slotT[(rd * R + cd) * nstr + lane % nstr], v);
+                        atomicAdd(&sm.slotN[(rd * R + cd) * nstr + lane % nstr], 1);
                     }
                     if constexpr (kDump) {
                         if (rd != 0xFF) {
@@ -379,8 +423,12 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
                 const long long p = a.pairmap ? (long long)a.pairmap[gi * D + gj]
                                               : tri_index(gi, gj, D);
                 if (p >= 0) {
-                    const double T = sm.slotT[rd * R + cd];
-                    const int nz = sm.slotN[rd * R + cd];
+                    double T = 0.0;
+                    int nz = 0;
+                    for (int k = 0; k < nstr; ++k) {
+                        T += sm.slotT[(rd * R + cd) * nstr + k];
+                        nz += sm.slotN[(rd * R + cd) * nstr + k];
+                    }
                     const double L = a.dof_L[gi * S + s] + a.dof_L[gj * S + s];
                     const double area = a.dof_w[gi] * a.dof_w[gj];
                     const double ml = (logn + log(area)) - T / n + L / n;
@@ -393,12 +441,15 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
                     }
                 }
             }
-            sm.slotT[rd * R + cd] = 0.0;
-            sm.slotN[rd * R + cd] = 0;
+            for (int k = 0; k < nstr; ++k) {
+                sm.slotT[(rd * R + cd) * nstr + k] = 0.0;
+                sm.slotN[(rd * R + cd) * nstr + k] = 0;
+            }
         }
-        // the parked counts overwrote buffer g&1: clear it and its staged
-        // codes so the next staging into it starts from zeros
-        {
+        // parked counts overwrote buffer g&1: clear it and its staged codes
+        // so the next staging into it starts from zeros
+        if (rezero_g >= 0) {
+            const int g = rezero_g;
             uint4* p = reinterpret_cast<uint4*>(sm.oh + (size_t)((g & 1) * 2) * kOpBytes);
             const uint4 z = make_uint4(0, 0, 0, 0);
             for (int i = tid; i < 2 * kOpBytes / 16; i += 256) p[i] = z;
@@ -412,34 +463,47 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
     // (the bulk of the work) run a branch-free loop: all 16 MFMA blocks of a
     // wave unconditionally (blocks of padding rows multiply zeros and are
     // skipped by the epilogue), byte writes that have nothing to do go to a
-    // trash word instead of being predicated, and the codes a slot staged
-    // two chunks ago stay in registers.  The loop body is then one basic
-    // block, so the scheduler can spread the staging of chunk g+1 over the
-    // issue slots between the MFMAs of chunk g.
+    // trash row instead of being predicated, and the codes a slot staged
+    // two chunks ago stay in registers.  The loop body is straight-line code,
+    // so the scheduler can spread the staging over the issue slots between
+    // MFMAs, and the fragment reads of the next chunk are software-pipelined
+    // behind the MFMAs of the current one.
     bool done = false;
-    uint32_t bad_acc = 0;
     if constexpr (!kDump && kSlots <= 4) {
-        const bool fast = !diag && nr == R && nc == R && aligned;
+        // Codes were range-checked by the marginal pass that runs first on the
+        // same stream (any code >= nb sets the flag); with the flag set the
+        // results are discarded, so skip the work.
+        const bool fast = !diag && nr == R && nc == R && aligned;   // WG-uniform
         if (fast) {
             done = true;
-            uint8_t* const trash = smem_raw + omap + 512;
-            // slot k of this thread: DOF ld = tid/16 + 16k, frames 4*(tid%16)..+3
+            uint8_t* const trash = smem_raw + omap + 512;   // kLdsRow bytes
+            // one read of the flag for the whole workgroup
+            __syncthreads();
+            if (tid == 0) *reinterpret_cast<int*>(trash) = *reinterpret_cast<volatile int*>(a.flag);
+            __syncthreads();
+            if (*reinterpret_cast<const int*>(trash) != 0) return;
+            // slot k of this thread: DOF ld = tid/16 + 16k, frames 4*(tid%16)..+3.
+            // Slots beyond the 2R*16 live ones repeat a live slot of another
+            // wave (same bytes to the same addresses), so that every lane does
+            // the same work without predication.
             const int k4 = (tid & 15) * 4;
+            const int nlive = 2 * R * kGroups;
             const uint8_t* src[kSlots];
             uint32_t dst[kSlots];
-            bool live[kSlots];
 #pragma unroll
             for (int k = 0; k < kSlots; ++k) {
-                const int ld = (tid >> 4) + 16 * k;
-                live[k] = ld < 2 * R;
+                const int q = (tid + 256 * k) % nlive;
+                const int ld = q / kGroups;
                 const bool isA = ld < R;
                 const int dof = isA ? r0 + ld : c0 + (ld - R);
-                src[k] = a.codes + (live[k] ? (uint64_t)dof * a.stride + k4 : 0);
+                src[k] = a.codes + (uint64_t)dof * a.stride + k4;
                 dst[k] = (uint32_t)((isA ? 0 : kOpBytes) + (isA ? ld : ld - R) * nb * kLdsRow + k4);
             }
+            // codes each slot set in buffer 0 / 1 (code 0 where nothing was set:
+            // clearing a zero byte of the slot's own frame column is harmless)
             uint32_t cur[kSlots][kSuper], nxt[kSlots][kSuper], last0[kSlots], last1[kSlots];
 #pragma unroll
-            for (int k = 0; k < kSlots; ++k) last0[k] = last1[k] = 0xFFFFFFFFu;
+            for (int k = 0; k < kSlots; ++k) last0[k] = last1[k] = 0u;
             auto advance = [](int& s, int& c, int n) __attribute__((always_inline)) {
                 if (++c == n) { c = 0; ++s; }
             };
@@ -456,43 +520,36 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
                     for (int k = 0; k < kSlots; ++k) {
                         // frames outside the chunk read a harmless in-bounds word;
                         // staging ignores them by position
-                        const bool ok = live[k] & (k4 < nv);
-                        d[k][c] = *reinterpret_cast<const uint32_t*>(ok ? src[k] + f0 : a.codes);
+                        d[k][c] = *reinterpret_cast<const uint32_t*>(k4 < nv ? src[k] + f0 : a.codes);
                     }
                     advance(ls, lc, nch);
                 }
             };
-            // chunk with nv valid frames from dword cis of cw into buffer b
+            // chunk with nv valid frames (a multiple of 4) from dword cis of
+            // cw into buffer b: clear the bytes the slot set two chunks ago,
+            // set the new ones.  A lane whose 4 frames lie beyond the chunk
+            // sets code 0 in the trash row.
             auto stage = [&](int b, int cis, int nv, const uint32_t (&cw)[kSlots][kSuper])
                 __attribute__((always_inline)) {
                 uint8_t* const buf = sm.oh + (size_t)b * 2 * kOpBytes;
+                const bool in = k4 < nv;
 #pragma unroll
                 for (int k = 0; k < kSlots; ++k) {
+                    // select dword cis with masks (a select chain on a runtime
+                    // index becomes an indexed scratch access)
                     uint32_t wv = 0;
 #pragma unroll
                     for (int c = 0; c < kSuper; ++c) wv |= cw[k][c] & (0u - (uint32_t)(cis == c));
+                    wv = in ? wv : 0u;
                     const uint32_t old = b ? last1[k] : last0[k];
                     uint8_t* const row = buf + dst[k];
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        const uint32_t ob = (old >> (8 * e)) & 0xFFu;
-                        uint8_t* const p = ob != 0xFFu ? row + ob * kLdsRow : trash;
-                        p[e] = 0;
-                    }
-                    uint32_t stored = 0;
+                    for (int e = 0; e < 4; ++e) row[((old >> (8 * e)) & 0xFFu) * kLdsRow + e] = 0;
+                    uint8_t* const srow = in ? row : trash + k4;
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        // bitwise ops on purpose: short-circuit forms become branches
-                        const uint32_t cv = (wv >> (8 * e)) & 0xFFu;
-                        const uint32_t in = (uint32_t)live[k] & (uint32_t)(k4 + e < nv);
-                        const uint32_t ok = in & (uint32_t)(cv < (uint32_t)nb);
-                        bad_acc |= in & ~ok;
-                        uint8_t* const p = ok ? row + cv * kLdsRow : trash;
-                        p[e] = 1;
-                        stored |= (ok ? cv : 0xFFu) << (8 * e);
-                    }
-                    last1[k] = b ? stored : last1[k];
-                    last0[k] = b ? last0[k] : stored;
+                    for (int e = 0; e < 4; ++e) srow[((wv >> (8 * e)) & 0xFFu) * kLdsRow + e] = 1;
+                    last1[k] = b ? wv : last1[k];
+                    last0[k] = b ? last0[k] : wv;
                 }
             };
             auto frags = [&](const uint8_t* base, int kk, v4i (&af)[4], v4i (&bf)[4])
@@ -515,59 +572,81 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
                         acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
             };
 
+            int* const parkf = reinterpret_cast<int*>(smem_raw + lay.park) + wave * 64 * 16;
+            int ps = 0, pc = 0;      // position of the next chunk to stage
+            int ms = 0, mc = 0;      // position of chunk g
             __syncthreads();
             load4(cur);
             load4(nxt);
-            stage(0, 0, nvalid_of(0, 0), cur);
+            stage(0, 0, nvalid_of(ps, pc), cur);
+            advance(ps, pc, nch);
             __syncthreads();
-            int ms = 0, mc = 0;   // chunk g = ms * nch + mc
-            for (int g = 0; g < G; ++g) {
-                const int b = g & 1;
-                if (((g + 1) & (kSuper - 1)) == 0) {
+            v4i xa0[4], xb0[4], xa1[4], xb1[4];   // fragments of a chunk in buffer 0
+            v4i ya0[4], yb0[4], ya1[4], yb1[4];   // fragments of a chunk in buffer 1
+            frags(sm.oh, 0, xa0, xb0);
+            frags(sm.oh, 1, xa1, xb1);
+            stage(1, 1, nvalid_of(ps, pc), cur);
+            advance(ps, pc, nch);
+            mfma16(xa0, xb0);
+            __syncthreads();
+            // Iteration g (chunk g in buffer b = g & 1, its second-half
+            // fragments already in registers; chunk g+1 staged in buffer b^1):
+            // read chunk g+1's fragments and multiply chunk g's second half
+            // while they arrive; run the step epilogue if chunk g ends a step;
+            // stage chunk g+2 into buffer b (all of its reads completed before
+            // the last barrier) between the MFMAs of chunk g+1's first half.
+            // No MFMA waits on an LDS read issued after the preceding barrier.
+            auto iter = [&](int g, int b, const v4i (&ha)[4], const v4i (&hb)[4],
+                            v4i (&na0)[4], v4i (&nb0)[4], v4i (&na1)[4], v4i (&nb1)[4])
+                __attribute__((always_inline)) {
+                if (((g + 2) & (kSuper - 1)) == 0) {   // chunk g+2 opens a super-chunk
 #pragma unroll
                     for (int k = 0; k < kSlots; ++k)
 #pragma unroll
                         for (int c = 0; c < kSuper; ++c) cur[k][c] = nxt[k][c];
                     load4(nxt);
                 }
-                int s1 = ms, c1 = mc;
-                advance(s1, c1, nch);
-                const int nv1 = nvalid_of(s1, c1);
-                const uint8_t* base = sm.oh + (size_t)b * 2 * kOpBytes;
-                v4i a0[4], b0[4], a1[4], b1[4];
-                frags(base, 0, a0, b0);
-                frags(base, 1, a1, b1);
-                mfma16(a0, b0);
-                stage(b ^ 1, (g + 1) & (kSuper - 1), nv1, cur);
-                mfma16(a1, b1);
-                // fragment reads first, then one MFMA at a time with a share
-                // of the staging VALU work and one LDS write in its shadow
-                __builtin_amdgcn_sched_group_barrier(0x100, 16, 0);
+                const uint8_t* nbase = sm.oh + (size_t)(b ^ 1) * 2 * kOpBytes;
+                frags(nbase, 0, na0, nb0);
+                frags(nbase, 1, na1, nb1);
+                mfma16(ha, hb);
+                // a wave may have at most 15 LDS operations in flight: issue 6
+                // reads, then one read per MFMA, so that no read waits for a
+                // slot while the MFMA pipe idles
+                __builtin_amdgcn_sched_group_barrier(0x100, 6, 0);
 #pragma unroll
-                for (int i = 0; i < 32; ++i) {
+                for (int i = 0; i < 10; ++i) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                    __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+                }
+#pragma unroll
+                for (int i = 0; i < 6; ++i) __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                if (mc == nch - 1) {
+                    __syncthreads();
+                    epilogue(ms, parkf, -1);
+                }
+                advance(ms, mc, nch);
+                stage(b, (g + 2) & (kSuper - 1), nvalid_of(ps, pc), cur);
+                advance(ps, pc, nch);
+                mfma16(na0, nb0);
+                // one MFMA at a time with a share of the staging VALU work and
+                // one LDS write in its shadow
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
                     __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
                     __builtin_amdgcn_sched_group_barrier(0x002, 2 * kSlots, 0);
                     __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
                 }
-                if (mc == nch - 1) {
-                    __syncthreads();            // every wave is done with buffer b
-                    epilogue(ms, g);
-#pragma unroll
-                    for (int k = 0; k < kSlots; ++k) {
-                        last1[k] = b ? 0xFFFFFFFFu : last1[k];
-                        last0[k] = b ? last0[k] : 0xFFFFFFFFu;
-                    }
-                }
-                ms = s1;
-                mc = c1;
                 __syncthreads();
+            };
+            for (int g = 0; g < G; g += 2) {
+                iter(g, 0, xa1, xb1, ya0, yb0, ya1, yb1);
+                if (g + 1 >= G) break;
+                iter(g + 1, 1, ya1, yb1, xa0, xb0, xa1, xb1);
             }
         }
     }
-    if (done) {
-        if (bad_acc) atomicOr(a.flag, 1);
-        return;
-    }
+    if (done) return;
 
     // ---- general path ------------------------------------------------------
     // iteration g: multiply the first half of chunk g, stage chunk g+1 while
@@ -598,7 +677,9 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
         mfma_half(g, 1);
         if (g % nch == nch - 1) {
             __syncthreads();                 // every wave is done with buffer g&1
-            epilogue(g / nch, g);
+            epilogue(g / nch,
+                     reinterpret_cast<int*>(sm.oh + (size_t)((g & 1) * 2) * kOpBytes) + wave * 64 * 16,
+                     g);
         }
         __syncthreads();
     }
@@ -606,10 +687,7 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
 
 }  // namespace
 
-size_t pair_tile_lds_bytes(int R) {
-    size_t a, b, c, d, e;
-    return lds_layout(R, &a, &b, &c, &d, &e);
-}
+size_t pair_tile_lds_bytes(int R) { return lds_layout(R).total; }
 
 template <bool kDump, int kSlots>
 static hipError_t launch_t(const PairKernelArgs& a, int n_tiles, size_t lds, hipStream_t st) {
