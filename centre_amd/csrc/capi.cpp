@@ -833,6 +833,15 @@ extern "C" int centre_memcpy_h2d(centre_ctx* ctx, void* d_dst, const void* src, 
     return 0;
 }
 
+extern "C" int centre_memcpy_d2d(centre_ctx* ctx, void* d_dst, const void* d_src, size_t bytes) {
+    if (!ctx) return -3;
+    if (!bytes) return 0;
+    if (!d_dst || !d_src) return fail(ctx, -3, "null pointer");
+    HIPCHK(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
 extern "C" int centre_memcpy_d2h(centre_ctx* ctx, void* dst, const void* d_src, size_t bytes) {
     if (!ctx) return -3;
     if (!bytes) return 0;

@@ -51,6 +51,7 @@ _SIGS = {
     "centre_dev_free": ([P, P], I32),
     "centre_memcpy_h2d": ([P, P, P, C.c_size_t], I32),
     "centre_memcpy_d2h": ([P, P, P, C.c_size_t], I32),
+    "centre_memcpy_d2d": ([P, P, P, C.c_size_t], I32),
     "centre_device_sync": ([P], I32),
 }
 
@@ -155,6 +156,9 @@ class Context:
         d = DevArray(self, x.shape, x.dtype)
         d.upload(x)
         return d
+
+    def copy_d2d(self, dst: int, src: int, nbytes: int):
+        self.check(self._L.centre_memcpy_d2d(self.h, dst, src, nbytes))
 
     def last_pair_kernel_ms(self) -> float:
         return float(self._L.centre_last_pair_kernel_ms(self.h))

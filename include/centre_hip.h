@@ -171,6 +171,8 @@ int centre_memcpy_h2d(centre_ctx *ctx, void *d_dst, const void *src,
                       size_t bytes);
 int centre_memcpy_d2h(centre_ctx *ctx, void *dst, const void *d_src,
                       size_t bytes);
+int centre_memcpy_d2d(centre_ctx *ctx, void *d_dst, const void *d_src,
+                      size_t bytes);
 /* hipDeviceSynchronize on the context's device. */
 int centre_device_sync(centre_ctx *ctx);
 
