@@ -2,6 +2,9 @@
 
 Every ``csrc/*.hip`` and ``csrc/*.cpp`` file is compiled with hipcc for
 ``--offload-arch=gfx950`` and linked into ``centre_amd/libcentre_hip.so``.
+The C++ host (``host/*.cpp``, the drop-in for the centre_OMP executable) is
+compiled with g++ against that library's C ABI into ``centre_amd/centre_hip``
+(it loads libhdf5 at run time for NetCDF-4 input).
 Objects go to ``centre_amd/_build/``.  Always recompiles when called.
 """
 from __future__ import annotations
@@ -17,6 +20,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libcentre_hip.so")
 BUILD = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HOST_BIN = os.path.join(HERE, "centre_hip")
 ARCH = "gfx950"
 FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -46,9 +50,22 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, OUT)
+    build_host()
     if verbose:
-        print(f"built {OUT}")
+        print(f"built {OUT} and {HOST_BIN}")
     return OUT
+
+
+def build_host() -> str:
+    srcs = sorted(glob.glob(os.path.join(HERE, "host", "*.cpp")))
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(os.path.dirname(HERE), "include")]
+    libs = ["-L" + HERE, "-lcentre_hip", "-Wl,-rpath,$ORIGIN", "-ldl"]
+    tmp = HOST_BIN + ".tmp"
+    r = subprocess.run(cmd + srcs + ["-o", tmp] + libs, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, HOST_BIN)
+    return HOST_BIN
 
 
 if __name__ == "__main__":

@@ -346,12 +346,19 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
                     nhi += h ? nz : 0;
                 };
                 // counts below the table size (the usual case) are one LDS
-                // read each, all issued before use; a wave holding a larger
-                // count computes its 16 terms by the series
-                if (__builtin_amdgcn_ballot_w64(mx >= ntab) == 0) {
+                // read each, all issued before use; the terms of larger counts
+                // are recomputed by the series, per count position and only
+                // where some lane of the wave holds one
+                if (ntab > 0) {
                     double tv[16];
 #pragma unroll
-                    for (int r = 0; r < 16; ++r) tv[r] = sm.ctab[c[r]];
+                    for (int r = 0; r < 16; ++r) tv[r] = sm.ctab[min(c[r], ntab - 1)];
+                    if (__builtin_amdgcn_ballot_w64(mx >= ntab) != 0) {
+#pragma unroll
+                        for (int r = 0; r < 16; ++r)
+                            if (__builtin_amdgcn_ballot_w64(c[r] >= ntab) != 0 && c[r] >= ntab)
+                                tv[r] = (double)c[r] * log_count(c[r], sm.logt, sm.invt);
+                    }
 #pragma unroll
                     for (int r = 0; r < 16; ++r) add(r, tv[r]);
                 } else {

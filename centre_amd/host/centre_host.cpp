@@ -21,6 +21,7 @@
 #include <iostream>
 #include <map>
 #include <numeric>
+#include <random>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -28,6 +29,7 @@
 #include "../../include/centre_hip.h"
 #include "config.hpp"
 #include "nc_classic.hpp"
+#include "ncfile.hpp"
 
 using namespace centre;
 
@@ -66,7 +68,8 @@ bool file_exists(const std::string& p) { return std::ifstream(p).good(); }
 const char* TYPE_VAR[3] = {"bond", "angle", "dihedral"};
 const char* BIN_FILE[3] = {"bin_bonds.nc", "bin_angles.nc", "bin_torsions.nc"};
 const char* ONE_TAG[3] = {"bnd", "ang", "tor"};
-const char* ONE_CONTENT[3] = {"B", "A", "D"};
+const char* ONE_CONTENT[3] = {"B", "A", "D"};   // entcontri-1d content
+const char* BIN_CONTENT[3] = {"B", "A", "T"};   // bin file content (Discretizer.cpp:40-60)
 const uint32_t ONE_SET[3] = {B1D, A1D, D1D};
 const char* EST_NAME[4] = {"ML", "MM", "CS", "JS"};
 
@@ -140,14 +143,14 @@ void write_bin_file(const std::string& path, int type, const std::vector<uint8_t
     const int de = w.add_dim("extremes", 4);
     const int ds = w.add_dim("bin_scheme_count", schemes.size());
     const int df = w.add_dim("frames", F);
-    const std::string title = std::string(ONE_CONTENT[type]) + " Integer Vectors";
+    const std::string title = std::string(BIN_CONTENT[type]) + " Integer Vectors";
     w.add_attr("title", nc::Attr::of_text(title));
     w.add_attr("application", nc::Attr::of_text("CENTRE"));
     w.add_attr("program", nc::Attr::of_text("Real2Int"));
     w.add_attr("programVersion", nc::Attr::of_text("1.0.0"));
     w.add_attr("convention", nc::Attr::of_text("CENTRE_Real"));
     w.add_attr("conventionVersion", nc::Attr::of_text("1.0"));
-    w.add_attr("content", nc::Attr::of_text(type == 2 ? "T" : ONE_CONTENT[type]));
+    w.add_attr("content", nc::Attr::of_text(BIN_CONTENT[type]));
     w.add_attr("ncoordims", nc::Attr::of<uint32_t>(nc::UINT, {n_dof}));
     w.add_attr("nbinSchemes", nc::Attr::of<uint8_t>(nc::UBYTE, {(uint8_t)schemes.size()}));
     w.add_attr("binSchemes", nc::Attr::of<uint8_t>(nc::UBYTE, schemes));
@@ -209,18 +212,52 @@ struct Host {
 
     std::string out(const std::string& f) const { return join(in.outfilepath, f); }
 
-    // Discretizer::run (Discretizer.cpp:314-361)
-    void discretize(nc::Reader& bat) {
+    // Frame shuffle index (main_smp.cpp:194-201, 312-325): std::shuffle of
+    // 0..F-1 with mt19937(randseed + 1000), randseed 0 = random_device.
+    std::vector<uint64_t> shuffle_index(uint64_t F) {
+        std::random_device rd;
+        uint32_t seed = rd();
+        if (in.randseed != 0) seed = (uint32_t)(in.randseed + 1000);
+        std::mt19937 gen(seed);
+        std::vector<uint64_t> idx(F);
+        std::iota(idx.begin(), idx.end(), 0ull);
+        std::shuffle(idx.begin(), idx.end(), gen);
+        std::ofstream ofp(out("shuffle_indices.bin"), std::ios::binary);
+        ofp.write(reinterpret_cast<const char*>(idx.data()), idx.size() * sizeof(idx[0]));
+        return idx;
+    }
+
+    // Discretizer::run (Discretizer.cpp:314-361).  DOF counts come from the
+    // input file (discretization.nbond/nangle/ndihed, first DOFs of the BAT
+    // file), frames from discretization.nframe (all frames when absent).
+    void discretize(const ncf::File& bat) {
         Timer t;
         const std::vector<uint8_t> schemes = in.nbins;
         std::ofstream minfile;
         if (in.pdfmethod == PdfMethod::VonMisesKDE) minfile.open(out("vmkde_torsions_minimas.out"));
+        uint64_t F = F_file;
+        if (!in.nframes.empty()) {
+            F = 0;
+            for (auto v : in.nframes) F += v;
+            if (F > F_file) throw Fail("discretization.nframe exceeds the frames of the BAT file");
+        }
+        // Real2Int::shuffleArray swaps over codes of F frames; with several
+        // bin schemes the code vector is longer than the index and the
+        // reference leaves it unshuffled (Real2Int.cpp:12-25)
+        std::vector<uint64_t> shuf;
+        if (in.shuffleframes) shuf = shuffle_index(F);
+        const unsigned ncfg[3] = {in.nbond, in.nangle, in.ndihed};
         for (int ty = 0; ty < 3; ++ty) {
             TypeData& T = td[ty];
-            T.n_all = (uint32_t)bat.shape(TYPE_VAR[ty])[1];
+            const uint32_t nfile = bat.has_var(TYPE_VAR[ty]) ? (uint32_t)bat.shape(TYPE_VAR[ty])[1] : 0;
+            T.n_all = ncfg[ty] ? std::min<uint32_t>(ncfg[ty], nfile) : nfile;
             if (!T.n_all) continue;
-            const uint64_t F = F_file;
-            std::vector<double> x = bat.read<double>(TYPE_VAR[ty], {0, 0}, {F, T.n_all});
+            std::vector<double> x = bat.read_f64(TYPE_VAR[ty], {0, 0}, {F, nfile});
+            if (T.n_all != nfile) {   // keep the first n_all DOFs of each frame
+                for (uint64_t f = 0; f < F; ++f)
+                    std::copy_n(&x[f * nfile], T.n_all, &x[f * T.n_all]);
+                x.resize(F * T.n_all);
+            }
             Dev dx(ctx, x.size() * 8);
             dx.put(x.data(), x.size() * 8);
             const bool vm = in.pdfmethod == PdfMethod::VonMisesKDE && ty == 2;
@@ -255,10 +292,16 @@ struct Host {
                 }
             }
             dc.get(T.codes.data(), T.codes.size());
+            if (!shuf.empty() && nsch == 1)
+                for (uint32_t d = 0; d < T.n_all; ++d) {
+                    uint8_t* row = &T.codes[(size_t)d * F];
+                    for (uint64_t i = 0; i < F; ++i) std::swap(row[i], row[shuf[i]]);
+                }
             write_bin_file(out(BIN_FILE[ty]), ty,
                            vm ? std::vector<uint8_t>{(uint8_t)in.nmaxconf} : schemes, T.n_all, F,
                            T.codes, T.ext);
         }
+        F_file = F;
         times["discretize"] = t.s();
     }
 
@@ -266,12 +309,12 @@ struct Host {
         for (int ty = 0; ty < 3; ++ty) {
             TypeData& T = td[ty];
             if (!file_exists(out(BIN_FILE[ty]))) continue;
-            nc::Reader r(out(BIN_FILE[ty]));
-            auto shp = r.shape("coord");
+            auto r = ncf::open(out(BIN_FILE[ty]));
+            auto shp = r->shape("coord");
             T.n_all = (uint32_t)shp[0];
             F_file = shp[2];
-            T.codes = r.read<uint8_t>("coord");
-            T.ext = r.read<double>("extreme");
+            T.codes = r->read_u8("coord");
+            T.ext = r->read_f64("extreme");
         }
     }
 
@@ -450,14 +493,14 @@ void Host::run_entropy() {
 
 std::vector<double> contri_of(const std::string& path, int est, int sch, int step,
                               std::vector<uint32_t>& ids, int& order) {
-    nc::Reader r(path);
-    auto shp = r.shape("contri_S");
-    order = (int)r.shape("id")[1];
-    ids = r.read<uint32_t>("id");
+    auto r = ncf::open(path);
+    auto shp = r->shape("contri_S");
+    order = (int)r->shape("id")[1];
+    ids = r->read_u32("id");
     std::vector<double> v(shp[0]);
     if (shp[0] == 0) return v;
-    return r.read<double>("contri_S", {0, (uint64_t)est, (uint64_t)sch, (uint64_t)step},
-                          {shp[0], 1, 1, 1});
+    return r->read_f64("contri_S", {0, (uint64_t)est, (uint64_t)sch, (uint64_t)step},
+                       {shp[0], 1, 1, 1});
 }
 
 // EntropyScorer::run (EntropyScorer.cpp:7-875)
@@ -563,8 +606,11 @@ void Host::run_scoring() {
                     const char tc[3] = {'B', 'A', 'T'};
                     auto type_of = [&](uint32_t v) { return v < off[1] ? 0 : (v < off[2] ? 1 : 2); };
                     auto id_of = [&](uint32_t v) { int ty = type_of(v); return id1[ty][v - off[ty]]; };
-                    for (uint32_t q = 0; q < nt; ++q) {
-                        const uint32_t ei = tree[q];
+                    // edges by MI, largest first (cmpEdgeMIST, main_common.cpp:295)
+                    std::vector<uint32_t> order(tree.begin(), tree.begin() + nt);
+                    std::sort(order.begin(), order.end(),
+                              [&](uint32_t x, uint32_t y) { return w[x] > w[y]; });
+                    for (uint32_t ei : order) {
                         tk[ekind[ei]] += w[ei];
                         tf << std::setw(3) << std::left << tc[type_of(src[ei])] << ", " << std::setw(3)
                            << std::left << tc[type_of(dst[ei])] << ", " << std::setw(9) << std::left
@@ -585,7 +631,58 @@ void Host::run_scoring() {
 void usage() {
     std::cout << "usage: centre_hip -i <input.cfg> [-O]\n"
                  "  -i  CENTRE input file (same keys as centre_OMP)\n"
-                 "  -O  overwrite existing output files\n";
+                 "  -O  overwrite existing output files\n"
+                 "       centre_hip --inspect <file.nc> [var ...]\n"
+                 "  print the format, global text attributes and, per variable,\n"
+                 "  its shape and the sum of its values (no GPU is used)\n"
+                 "       centre_hip --print-config <input.cfg>\n"
+                 "  print the parsed input parameters (no GPU is used)\n";
+}
+
+int inspect(int argc, char** argv, int first) {
+    auto f = ncf::open(argv[first]);
+    std::cout << "format " << f->format() << "\n";
+    for (const char* a : {"title", "convention", "content", "program", "application"}) {
+        const std::string t = f->text_attr(a);
+        if (!t.empty()) std::cout << "attr " << a << " " << t << "\n";
+    }
+    for (int i = first + 1; i < argc; ++i) {
+        const std::string v = argv[i];
+        if (!f->has_var(v)) {
+            std::cout << "var " << v << " absent\n";
+            continue;
+        }
+        auto shp = f->shape(v);
+        std::cout << "var " << v << " shape";
+        for (size_t k = 0; k < shp.size(); ++k) std::cout << (k ? "," : " ") << shp[k];
+        const auto x = f->read_f64(v);
+        double sum = 0.0;
+        for (double d : x) sum += d;
+        std::cout << " sum " << std::setprecision(17) << sum << "\n";
+    }
+    return 0;
+}
+
+int print_config(const char* path) {
+    std::ifstream f(path);
+    if (!f) throw Fail(std::string("cannot read input file ") + path);
+    std::stringstream ss;
+    ss << f.rdbuf();
+    Inputs in = inputs_from_symbols(parse_ini(ss.str()));
+    for (auto& e : in.errors) std::cout << "error " << e << "\n";
+    std::cout << "nsteps " << in.nsteps << "\ndiscretize " << in.discretize << "\ncalcentropy "
+              << in.calcentropy << "\ngenconvgdata " << in.genconvgdata << "\ninfilepath "
+              << in.infilepath << "\noutfilepath " << in.outfilepath << "\npdfmethod "
+              << (in.pdfmethod == PdfMethod::Histogram ? "histogram" : "vonmiseskde")
+              << "\nnbins";
+    for (auto b : in.nbins) std::cout << " " << (int)b;
+    std::cout << "\nreferencenbins " << in.referencenbins << "\nworkset " << in.workset
+              << "\nusesubset " << in.usesubset << "\nuseneighbor " << in.useneighbor
+              << "\njacobian " << in.jacobian << "\nestimators";
+    for (auto e : in.estimators) std::cout << " " << EST_NAME[e];
+    std::cout << "\nmethod " << (int)in.method << "\nkappa " << in.kappa << "\nnmaxconf "
+              << in.nmaxconf << "\ncacheentcdimspercpu " << in.cacheentcdimspercpu << "\n";
+    return in.ok() ? 0 : 1;
 }
 
 }  // namespace
@@ -593,6 +690,13 @@ void usage() {
 int main(int argc, char** argv) {
     std::string cfg;
     bool overwrite = false;
+    try {
+        if (argc >= 3 && !std::strcmp(argv[1], "--inspect")) return inspect(argc, argv, 2);
+        if (argc >= 3 && !std::strcmp(argv[1], "--print-config")) return print_config(argv[2]);
+    } catch (const std::exception& e) {
+        std::cerr << "centre_hip: " << e.what() << "\n";
+        return 2;
+    }
     for (int i = 1; i < argc; ++i) {
         if (!std::strcmp(argv[i], "-i") && i + 1 < argc) cfg = argv[++i];
         else if (!std::strcmp(argv[i], "-O")) overwrite = true;
@@ -610,18 +714,28 @@ int main(int argc, char** argv) {
             for (auto& e : h.in.errors) std::cerr << "InputReadError:: " << e << "\n";
             return 1;
         }
-        const std::string batp = join(h.in.infilepath, h.in.fnames[0]);
-        nc::Reader bat(batp);
-        auto conv = bat.attrs().find("convention");
-        if (conv == bat.attrs().end() || conv->second.text() != "CENTRE")
-            throw Fail(batp + ": global attribute convention is not \"CENTRE\"");
-        h.F_file = bat.shape(TYPE_VAR[0])[0];
-        derive(h.in, h.F_file);
-        if (!overwrite && h.in.discretize && file_exists(join(h.in.outfilepath, BIN_FILE[0])))
-            throw Fail("output files exist in " + h.in.outfilepath + " (use -O to overwrite)");
-        ck(nullptr, centre_init(h.in.device, &h.ctx), "centre_init (no GPU?)");
-        if (h.in.discretize) h.discretize(bat);
-        else h.load_bins();
+        if (h.in.discretize) {
+            const std::string batp = join(h.in.infilepath, h.in.fnames[0]);
+            auto bat = ncf::open(batp);
+            // NetcdfFile::getNetcdfConventions (NetcdfFile.cpp:88-110)
+            if (bat->text_attr("convention") != "CENTRE")
+                throw Fail(batp + ": global attribute convention is not \"CENTRE\"");
+            for (const char* v : TYPE_VAR)
+                if (bat->has_var(v)) {
+                    h.F_file = bat->shape(v)[0];
+                    break;
+                }
+            derive(h.in, h.F_file);
+            if (!overwrite && file_exists(join(h.in.outfilepath, BIN_FILE[0])))
+                throw Fail("output files exist in " + h.in.outfilepath + " (use -O to overwrite)");
+            ck(nullptr, centre_init(h.in.device, &h.ctx), "centre_init (no GPU?)");
+            h.discretize(*bat);
+        } else {
+            // entropy from the bin files a previous run wrote (outfilepath)
+            h.load_bins();
+            ck(nullptr, centre_init(h.in.device, &h.ctx), "centre_init (no GPU?)");
+        }
+        derive(h.in, h.F_file);   // entropy frames within the discretised ones
         if (h.in.calcentropy) h.run_entropy();
         if (h.in.genconvgdata) h.run_scoring();
         std::ofstream info(join(h.in.outfilepath, h.in.infofile));

@@ -1,6 +1,7 @@
 // NetCDF classic-family reader and writer (see nc_classic.hpp).
 #include "nc_classic.hpp"
 
+#include <algorithm>
 #include <cstring>
 #include <fstream>
 #include <numeric>

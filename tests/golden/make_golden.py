@@ -10,7 +10,9 @@ and outputs for ``test_examples/example-{1,2}``:
 * ``entcontri_*.nc`` (NetCDF-4/HDF5): ``id`` and ``contri_S``, dumped with
   ``h5dump -m %.17g`` (17 significant digits round-trip a double exactly);
 * ``hist_tor-1d.nc`` (CDF-5): uint64 counts, extrema and bin mids;
-* the convergence CSVs, the MIST tree CSVs and ``vmkde_torsions_minimas.out``.
+* the convergence CSVs, the MIST tree CSVs and ``vmkde_torsions_minimas.out``;
+* one stored NetCDF-4 file copied as is (``ex1h_entcontri_ang-1d.nc``, 18 KB),
+  the format fixture of the host's NetCDF-4 reader.
 
 Usage:  python tests/golden/make_golden.py [/root/reference]
 """
@@ -125,6 +127,9 @@ def main(ref: str) -> None:
 
     dst = os.path.join(HERE, "reference_examples.npz")
     np.savez_compressed(dst, **out)
+    import shutil
+    shutil.copyfile(os.path.join(ex1, "outfiles-h-ref", "entcontri_ang-1d.nc"),
+                    os.path.join(HERE, "ex1h_entcontri_ang-1d.nc"))
     print(f"wrote {dst} ({os.path.getsize(dst) / 1e6:.2f} MB, "
           f"{len(out)} arrays)")
 

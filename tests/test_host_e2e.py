@@ -1,0 +1,146 @@
+"""End to end through the C++ host on one GPU, against the reference's stored
+example-1 run (histogram, 30 bins, 4 steps, ML, MIST).
+
+Inputs are the reference's own stored bin-code files (codes and extrema, as
+fixtures), a subset file listing the DOFs its 1-D files hold and a neighbour
+file listing the pairs its 2-D files hold; the host runs the entropy and
+scoring stages (``discretize = false``, as centre_OMP does when restarted
+from its bin files) and its outputs are compared with the stored ones:
+every 1-D and 2-D entropy contribution (1e-10 relative), the MIE and MIST
+convergence tables (printed with 4 decimals) and the MIST trees of every step.
+"""
+import csv
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from centre_amd import cdf
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HOST = os.path.join(ROOT, "centre_amd", "centre_hip")
+TYPES = (("b", "bin_bonds.nc", "B", "bond", "bnd"), ("a", "bin_angles.nc", "A", "angle", "ang"),
+         ("t", "bin_torsions.nc", "T", "torsion", "tor"))
+KIND_WORD = {"bnd": "bond", "ang": "angle", "tor": "torsion",
+             "ba": "bacross", "bd": "bdcross", "ad": "adcross"}
+
+
+def write_inputs(d, g):
+    out = os.path.join(d, "out")
+    os.makedirs(out)
+    for t, fname, content, _, _ in TYPES:
+        codes = g[f"ex1_codes_{t}"]
+        D, F = codes.shape
+        sch = g[f"ex1_binschemes_{t}"].astype(np.uint8)
+        cdf.write_dataset(
+            os.path.join(out, fname),
+            [("dims", D), ("extremes", 4), ("bin_scheme_count", len(sch)), ("frames", F)],
+            [cdf.VarSpec("dimid", ("dims",), g[f"ex1_dimid_{t}"].astype(np.uint32)),
+             cdf.VarSpec("extreme", ("dims", "extremes"), g[f"ex1_extreme_{t}"]),
+             cdf.VarSpec("coord", ("dims", "bin_scheme_count", "frames"), codes[:, None, :])],
+            attrs={"title": f"{content} Integer Vectors", "application": "CENTRE",
+                   "program": "Real2Int", "programVersion": "1.0.0",
+                   "convention": "CENTRE_Real", "conventionVersion": "1.0", "content": content,
+                   "ncoordims": np.array([D], np.uint32),
+                   "nbinSchemes": np.array([len(sch)], np.uint8), "binSchemes": sch},
+            version=5, unlimited="dims")
+    sub = []
+    for t, _, _, word, tag in TYPES:
+        ids = g[f"ex1_id_{tag}1d"][:, 0]
+        sub.append(f"{word} {g[f'ex1_codes_{t}'].shape[0]} {len(ids)} " + " ".join(map(str, ids)) + ";")
+    with open(os.path.join(d, "subset.txt"), "w") as f:
+        f.write("\n".join(sub) + "\n")
+    neigh = []
+    for tag, word in KIND_WORD.items():
+        ids = g[f"ex1_id_{tag}2d"]
+        rows = {}
+        for i, j in ids:
+            rows.setdefault(int(i), []).append(int(j))
+        for i in sorted(rows):
+            neigh.append(f"{word} {i} {len(rows[i])} " + " ".join(map(str, sorted(rows[i]))) + ";")
+    with open(os.path.join(d, "neigh.txt"), "w") as f:
+        f.write("\n".join(neigh) + "\n")
+    cfg = os.path.join(d, "centre.cfg")
+    with open(cfg, "w") as f:
+        f.write(f"""[control]
+    infilepath           = {d}/
+    outfilepath          = {out}/
+    infofile             = progress-1.info
+    nsteps               = 4
+    discretize           = false
+    calcentropy          = true
+    genconvgdata         = true
+    cacheentcdimspercpu  = 10
+[discretization]
+    fname                = 0O.lig-s500_dfs_11-8-7_bat.nc
+    pdfmethod            = histogram
+[histogram]
+    referencenbins       = 30
+    nbins                = 30
+[entropy]
+    workset              = 2D
+    usesubset            = true
+    useneighbor          = true
+    jacobian             = true
+    subsetfile           = subset.txt
+    neighborfile         = neigh.txt
+    scoringmethod        = MIST
+    estimator            = ML
+""")
+    return cfg, out
+
+
+def read_table(path):
+    with open(path) as f:
+        rows = [[c.strip() for c in r] for r in csv.reader(f)]
+    return rows[0], rows[1:]
+
+
+@pytest.mark.gpu
+def test_host_example1_entropy_and_scoring(tmp_path, golden):
+    assert os.path.exists(HOST), "host binary missing: run __graft_entry__.build()"
+    cfg, out = write_inputs(str(tmp_path), golden)
+    print("running host on example-1", flush=True)
+    r = subprocess.run([HOST, "-i", cfg], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+    # entropy contributions: same records, same values
+    for tag in ("bnd", "ang", "tor"):
+        ds = cdf.open_dataset(os.path.join(out, f"entcontri_{tag}-1d.nc"))
+        assert ds.attrs["convention"] == "EntropyCont" and ds.attrs["order"][0] == 1
+        ids, S = ds.read("id")[:, 0], ds.read("contri_S")[:, 0, 0, :]
+        np.testing.assert_array_equal(ids, golden[f"ex1_id_{tag}1d"][:, 0])
+        np.testing.assert_allclose(S, golden[f"ex1_S_{tag}1d"], rtol=1e-10, atol=1e-12)
+    for tag in KIND_WORD:
+        ds = cdf.open_dataset(os.path.join(out, f"entcontri_{tag}-2d.nc"))
+        same = tag in ("bnd", "ang", "tor")
+        assert ds.attrs["storagetype"][0] == (3 if same else 0)
+        assert ds.attrs["title"][0] == (1 if same else 0)
+        ids, S = ds.read("id"), ds.read("contri_S")[:, 0, 0, :]
+        want = {(int(i), int(j)): v for (i, j), v in zip(golden[f"ex1_id_{tag}2d"], golden[f"ex1_S_{tag}2d"])}
+        got = {(int(i), int(j)): v for (i, j), v in zip(ids, S)}
+        assert got.keys() == want.keys(), tag
+        a = np.array([got[k] for k in want])
+        b = np.array([want[k] for k in want])
+        np.testing.assert_allclose(a, b, rtol=1e-10, atol=1e-12, err_msg=tag)
+
+    # convergence tables (4 decimals as printed)
+    for meth in ("MIE", "MIST"):
+        hdr, rows = read_table(os.path.join(out, f"{meth}-ML_convergence_b-30.csv"))
+        assert hdr == list(golden[f"ex1h_{meth}_hdr"])
+        got = np.array([[float(v) for v in row] for row in rows])
+        np.testing.assert_allclose(got, golden[f"ex1h_{meth}_csv"], rtol=0, atol=1.5e-4, err_msg=meth)
+
+    # MIST trees: same edges, same MI, largest MI first
+    tcode = {"B": 0, "A": 1, "T": 2}
+    for s in range(4):
+        hdr, rows = read_table(os.path.join(out, f"Tree-MIST-ML_b-30_s-{s}.csv"))
+        assert hdr == ["TD1", "TD2", "IndexDim1", "IndexDim2", "MI"]
+        got = np.array([[tcode[r[0]], tcode[r[1]], int(r[2]), int(r[3]), float(r[4])] for r in rows])
+        want = golden[f"ex1h_tree{s}"]
+        assert got.shape == want.shape
+        key = lambda m: sorted(tuple(int(x) for x in e[:4]) for e in m)
+        assert key(got) == key(want), f"tree edges differ at step {s}"
+        assert np.all(np.diff(got[:, 4]) <= 0)
+        np.testing.assert_allclose(np.sort(got[:, 4]), np.sort(want[:, 4]), rtol=0, atol=1.5e-6)
