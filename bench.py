@@ -38,6 +38,11 @@ def parse():
     ap.add_argument("--cpu-pairs", type=int, default=0,
                     help="pairs in the CPU-baseline sample (0 = all pairs)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="pairs", choices=["pairs", "screen"],
+                    help="pairs: the headline full pairwise pair stage (default); "
+                         "screen: learned pair screen vs the plain path at 6,000 DOF")
+    ap.add_argument("--screen-train", type=int, default=20, help="training systems (screen)")
+    ap.add_argument("--screen-dof", type=int, default=6000, help="evaluation DOFs (screen)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after timing, copy 256 MiB device to device once (the "
                          "known-size kernel the HBM counters are scaled by)")
@@ -71,6 +76,8 @@ def main():
 
     from centre_amd import hip
 
+    if args.workload == "screen":
+        return screen_workload(args, hip)
     nb, S, F = args.nbins, args.nsteps, args.frames
     nbd, nan, ndh = args.n_bond, args.n_angle, args.n_dih
     D = nbd + nan + ndh
@@ -204,6 +211,175 @@ def main():
     }
     print(json.dumps(out), flush=True)
     plan.close()
+    ctx.close()
+
+
+# ---- learned pair screen (cfg: 6,000 DOF x 100k frames, 50 bins) ---------------
+def _system(ctx, hip, seed, D, F, nb):
+    nbd = D // 3
+    nan = D // 3
+    ndh = D - nbd - nan
+    bat = hip.synth_bat(seed, nbd, nan, ndh, F)
+    d_bat = ctx.to_device(bat)
+    del bat
+    d_codes = ctx.alloc((D, F), np.uint8)
+    d_ext = ctx.alloc((D, 4), np.float64)
+    types = np.array([1] * nbd + [2] * nan + [3] * ndh, np.uint8)
+    col = 0
+    for t, n in ((1, nbd), (2, nan), (3, ndh)):
+        ctx.dev_discretize_hist(d_bat.ptr + 8 * col, D, n, F, t, [nb], nb, True,
+                                d_codes.ptr + col * F, d_ext.ptr + 32 * col)
+        col += n
+    ctx.sync()
+    d_bat.free()
+    return d_codes, d_ext, ctx.to_device(types)
+
+
+def _pass(ctx, hip, d_codes, d_types, d_ext, D, F, nb, S, tiles=None):
+    """Pair stage over all pairs (or the given tiles): returns MI of the last
+    step in triangle order (pairs outside the tiles: NaN), S1 and seconds."""
+    P = D * (D - 1) // 2
+    d_s1 = ctx.alloc((D, 1, S), np.float64)
+    d_s2 = ctx.alloc((P, 1, S), np.float64)
+    d_mi = ctx.alloc((P, S), np.float64)
+    ctx.device_sync()
+    t0 = time.perf_counter()
+    plan = hip.PairPlan(ctx, D, nb, tiles=tiles) if tiles is not None else hip.PairPlan(ctx, D, nb)
+    ctx.dev_pair_entropy(plan, d_codes.ptr, F, F, d_types.ptr, d_ext.ptr, S, [0], True, False,
+                         d_s1.ptr, d_s2.ptr, d_mi.ptr)
+    ctx.device_sync()
+    dt = time.perf_counter() - t0
+    plan.close()
+    mi = d_mi.numpy()[:, S - 1].copy()
+    s1 = d_s1.numpy()[:, 0, S - 1].copy()
+    for a in (d_s1, d_s2, d_mi):
+        a.free()
+    return mi, s1, dt
+
+
+def _tile_pairs(tiles, D, R):
+    """Pair indices (i < j) inside the given DOF tiles."""
+    a, b = np.divmod(np.arange(R * R), R)
+    i = (tiles[:, 0:1] * R + a[None, :]).ravel()
+    j = (tiles[:, 1:2] * R + b[None, :]).ravel()
+    ok = (i < j) & (j < D)
+    return i[ok], j[ok]
+
+
+def _tri(i, j, D):
+    i = i.astype(np.int64)
+    return i * D - i * (i + 1) // 2 + (j.astype(np.int64) - i - 1)
+
+
+def _mist(ctx, D, i, j, w, s1sum):
+    tree = ctx.mst(D, i.astype(np.uint32), j.astype(np.uint32), w)
+    return s1sum - float(w[tree].sum()), set(zip(i[tree].tolist(), j[tree].tolist()))
+
+
+def screen_workload(args, hip):
+    """Learned pair screen against the plain path (one JSON line)."""
+    from centre_amd import screen
+    nb, S, F = 50, 10, 100000
+    n_sub, topk, err_budget = 2000, 2, 5e-4
+    ctx = hip.Context(0)
+    R = int(hip.lib().centre_pair_dofs_per_tile(nb))
+    t_start = time.perf_counter()
+    # training data: plain-path MI of seeded 1,000-DOF systems
+    feats, targets, held = [], [], []
+    n_hold = min(4, max(1, args.screen_train // 5))
+    for k in range(args.screen_train):
+        d_codes, d_ext, d_types = _system(ctx, hip, 0xCE27E + 100 + k, 1000, F, nb)
+        d_feat = screen.features(ctx, d_codes, F, d_types, d_ext, nb, n_sub)
+        mi, s1, _ = _pass(ctx, hip, d_codes, d_types, d_ext, 1000, F, nb, S)
+        if k < args.screen_train - n_hold:
+            feats.append(d_feat.numpy())
+            targets.append(screen.target(mi))
+        else:
+            held.append((d_feat.numpy(), mi, s1))
+        for a in (d_codes, d_ext, d_types, d_feat):
+            a.free()
+    ftr, ttr = np.concatenate(feats), np.concatenate(targets)
+    net = screen.Screen(ctx, screen.init_params(ftr, 1))
+    d_f, d_t = ctx.to_device(ftr), ctx.to_device(ttr)
+    t0 = time.perf_counter()
+    loss = net.train(d_f, d_t, len(ftr), epochs=4, batch=4096, lr=2e-3, seed=3)
+    t_train = time.perf_counter() - t0
+    d_f.free()
+    d_t.free()
+    # cutoff: the largest one whose MIST error on the held-back training
+    # systems stays within half the 0.1% budget
+    cut_grid = [1e-3, 5e-3, 8e-3, 1e-2, 1.1e-2, 1.2e-2, 1.3e-2, 1.4e-2, 1.5e-2, 1.6e-2, 1.7e-2,
+                1.8e-2, 2e-2]
+    iu, ju = np.triu_indices(1000, 1)
+    chosen = cut_grid[0]
+    tune = []
+    for cut in cut_grid:
+        worst = 0.0
+        frac = []
+        for fh, mih, s1h in held:
+            d_fh = ctx.to_device(fh)
+            d_ph = net.predict(d_fh, len(fh))
+            tl = ctx.screen_tiles(d_ph.ptr, 1000, nb, cut, topk)
+            d_fh.free()
+            d_ph.free()
+            i, j = _tile_pairs(tl, 1000, R)
+            full, _ = _mist(ctx, 1000, iu, ju, mih, s1h.sum())
+            scr, _ = _mist(ctx, 1000, i, j, mih[_tri(i, j, 1000)], s1h.sum())
+            worst = max(worst, abs(scr - full) / abs(full))
+            frac.append(len(i) / len(mih))
+        tune.append({"cutoff": cut, "held_max_rel_err": worst, "pairs_computed": float(np.mean(frac))})
+        if worst <= err_budget:
+            chosen = cut
+    # evaluation system
+    D = args.screen_dof
+    P = D * (D - 1) // 2
+    d_codes, d_ext, d_types = _system(ctx, hip, 0xCE27E + 4, D, F, nb)
+    iu, ju = np.triu_indices(D, 1)
+    mi, s1, t_pass = _pass(ctx, hip, d_codes, d_types, d_ext, D, F, nb, S)
+    t0 = time.perf_counter()
+    full_mist, full_tree = _mist(ctx, D, iu, ju, mi, s1.sum())
+    t_plain = t_pass + time.perf_counter() - t0
+
+    def screened(cut):
+        ctx.device_sync()
+        t0 = time.perf_counter()
+        d_feat = screen.features(ctx, d_codes, F, d_types, d_ext, nb, n_sub)
+        d_pred = net.predict(d_feat, P)
+        tiles = ctx.screen_tiles(d_pred.ptr, D, nb, cut, topk)
+        t_sel = time.perf_counter() - t0
+        d_feat.free()
+        d_pred.free()
+        mi_s, _, t_p = _pass(ctx, hip, d_codes, d_types, d_ext, D, F, nb, S, tiles=tiles)
+        t1 = time.perf_counter()
+        i, j = _tile_pairs(tiles, D, R)
+        mst, tree = _mist(ctx, D, i, j, mi_s[_tri(i, j, D)], s1.sum())
+        t_m = time.perf_counter() - t1
+        tot = t_sel + t_p + t_m
+        return {"cutoff": cut, "tiles": int(len(tiles)), "pairs_computed": len(i) / P,
+                "seconds": tot, "select_s": t_sel, "pass_s": t_p, "mst_s": t_m,
+                "speedup": t_plain / tot, "mist": mst,
+                "mist_rel_err": abs(mst - full_mist) / abs(full_mist),
+                "mst_edges_agree": len(tree & full_tree) / max(1, len(full_tree))}
+
+    at_spec = screened(1e-3)
+    at_tuned = screened(chosen)
+    out = {
+        "metric": "screened MIST speedup over the plain GPU path at <=0.1% S error",
+        "value": at_tuned["speedup"], "unit": "x", "higher_is_better": True,
+        "n_gpus": 1, "data": "synthetic (seeded Philox BAT generator, latent-coupled)",
+        "config": {"workload": f"learned pair screen, {D} DOF x {F} frames, {nb} bins, {S} steps, "
+                               f"ML, MIST; trained on {args.screen_train - n_hold} + "
+                               f"{n_hold} held-back 1,000-DOF systems",
+                   "n_sub": n_sub, "topk_partners": topk},
+        "plain": {"seconds": t_plain, "pass_s": t_pass, "mist": full_mist, "pairs": P},
+        "screen_at_1e-3": at_spec,
+        "screen_tuned": at_tuned,
+        "cutoff_tuning": tune,
+        "training": {"pairs": int(len(ftr)), "epochs": 4, "seconds": t_train,
+                     "loss_by_epoch": loss.tolist()},
+        "wall_s": time.perf_counter() - t_start,
+    }
+    print(json.dumps(out), flush=True)
     ctx.close()
 
 

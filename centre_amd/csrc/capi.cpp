@@ -231,6 +231,45 @@ int centre_dev_dof_entropy(centre_ctx* ctx, const uint8_t* d_codes,
 }
 
 // ---------------------------------------------------------------------------
+int centre_pair_dofs_per_tile(int nbins) {
+    return (nbins < 1 || nbins > 255) ? -3 : dofs_per_tile(nbins);
+}
+
+int centre_pair_plan_from_tiles(centre_ctx* ctx, uint32_t n_dof, int nbins, const int32_t* tiles,
+                                uint64_t n_tiles, centre_pair_plan** out) {
+    if (!ctx || !out) return -3;
+    *out = nullptr;
+    if (nbins < 1 || nbins > 255) return fail(ctx, -3, "nbins must be in 1..255");
+    if (n_dof < 2) return fail(ctx, -3, "need at least two DOFs");
+    if (n_tiles && !tiles) return fail(ctx, -3, "null tile list");
+    const int R = dofs_per_tile(nbins);
+    const int64_t nblk = (n_dof + R - 1) / R;
+    std::vector<int2> tl(n_tiles);
+    for (uint64_t t = 0; t < n_tiles; ++t) {
+        const int32_t I = tiles[2 * t], J = tiles[2 * t + 1];
+        if (I < 0 || J < I || J >= nblk) return fail(ctx, -3, "tile index out of range or row > column");
+        tl[t] = make_int2(I, J);
+    }
+    HIPCHK(hipSetDevice(ctx->device));
+    auto* plan = new centre_pair_plan();
+    plan->n_dof = n_dof;
+    plan->nbins = (uint8_t)nbins;
+    plan->dofs_per_tile = R;
+    plan->all_pairs = true;
+    plan->n_canon = (uint64_t)n_dof * (n_dof - 1) / 2;
+    plan->n_rec = plan->n_canon;
+    plan->n_tiles = (int)tl.size();
+    hipError_t e = hipMalloc(&plan->d_tiles, sizeof(int2) * (tl.size() ? tl.size() : 1));
+    if (e == hipSuccess && !tl.empty())
+        e = hipMemcpy(plan->d_tiles, tl.data(), sizeof(int2) * tl.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        centre_pair_plan_destroy(plan);
+        return fail(ctx, (int)e, std::string("plan tiles: ") + hipGetErrorString(e));
+    }
+    *out = plan;
+    return 0;
+}
+
 int centre_pair_plan_create(centre_ctx* ctx, uint32_t n_dof, int nbins,
                             const uint32_t* pairs, uint64_t n_pair,
                             centre_pair_plan** out) {

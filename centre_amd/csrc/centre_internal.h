@@ -178,3 +178,35 @@ hipError_t launch_segment_sum(const double* x, const uint64_t* d_beg, int n_seg,
 hipError_t launch_gather_rec(const double* src, const uint64_t* rec2canon,
                              uint64_t n_rec, int width, double* dst,
                              hipStream_t st);
+
+// ---- learned pair screen (screen.hip) ----------------------------------------
+constexpr int kScreenFeat = 8;     // features per pair
+constexpr int kScreenHidden = 64;  // width of both hidden layers
+// parameter vector (floats): W1[H][F], b1[H], W2[H][H], b2[H], w3[H], b3, mu[F], sd[F]
+constexpr int kOffW1 = 0;
+constexpr int kOffB1 = kOffW1 + kScreenHidden * kScreenFeat;
+constexpr int kOffW2 = kOffB1 + kScreenHidden;
+constexpr int kOffB2 = kOffW2 + kScreenHidden * kScreenHidden;
+constexpr int kOffW3 = kOffB2 + kScreenHidden;
+constexpr int kOffB3 = kOffW3 + kScreenHidden;
+constexpr int kOffMu = kOffB3 + 1;
+constexpr int kOffSd = kOffMu + kScreenFeat;
+constexpr int kScreenParams = kOffSd + kScreenFeat;
+hipError_t launch_subsample(const uint8_t* codes, uint64_t stride, uint32_t D, uint64_t step,
+                            uint32_t n_sub, uint8_t* out, hipStream_t st);
+hipError_t launch_dof_moments(const uint8_t* sub, uint32_t D, uint32_t n_sub, int nb, float* X,
+                              float* Y, hipStream_t st);
+hipError_t launch_corr(const float* X, const float* Y, uint32_t D, uint32_t n, const int2* blocks,
+                       int n_blocks, float* lin, float* circ, hipStream_t st);
+int corr_block_size();
+hipError_t launch_features(uint32_t D, const double* s1, const uint8_t* types, const double* mi_sub,
+                           const float* lin, const float* circ, float* feat, hipStream_t st);
+hipError_t launch_mlp_forward(const float* params, const float* feat, uint64_t n, float* out,
+                              hipStream_t st);
+hipError_t launch_mlp_grad(const float* params, const float* feat, const float* target,
+                           const uint32_t* idx, uint64_t n, float* grad, double* loss,
+                           hipStream_t st);
+hipError_t launch_tile_select(const float* pred, uint32_t D, int R, int k, float* tmax,
+                              uint8_t* mark, hipStream_t st);
+hipError_t launch_adam(float* params, const float* grad, float* m, float* v, float scale,
+                       float lr, int t, hipStream_t st);

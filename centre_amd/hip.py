@@ -27,6 +27,14 @@ _SIGS = {
     "centre_sync": ([P], I32),
     "centre_stream": ([P], P),
     "centre_last_pair_kernel_ms": ([P], C.c_float),
+    "centre_screen_n_params": ([], I32),
+    "centre_screen_tiles": ([P, P, U32, I32, C.c_float, I32, P, P], I32),
+    "centre_pair_plan_from_tiles": ([P, U32, I32, P, U64, P], I32),
+    "centre_pair_dofs_per_tile": ([I32], I32),
+    "centre_screen_features": ([P, P, U64, U32, U64, P, P, I32, U32, P], I32),
+    "centre_mlp_forward": ([P, P, P, U64, P], I32),
+    "centre_mlp_loss_grad": ([P, P, P, P, U64, P, P], I32),
+    "centre_mlp_train": ([P, P, P, P, U64, I32, I32, C.c_float, U64, P], I32),
     "centre_dev_discretize_hist": ([P, P, U64, U32, U64, I32, P, I32, I32, I32, P, P], I32),
     "centre_discretize_hist": ([P, P, U64, U32, U64, I32, P, I32, I32, I32, P, P], I32),
     "centre_dev_discretize_vmkde": ([P, P, U64, U32, U64, I32, F64, U32, F64, U32, U32,
@@ -156,6 +164,36 @@ class Context:
         d = DevArray(self, x.shape, x.dtype)
         d.upload(x)
         return d
+
+    # -- learned pair screen (device pointers) ---------------------------------
+    def screen_features(self, d_codes, stride, n_dof, n_frame, d_types, d_ext, nbins,
+                        n_sub, d_feat):
+        self.check(self._L.centre_screen_features(self.h, d_codes, stride, n_dof, n_frame,
+                                                  d_types, d_ext, nbins, n_sub, d_feat))
+
+    def screen_tiles(self, d_pred, n_dof, nbins, cutoff, topk) -> np.ndarray:
+        R = int(self._L.centre_pair_dofs_per_tile(nbins))
+        nblk = (n_dof + R - 1) // R
+        buf = np.zeros((nblk * (nblk + 1) // 2, 2), np.int32)
+        n = C.c_uint64(0)
+        self.check(self._L.centre_screen_tiles(self.h, d_pred, n_dof, nbins, cutoff, topk, _p(buf),
+                                               C.byref(n)))
+        return buf[:n.value].copy()
+
+    def mlp_forward(self, d_params, d_feat, n, d_out):
+        self.check(self._L.centre_mlp_forward(self.h, d_params, d_feat, n, d_out))
+
+    def mlp_loss_grad(self, d_params, d_feat, d_target, n, d_grad) -> float:
+        loss = C.c_double(0.0)
+        self.check(self._L.centre_mlp_loss_grad(self.h, d_params, d_feat, d_target, n, d_grad,
+                                                C.byref(loss)))
+        return loss.value
+
+    def mlp_train(self, d_params, d_feat, d_target, n, epochs, batch, lr, seed) -> np.ndarray:
+        hist = np.zeros(epochs, np.float64)
+        self.check(self._L.centre_mlp_train(self.h, d_params, d_feat, d_target, n, epochs,
+                                            batch, lr, seed, _p(hist)))
+        return hist
 
     def copy_d2d(self, dst: int, src: int, nbytes: int):
         self.check(self._L.centre_memcpy_d2d(self.h, dst, src, nbytes))
@@ -308,9 +346,16 @@ class DevArray:
 class PairPlan:
     """centre_pair_plan: the pair set and tiling of one pair-stage call."""
 
-    def __init__(self, ctx: Context, n_dof: int, nbins: int, pairs=None):
+    def __init__(self, ctx: Context, n_dof: int, nbins: int, pairs=None, tiles=None):
         self.ctx = ctx
         h = C.c_void_p()
+        if tiles is not None:
+            tl = _c(tiles, np.int32).reshape(-1, 2)
+            ctx.check(ctx._L.centre_pair_plan_from_tiles(ctx.h, n_dof, nbins, _p(tl), len(tl),
+                                                         C.byref(h)))
+            self.h = h
+            self._keep = tl
+            return
         pp = None if pairs is None else _c(pairs, np.uint32).reshape(-1, 2)
         ctx.check(ctx._L.centre_pair_plan_create(
             ctx.h, n_dof, nbins, _p(pp), 0 if pp is None else len(pp),

@@ -110,6 +110,15 @@ int centre_dev_dof_entropy(centre_ctx *ctx, const uint8_t *d_codes,
 int centre_pair_plan_create(centre_ctx *ctx, uint32_t n_dof, int nbins,
                             const uint32_t *pairs, uint64_t n_pair,
                             centre_pair_plan **out);
+/* A plan over every i < j pair of a subset of DOF tiles: tiles[n_tiles][2]
+ * are (row block, column block) indices with row <= column, a block being
+ * centre_pair_dofs_per_tile(nbins) consecutive DOFs.  Outputs keep the all-
+ * pairs triangle layout; entries of pairs outside the listed tiles are not
+ * written.  No reference counterpart (used by the learned pair screen). */
+int centre_pair_plan_from_tiles(centre_ctx *ctx, uint32_t n_dof, int nbins,
+                                const int32_t *tiles, uint64_t n_tiles,
+                                centre_pair_plan **out);
+int centre_pair_dofs_per_tile(int nbins);
 void centre_pair_plan_destroy(centre_pair_plan *plan);
 uint64_t centre_pair_plan_records(const centre_pair_plan *plan);
 int centre_pair_plan_tiles(const centre_pair_plan *plan);
@@ -175,6 +184,46 @@ int centre_memcpy_d2d(centre_ctx *ctx, void *d_dst, const void *d_src,
                       size_t bytes);
 /* hipDeviceSynchronize on the context's device. */
 int centre_device_sync(centre_ctx *ctx);
+
+/* ---- learned pair screen ------------------------------------------------------
+ * No reference counterpart (the reference restricts pairs only by a user
+ * neighbour list, Centre.cpp:459-593).  An MLP 8 -> 64 -> 64 -> 1 (FP32, ReLU)
+ * predicts log(MI) of a pair from 8 features; callers skip pairs whose
+ * predicted MI is below a cutoff.  Parameter vector (centre_screen_n_params()
+ * floats): W1[64][8], b1[64], W2[64][64], b2[64], w3[64], b3, mu[8], sd[8]
+ * (inputs are standardised as (f - mu) / sd; training leaves mu, sd alone).
+ *
+ * centre_screen_features: d_feat[P][8] (device, FP32) for all P = D(D-1)/2
+ * pairs i < j in triangle order: S1_i, S1_j (ML, all n_frame frames),
+ * circular and linear correlation and ML MI on n_sub evenly strided frames
+ * (n_sub a multiple of 4), type_i / 3, type_j / 3, same-type flag.
+ * centre_mlp_forward: d_out[n] = predicted log(MI).
+ * centre_mlp_loss_grad: d_grad = gradient of sum 0.5 (y - t)^2 over the n
+ * samples (FP32, accumulated with atomics), *loss = that sum.
+ * centre_mlp_train: Adam (beta 0.9 / 0.999), minibatches of `batch` samples
+ * in a seeded random order, `epochs` passes; epoch_loss[e] = mean loss. */
+int centre_screen_n_params(void);
+/* Tiles of the pair stage to run after screening: every tile (row block <=
+ * column block, blocks of centre_pair_dofs_per_tile(nbins) DOFs) holding a
+ * pair whose predicted log MI (d_pred, triangle order of all pairs) reaches
+ * log(cutoff), plus the tiles holding each DOF's topk (<= 4) best predicted
+ * partners.  tiles[] needs room for nblk*(nblk+1)/2 pairs of int32. */
+int centre_screen_tiles(centre_ctx *ctx, const float *d_pred, uint32_t n_dof,
+                        int nbins, float cutoff, int topk, int32_t *tiles,
+                        uint64_t *n_tiles);
+int centre_screen_features(centre_ctx *ctx, const uint8_t *d_codes,
+                           uint64_t code_stride, uint32_t n_dof,
+                           uint64_t n_frame, const uint8_t *d_types,
+                           const double *d_extrema, int nbins, uint32_t n_sub,
+                           float *d_feat);
+int centre_mlp_forward(centre_ctx *ctx, const float *d_params,
+                       const float *d_feat, uint64_t n, float *d_out);
+int centre_mlp_loss_grad(centre_ctx *ctx, const float *d_params,
+                         const float *d_feat, const float *d_target,
+                         uint64_t n, float *d_grad, double *loss);
+int centre_mlp_train(centre_ctx *ctx, float *d_params, const float *d_feat,
+                     const float *d_target, uint64_t n, int epochs, int batch,
+                     float lr, uint64_t seed, double *epoch_loss);
 
 /* ---- synthetic input ---------------------------------------------------------
  * No reference counterpart (test and benchmark input).  Fills out[n_frame]
