@@ -392,6 +392,234 @@ std::vector<uint32_t> canon_pairs(const centre_pair_plan* plan, const int* pm_ho
 
 }  // namespace
 
+namespace {
+
+// The pair kernel stages codes in 4-byte words, so every step of every code
+// row must start on a 4-byte boundary.  Codes laid out otherwise (a row
+// stride or a step length that is not a multiple of 4) are copied once to
+// rows of nsteps steps of round_up(ss, 4) frames.
+int step_aligned(centre_ctx* ctx, const uint8_t* codes, uint64_t stride, uint32_t D, uint64_t ss,
+                 int S, DevBuf& buf, const uint8_t** out, uint64_t* out_stride,
+                 uint64_t* out_sstride) {
+    if ((stride & 3) == 0 && (ss & 3) == 0 && (reinterpret_cast<uintptr_t>(codes) & 3) == 0) {
+        *out = codes;
+        *out_stride = stride;
+        *out_sstride = ss;
+        return 0;
+    }
+    const uint64_t ss4 = (ss + 3) & ~3ull;
+    HIPCHK(buf.alloc((size_t)D * S * ss4));
+    HIPCHK(launch_repack(codes, stride, D, ss, S, ss4, buf.as<uint8_t>(), ctx->stream));
+    *out = buf.as<uint8_t>();
+    *out_stride = (uint64_t)S * ss4;
+    *out_sstride = ss4;
+    return 0;
+}
+
+// Tile list of a batch: entries (row block, column block, system).
+struct BatchTile {
+    int I, J, sys;
+    double cost;   // frames x (1/2 on the diagonal)
+};
+
+// order: 1 = longest tiles first; 2 = system by system (row-major tiles);
+// 3 = systems by total cost, largest first, each system's tiles together.
+std::vector<BatchTile> batch_tiles(const centre_batch_system* sys, int n_sys, int R, int nsteps,
+                                   int order, uint64_t max_frames_per_step, double sample) {
+    std::vector<BatchTile> out;
+    std::vector<double> sys_cost(n_sys, 0.0);
+    for (int k = 0; k < n_sys; ++k) {
+        const int nblk = (int)((sys[k].n_dof + R - 1) / R);
+        const double frames =
+            (double)std::min<uint64_t>(sys[k].n_frame / nsteps, max_frames_per_step) * nsteps;
+        const size_t first = out.size();
+        for (int I = 0; I < nblk; ++I)
+            for (int J = I; J < nblk; ++J)
+                out.push_back({I, J, k, frames * (I == J ? 0.5 : 1.0)});
+        if (sample < 1.0) {   // calibration: an evenly spread share of the system's tiles
+            std::vector<BatchTile> keep;
+            const size_t n = out.size() - first;
+            const size_t m = std::max<size_t>(1, (size_t)(n * sample));
+            for (size_t q = 0; q < m; ++q) keep.push_back(out[first + q * n / m]);
+            out.resize(first);
+            out.insert(out.end(), keep.begin(), keep.end());
+        }
+        for (size_t q = first; q < out.size(); ++q) sys_cost[k] += out[q].cost;
+    }
+    if (order == 1)
+        std::stable_sort(out.begin(), out.end(),
+                         [](const BatchTile& x, const BatchTile& y) { return x.cost > y.cost; });
+    else if (order == 3)
+        std::stable_sort(out.begin(), out.end(), [&](const BatchTile& x, const BatchTile& y) {
+            return sys_cost[x.sys] > sys_cost[y.sys] ||
+                   (sys_cost[x.sys] == sys_cost[y.sys] && x.sys < y.sys);
+        });
+    return out;
+}
+
+struct BatchRun {
+    double total_ms = 0.0;
+    std::vector<double> sys_ms;
+};
+
+// One persistent launch over a prepared tile list.
+int batch_launch(centre_ctx* ctx, const PairKernelArgs& base, const std::vector<BatchSystem>& bs,
+                 const std::vector<BatchTile>& tl, int grid, BatchRun* run) {
+    const int n_sys = (int)bs.size();
+    std::vector<int2> tiles(tl.size());
+    std::vector<int> tsys(tl.size()), per(n_sys, 0);
+    for (size_t q = 0; q < tl.size(); ++q) {
+        tiles[q] = make_int2(tl[q].I, tl[q].J);
+        tsys[q] = tl[q].sys;
+        per[tl[q].sys]++;
+    }
+    DevBuf dt, ds, dsys, dper, dcnt, dtime, dwork;
+    HIPCHK(dt.alloc(sizeof(int2) * tiles.size()));
+    HIPCHK(ds.alloc(sizeof(int) * tsys.size()));
+    HIPCHK(dsys.alloc(sizeof(BatchSystem) * n_sys));
+    HIPCHK(dper.alloc(sizeof(int) * n_sys));
+    HIPCHK(dcnt.alloc(sizeof(int) * n_sys));
+    HIPCHK(dtime.alloc(sizeof(unsigned long long) * (n_sys + 1)));
+    HIPCHK(dwork.alloc(sizeof(int)));
+    HIPCHK(hipMemcpyAsync(dt.p, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ds.p, tsys.data(), sizeof(int) * tsys.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dsys.p, bs.data(), sizeof(BatchSystem) * n_sys, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dper.p, per.data(), sizeof(int) * n_sys, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemsetAsync(dcnt.p, 0, sizeof(int) * n_sys, ctx->stream));
+    HIPCHK(hipMemsetAsync(dtime.p, 0, sizeof(unsigned long long) * (n_sys + 1), ctx->stream));
+    HIPCHK(hipMemsetAsync(dwork.p, 0, sizeof(int), ctx->stream));
+    PairKernelArgs a = base;
+    a.systems = dsys.as<BatchSystem>();
+    a.tiles = dt.as<int2>();
+    a.tile_sys = ds.as<int>();
+    a.work = dwork.as<int>();
+    a.n_work = (int)tiles.size();
+    a.sys_count = dcnt.as<int>();
+    a.sys_tiles = dper.as<int>();
+    a.sys_time = dtime.as<unsigned long long>() + 1;
+    a.t0 = dtime.as<unsigned long long>();
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    HIPCHK(launch_pair_persistent(a, grid, ctx->stream));
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    std::vector<unsigned long long> stamps(n_sys + 1);
+    HIPCHK(hipMemcpyAsync(stamps.data(), dtime.p, sizeof(unsigned long long) * (n_sys + 1),
+                          hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    int khz = 100000;
+    (void)hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device);
+    run->total_ms = ms;
+    run->sys_ms.assign(n_sys, 0.0);
+    for (int k = 0; k < n_sys; ++k)
+        run->sys_ms[k] = stamps[k + 1] >= stamps[0] && per[k]
+                             ? (double)(stamps[k + 1] - stamps[0]) / (double)khz
+                             : 0.0;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int centre_batch_pair_entropy(centre_ctx* ctx, const centre_batch_system* sys, int n_sys,
+                                         int nbins, int nsteps, const int* ests, int nest,
+                                         int jacobian, int kde, int order, double* sys_ms,
+                                         double* total_ms, int* order_used) {
+    if (!ctx) return -3;
+    if (!sys || n_sys < 1) return fail(ctx, -3, "no systems");
+    if (nbins < 1 || nbins > 255) return fail(ctx, -3, "nbins must be in 1..255");
+    if (nsteps < 1 || nsteps > 255) return fail(ctx, -3, "nsteps must be in 1..255");
+    if (!ests || nest < 1) return fail(ctx, -3, "no estimators");
+    if (order < 0 || order > 3) return fail(ctx, -3, "order must be 0 (autotune) .. 3");
+    int slot_ml = -1, slot_mm = -1;
+    for (int e = 0; e < nest; ++e) {
+        if (ests[e] == 0 && slot_ml < 0) slot_ml = e;
+        else if (ests[e] == 1 && slot_mm < 0) slot_mm = e;
+        else return fail(ctx, -3, "batched pair stage supports the ML and MM estimators");
+    }
+    for (int k = 0; k < n_sys; ++k) {
+        const centre_batch_system& b = sys[k];
+        if (!b.d_codes || !b.d_types || !b.d_extrema || !b.d_s2)
+            return fail(ctx, -3, "null pointer in system " + std::to_string(k));
+        if (b.n_dof < 2) return fail(ctx, -3, "system " + std::to_string(k) + " has < 2 DOFs");
+        if (b.n_frame < (uint64_t)nsteps || b.code_stride < b.n_frame)
+            return fail(ctx, -3, "system " + std::to_string(k) + ": bad n_frame or code_stride");
+    }
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipMemsetAsync(ctx->d_flag, 0, sizeof(int), ctx->stream));
+    // marginals of every system
+    std::vector<DevBuf> s1(n_sys), L(n_sys), w(n_sys), s1e0(n_sys), packed(n_sys);
+    std::vector<BatchSystem> bs(n_sys);
+    for (int k = 0; k < n_sys; ++k) {
+        const centre_batch_system& b = sys[k];
+        const uint64_t D = b.n_dof;
+        HIPCHK(s1[k].alloc(sizeof(double) * D * nest * nsteps));
+        HIPCHK(L[k].alloc(sizeof(double) * D * nsteps));
+        HIPCHK(w[k].alloc(sizeof(double) * D));
+        if (int r = run_dof1d(ctx, b.d_codes, b.code_stride, b.n_dof, b.n_frame, b.d_types,
+                              b.d_extrema, nbins, nsteps, ests, nest, jacobian, kde,
+                              s1[k].as<double>(), L[k].as<double>(), w[k].as<double>(), nullptr))
+            return r;
+        if (b.d_s1)
+            HIPCHK(hipMemcpyAsync(b.d_s1, s1[k].p, sizeof(double) * D * nest * nsteps,
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+        const double* s1_first = s1[k].as<double>();
+        if (nest > 1) {   // the kernel reads [D][S] of the first estimator
+            HIPCHK(s1e0[k].alloc(sizeof(double) * D * nsteps));
+            HIPCHK(hipMemcpy2DAsync(s1e0[k].p, sizeof(double) * nsteps, s1[k].p,
+                                    sizeof(double) * nsteps * nest, sizeof(double) * nsteps, D,
+                                    hipMemcpyDeviceToDevice, ctx->stream));
+            s1_first = s1e0[k].as<double>();
+        }
+        const uint8_t* pc = nullptr;
+        uint64_t pstride = 0, psstride = 0;
+        if (int r = step_aligned(ctx, b.d_codes, b.code_stride, b.n_dof, b.n_frame / nsteps, nsteps,
+                                 packed[k], &pc, &pstride, &psstride))
+            return r;
+        bs[k] = BatchSystem{pc, pstride, (int)b.n_dof, b.n_frame / nsteps, psstride, b.d_s2,
+                            b.d_mi, w[k].as<double>(), L[k].as<double>(), s1_first};
+    }
+    PairKernelArgs base{};
+    base.nb = nbins;
+    base.R = dofs_per_tile(nbins);
+    base.nsteps = nsteps;
+    base.dof_w = nullptr;
+    base.slot_ml = slot_ml;
+    base.slot_mm = slot_mm;
+    base.nest = nest;
+    base.mi_from_mm = ests[0] == 1;
+    base.flag = ctx->d_flag;
+    int dev_cu = 0;
+    HIPCHK(hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    const int grid = dev_cu;   // one workgroup per CU: the tile's LDS fills most of a CU
+    // autotune: time each order on a calibration share of the batch
+    int chosen = order;
+    if (order == 0) {
+        double best = 1e300;
+        for (int cand = 1; cand <= 3; ++cand) {
+            auto tl = batch_tiles(sys, n_sys, base.R, nsteps, cand, 4096, 0.05);
+            // calibration outputs land in the real output buffers and are
+            // overwritten by the full run below
+            BatchRun run;
+            std::vector<BatchSystem> cbs = bs;
+            for (auto& c : cbs) c.ss = std::min<uint64_t>(c.ss, 4096);
+            if (int r = batch_launch(ctx, base, cbs, tl, grid, &run)) return r;
+            if (run.total_ms < best) {
+                best = run.total_ms;
+                chosen = cand;
+            }
+        }
+    }
+    auto tl = batch_tiles(sys, n_sys, base.R, nsteps, chosen, ~0ull, 1.0);
+    BatchRun run;
+    if (int r = batch_launch(ctx, base, bs, tl, grid, &run)) return r;
+    if (sys_ms)
+        for (int k = 0; k < n_sys; ++k) sys_ms[k] = run.sys_ms[k];
+    if (total_ms) *total_ms = run.total_ms;
+    if (order_used) *order_used = chosen;
+    ctx->last_pair_ms = (float)run.total_ms;
+    return check_flag(ctx);
+}
+
 extern "C" int centre_dev_pair_entropy(centre_ctx* ctx, const centre_pair_plan* plan,
                                        const uint8_t* d_codes, uint64_t code_stride,
                                        uint64_t n_frame, const uint8_t* d_types,
@@ -457,8 +685,10 @@ extern "C" int centre_dev_pair_entropy(centre_ctx* ctx, const centre_pair_plan* 
     }
 
     PairKernelArgs a{};
-    a.codes = d_codes;
-    a.stride = code_stride;
+    DevBuf packed;
+    if (int r = step_aligned(ctx, d_codes, code_stride, D, ss, nsteps, packed, &a.codes, &a.stride,
+                             &a.sstride))
+        return r;
     a.n_dof = (int)D;
     a.nb = nb;
     a.R = plan->dofs_per_tile;

@@ -45,6 +45,21 @@ inline int dofs_per_tile(int nb) {
     return r > 60 ? 60 : r;
 }
 
+// One system of a batched pair stage (device pointers; same nbins and steps
+// for every system of a batch).
+struct BatchSystem {
+    const uint8_t* codes;
+    uint64_t stride;
+    int n_dof;
+    uint64_t ss;
+    uint64_t sstride;
+    double* s2;
+    double* mi;
+    const double* dof_w;
+    const double* dof_L;
+    const double* dof_S1;
+};
+
 struct PairKernelArgs {
     const uint8_t* codes;   // [D] rows of `stride` bytes; frame f at +f
     uint64_t stride;
@@ -53,6 +68,7 @@ struct PairKernelArgs {
     int R;
     int nsteps;
     uint64_t ss;            // frames per convergence step
+    uint64_t sstride;       // code-row offset between step starts (>= ss)
     const int2* tiles;
     const int* pairmap;     // null => all i<j, canonical index by triangle
     const double* dof_w;    // [D] bin width (1.0 under kde)
@@ -66,6 +82,15 @@ struct PairKernelArgs {
     int mi_from_mm;         // MI uses the MM S2 (when MM is the first estimator)
     uint32_t* counts;       // dump mode: [P][S][nb*nb] uint32 prefix counts
     int* flag;              // error word: bit0 = code >= nbins
+    // batch (persistent launch): per-entry system of the tile list
+    const BatchSystem* systems;   // null = single system in the fields above
+    const int* tile_sys;          // [n_work] system of each tile list entry
+    int* work;                    // shared tile counter (starts at 0)
+    int n_work;                   // tile list entries
+    int* sys_count;               // [n_sys] finished tiles (start at 0), or null
+    const int* sys_tiles;         // [n_sys] tiles per system
+    unsigned long long* sys_time; // [n_sys] completion stamp (100 MHz)
+    unsigned long long* t0;       // launch stamp
 };
 
 
@@ -73,6 +98,9 @@ struct PairKernelArgs {
 hipError_t launch_pair_tiles(const PairKernelArgs& a, int n_tiles, bool dump,
                              hipStream_t st);
 size_t pair_tile_lds_bytes(int R);
+hipError_t launch_pair_persistent(const PairKernelArgs& a, int grid, hipStream_t st);
+hipError_t launch_repack(const uint8_t* codes, uint64_t stride, uint32_t D, uint64_t ss, int S,
+                         uint64_t ss4, uint8_t* out, hipStream_t st);
 
 struct Dof1dArgs {
     const uint8_t* codes;   // [D] rows of `stride` bytes

@@ -38,6 +38,8 @@
 // skipped by a wave-uniform mask.
 #include "centre_internal.h"
 
+#include <type_traits>
+
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
@@ -113,9 +115,27 @@ __device__ inline long long tri_index(long long i, long long j, long long D) {
     return i * D - i * (i + 1) / 2 + (j - i - 1);
 }
 
-template <bool kDump, int kSlots>
-__global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
+// One tile: all DOF pairs of row block tile.x x column block tile.y of
+// tile list entry t, over every frame and step.  With a batch (a_in.systems)
+// the entry belongs to system a_in.tile_sys[t], whose codes, sizes and
+// outputs replace those of a_in.
+template <bool kDump, int kSlots, bool kBatch>
+__device__ __forceinline__ void tile_body(const PairKernelArgs& a_in, int t) {
     extern __shared__ __align__(16) uint8_t smem_raw[];
+    PairKernelArgs a = a_in;
+    if constexpr (kBatch) {
+        const BatchSystem& sd = a_in.systems[a_in.tile_sys[t]];
+        a.codes = sd.codes;
+        a.stride = sd.stride;
+        a.n_dof = sd.n_dof;
+        a.ss = sd.ss;
+        a.sstride = sd.sstride;
+        a.s2 = sd.s2;
+        a.mi = sd.mi;
+        a.dof_w = sd.dof_w;
+        a.dof_L = sd.dof_L;
+        a.dof_S1 = sd.dof_S1;
+    }
     const int R = a.R, nb = a.nb, S = a.nsteps;
     const LdsLayout lay = lds_layout(R);
     const size_t omap = lay.map;
@@ -134,7 +154,7 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform
     const int wr = wave >> 1, wc = wave & 1;
-    const int2 tile = a.tiles[blockIdx.x];
+    const int2 tile = a.tiles[t];
     const int r0 = tile.x * R, c0 = tile.y * R;
     const int nr = min(R, a.n_dof - r0), nc = min(R, a.n_dof - c0);
     const bool diag = tile.x == tile.y;
@@ -181,7 +201,9 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
 
     const int nch = (int)((a.ss + kChunk - 1) / kChunk);
     const int G = S * nch;
-    const bool aligned = ((a.stride & 3) == 0) && ((a.ss & 3) == 0) &&
+    // dword staging needs every step to start on a 4-byte boundary; the
+    // step length itself may be any number of frames
+    const bool aligned = ((a.stride & 3) == 0) && ((a.sstride & 3) == 0) &&
                          ((reinterpret_cast<uintptr_t>(a.codes) & 3) == 0);
 
     // ---- code staging --------------------------------------------------------
@@ -198,8 +220,8 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
             uint64_t f0 = 0;
             if (g < G) {
                 const int s = g / nch, cc = g - s * nch;
-                f0 = (uint64_t)s * a.ss + (uint64_t)cc * kChunk;
-                nvalid = (int)min((uint64_t)kChunk, (uint64_t)(s + 1) * a.ss - f0);
+                f0 = (uint64_t)s * a.sstride + (uint64_t)cc * kChunk;
+                nvalid = (int)min((uint64_t)kChunk, a.ss - (uint64_t)cc * kChunk);
             }
 #pragma unroll
             for (int k = 0; k < kSlots; ++k) {
@@ -227,8 +249,8 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
         const int b = g & 1;
         const int cis = g - (g / kSuper) * kSuper;   // chunk within super-chunk
         const int sg = g / nch, cg = g - sg * nch;
-        const int nvalid = (int)min((uint64_t)kChunk,
-                                    (uint64_t)(sg + 1) * a.ss - ((uint64_t)sg * a.ss + (uint64_t)cg * kChunk));
+        const int nvalid = (int)min((uint64_t)kChunk, a.ss - (uint64_t)cg * kChunk);
+        (void)sg;
 #pragma unroll
         for (int k = 0; k < kSlots; ++k) {
             const int q = tid + 256 * k;
@@ -521,7 +543,7 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
             auto load4 = [&](uint32_t (&d)[kSlots][kSuper]) __attribute__((always_inline)) {
 #pragma unroll
                 for (int c = 0; c < kSuper; ++c) {
-                    const uint64_t f0 = (uint64_t)ls * a.ss + (uint64_t)lc * kChunk;
+                    const uint64_t f0 = (uint64_t)ls * a.sstride + (uint64_t)lc * kChunk;
                     const int nv = nvalid_of(ls, lc);
 #pragma unroll
                     for (int k = 0; k < kSlots; ++k) {
@@ -532,14 +554,18 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
                     advance(ls, lc, nch);
                 }
             };
-            // chunk with nv valid frames (a multiple of 4) from dword cis of
-            // cw into buffer b: clear the bytes the slot set two chunks ago,
-            // set the new ones.  A lane whose 4 frames lie beyond the chunk
-            // sets code 0 in the trash row.
-            auto stage = [&](int b, int cis, int nv, const uint32_t (&cw)[kSlots][kSuper])
+            // chunk with nv valid frames from dword cis of cw into buffer b:
+            // clear the bytes the slot set two chunks ago, set the new ones.
+            // Frames beyond the chunk (the tail of a step) set code 0 in the
+            // trash row and are remembered as code 0.
+            // kFull: all kChunk frames valid (every chunk but a step's last).
+            auto stage = [&](auto full, int b, int cis, int nv, const uint32_t (&cw)[kSlots][kSuper])
                 __attribute__((always_inline)) {
+                constexpr bool kFull = decltype(full)::value;
                 uint8_t* const buf = sm.oh + (size_t)b * 2 * kOpBytes;
-                const bool in = k4 < nv;
+                const int nvk = kFull ? 4 : nv - k4;   // valid frames of this lane's 4
+                const uint32_t vmask = nvk >= 4 ? 0xFFFFFFFFu
+                                                : (nvk <= 0 ? 0u : 0xFFFFFFFFu >> (8 * (4 - nvk)));
 #pragma unroll
                 for (int k = 0; k < kSlots; ++k) {
                     // select dword cis with masks (a select chain on a runtime
@@ -547,14 +573,16 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
                     uint32_t wv = 0;
 #pragma unroll
                     for (int c = 0; c < kSuper; ++c) wv |= cw[k][c] & (0u - (uint32_t)(cis == c));
-                    wv = in ? wv : 0u;
+                    if constexpr (!kFull) wv &= vmask;
                     const uint32_t old = b ? last1[k] : last0[k];
                     uint8_t* const row = buf + dst[k];
 #pragma unroll
                     for (int e = 0; e < 4; ++e) row[((old >> (8 * e)) & 0xFFu) * kLdsRow + e] = 0;
-                    uint8_t* const srow = in ? row : trash + k4;
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) srow[((wv >> (8 * e)) & 0xFFu) * kLdsRow + e] = 1;
+                    for (int e = 0; e < 4; ++e) {
+                        uint8_t* const srow = (kFull || nvk > e) ? row : trash + k4;
+                        srow[((wv >> (8 * e)) & 0xFFu) * kLdsRow + e] = 1;
+                    }
                     last1[k] = b ? wv : last1[k];
                     last0[k] = b ? last0[k] : wv;
                 }
@@ -585,14 +613,14 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
             __syncthreads();
             load4(cur);
             load4(nxt);
-            stage(0, 0, nvalid_of(ps, pc), cur);
+            stage(std::false_type{}, 0, 0, nvalid_of(ps, pc), cur);
             advance(ps, pc, nch);
             __syncthreads();
             v4i xa0[4], xb0[4], xa1[4], xb1[4];   // fragments of a chunk in buffer 0
             v4i ya0[4], yb0[4], ya1[4], yb1[4];   // fragments of a chunk in buffer 1
             frags(sm.oh, 0, xa0, xb0);
             frags(sm.oh, 1, xa1, xb1);
-            stage(1, 1, nvalid_of(ps, pc), cur);
+            stage(std::false_type{}, 1, 1, nvalid_of(ps, pc), cur);
             advance(ps, pc, nch);
             mfma16(xa0, xb0);
             __syncthreads();
@@ -633,7 +661,7 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
                     epilogue(ms, parkf, -1);
                 }
                 advance(ms, mc, nch);
-                stage(b, (g + 2) & (kSuper - 1), nvalid_of(ps, pc), cur);
+                stage(std::false_type{}, b, (g + 2) & (kSuper - 1), nvalid_of(ps, pc), cur);
                 advance(ps, pc, nch);
                 mfma16(na0, nb0);
                 // one MFMA at a time with a share of the staging VALU work and
@@ -692,7 +720,57 @@ __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
     }
 }
 
+template <bool kDump, int kSlots>
+__global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
+    tile_body<kDump, kSlots, false>(a, blockIdx.x);
+}
+
+// Persistent batch: every workgroup takes tile list entries from a shared
+// counter until the list is exhausted; the workgroup finishing the last tile
+// of a system stamps that system's completion time (100 MHz wall clock).
+template <int kSlots>
+__global__ __launch_bounds__(256) void pair_persist_kernel(PairKernelArgs a) {
+    __shared__ int s_t;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && a.t0) *a.t0 = wall_clock64();
+    for (;;) {
+        __syncthreads();   // the previous tile is done with LDS and s_t
+        if (threadIdx.x == 0) s_t = atomicAdd(a.work, 1);
+        __syncthreads();
+        const int t = s_t;
+        if (t >= a.n_work) break;
+        tile_body<false, kSlots, true>(a, t);
+        if (threadIdx.x == 0 && a.sys_count) {
+            const int sys = a.tile_sys[t];
+            __threadfence();
+            if (atomicAdd(&a.sys_count[sys], 1) == a.sys_tiles[sys] - 1)
+                a.sys_time[sys] = wall_clock64();
+        }
+    }
+}
+
+// Copies the ss frames of every step of every DOF to rows where each step
+// starts on a 4-byte boundary: out[d][s * ss4 + f] = codes[d][s * ss + f]
+// (padding bytes 0, never counted).
+__global__ __launch_bounds__(256) void repack_kernel(const uint8_t* __restrict__ codes,
+                                                     uint64_t stride, uint64_t ss, int S,
+                                                     uint64_t ss4, uint8_t* __restrict__ out) {
+    const uint64_t q = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const uint64_t d = blockIdx.y;
+    const uint64_t row = (uint64_t)S * ss4;
+    if (q >= row) return;
+    const uint64_t s = q / ss4, f = q - s * ss4;
+    out[d * row + q] = f < ss ? codes[d * stride + s * ss + f] : 0;
+}
+
 }  // namespace
+
+hipError_t launch_repack(const uint8_t* codes, uint64_t stride, uint32_t D, uint64_t ss, int S,
+                         uint64_t ss4, uint8_t* out, hipStream_t st) {
+    const uint64_t row = (uint64_t)S * ss4;
+    hipLaunchKernelGGL(repack_kernel, dim3((unsigned)((row + 255) / 256), D), dim3(256), 0, st,
+                       codes, stride, ss, S, ss4, out);
+    return hipGetLastError();
+}
 
 size_t pair_tile_lds_bytes(int R) { return lds_layout(R).total; }
 
@@ -714,6 +792,25 @@ static hipError_t launch_d(const PairKernelArgs& a, int n_tiles, size_t lds, hip
     if (need <= 2) return launch_t<kDump, 2>(a, n_tiles, lds, st);
     if (need <= 4) return launch_t<kDump, 4>(a, n_tiles, lds, st);
     return launch_t<kDump, 8>(a, n_tiles, lds, st);
+}
+
+template <int kSlots>
+static hipError_t launch_p(const PairKernelArgs& a, int grid, size_t lds, hipStream_t st) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&pair_persist_kernel<kSlots>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((pair_persist_kernel<kSlots>), dim3(grid), dim3(256), lds, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_pair_persistent(const PairKernelArgs& a, int grid, hipStream_t st) {
+    if (a.n_work <= 0 || grid <= 0) return hipSuccess;
+    const size_t lds = pair_tile_lds_bytes(a.R);
+    const int need = (2 * a.R * kGroups + 255) / 256;
+    if (need <= 1) return launch_p<1>(a, grid, lds, st);
+    if (need <= 2) return launch_p<2>(a, grid, lds, st);
+    if (need <= 4) return launch_p<4>(a, grid, lds, st);
+    return launch_p<8>(a, grid, lds, st);
 }
 
 hipError_t launch_pair_tiles(const PairKernelArgs& a, int n_tiles, bool dump,

@@ -27,6 +27,7 @@ _SIGS = {
     "centre_sync": ([P], I32),
     "centre_stream": ([P], P),
     "centre_last_pair_kernel_ms": ([P], C.c_float),
+    "centre_batch_pair_entropy": ([P, P, I32, I32, I32, P, I32, I32, I32, I32, P, P, P], I32),
     "centre_screen_n_params": ([], I32),
     "centre_screen_tiles": ([P, P, U32, I32, C.c_float, I32, P, P], I32),
     "centre_pair_plan_from_tiles": ([P, U32, I32, P, U64, P], I32),
@@ -85,6 +86,13 @@ def lib():
             f.restype = res
         _lib = L
     return _lib
+
+
+class BatchSystem(C.Structure):
+    """centre_batch_system (device pointers as integers)."""
+    _fields_ = [("d_codes", C.c_void_p), ("code_stride", C.c_uint64), ("n_dof", C.c_uint32),
+                ("n_frame", C.c_uint64), ("d_types", C.c_void_p), ("d_extrema", C.c_void_p),
+                ("d_s1", C.c_void_p), ("d_s2", C.c_void_p), ("d_mi", C.c_void_p)]
 
 
 def _p(a: Optional[np.ndarray]):
@@ -164,6 +172,19 @@ class Context:
         d = DevArray(self, x.shape, x.dtype)
         d.upload(x)
         return d
+
+    def batch_pair_entropy(self, systems, nbins, nsteps, ests=(0,), jacobian=True, kde=False,
+                           order=0):
+        """systems: list of BatchSystem -> (per-system ms, total ms, order used)."""
+        arr = (BatchSystem * len(systems))(*systems)
+        ests = _c(np.atleast_1d(ests), np.int32)
+        sys_ms = np.zeros(len(systems), np.float64)
+        tot = C.c_double(0.0)
+        used = C.c_int(0)
+        self.check(self._L.centre_batch_pair_entropy(
+            self.h, C.cast(arr, C.c_void_p), len(systems), nbins, nsteps, _p(ests), len(ests),
+            int(jacobian), int(kde), order, _p(sys_ms), C.byref(tot), C.byref(used)))
+        return sys_ms, tot.value, used.value
 
     # -- learned pair screen (device pointers) ---------------------------------
     def screen_features(self, d_codes, stride, n_dof, n_frame, d_types, d_ext, nbins,

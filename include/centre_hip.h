@@ -185,6 +185,35 @@ int centre_memcpy_d2d(centre_ctx *ctx, void *d_dst, const void *d_src,
 /* hipDeviceSynchronize on the context's device. */
 int centre_device_sync(centre_ctx *ctx);
 
+/* ---- batched pair stage ------------------------------------------------------
+ * No reference counterpart (centre_OMP runs one system per process): many
+ * independent systems, each with its own DOF count and frames, share one
+ * persistent launch.  Every system gets all its i < j pairs (triangle order)
+ * and its marginals, exactly as centre_dev_pair_entropy with an all-pairs
+ * plan would give them; ML and MM only.
+ * order: tile order of the shared work queue -- 1 longest tiles first,
+ * 2 system by system, 3 systems by total work, largest first; 0 = pick by
+ * timing the three on a calibration share of the batch (5% of each system's
+ * tiles, steps cut to 4,096 frames).  sys_ms[k] (optional): time from the
+ * launch to the completion of system k; total_ms: the whole launch;
+ * order_used: the order run. */
+typedef struct centre_batch_system {
+    const uint8_t *d_codes;    /* [n_dof][code_stride] */
+    uint64_t code_stride;
+    uint32_t n_dof;
+    uint64_t n_frame;
+    const uint8_t *d_types;    /* [n_dof] 1 bond, 2 angle, 3 torsion */
+    const double *d_extrema;   /* [n_dof][4] */
+    double *d_s1;              /* [n_dof][nest][nsteps] or NULL */
+    double *d_s2;              /* [P][nest][nsteps] */
+    double *d_mi;              /* [P][nsteps] or NULL */
+} centre_batch_system;
+int centre_batch_pair_entropy(centre_ctx *ctx, const centre_batch_system *sys,
+                              int n_sys, int nbins, int nsteps,
+                              const int *ests, int nest, int jacobian,
+                              int kde, int order, double *sys_ms,
+                              double *total_ms, int *order_used);
+
 /* ---- learned pair screen ------------------------------------------------------
  * No reference counterpart (the reference restricts pairs only by a user
  * neighbour list, Centre.cpp:459-593).  An MLP 8 -> 64 -> 64 -> 1 (FP32, ReLU)

@@ -1,0 +1,71 @@
+"""Batched pair stage (one persistent launch over many systems) against the
+single-system path: the same S1, S2 and MI for every system, whatever the
+tile order of the shared work queue, including the autotuned one."""
+import numpy as np
+import pytest
+
+from centre_amd import hip
+
+pytestmark = pytest.mark.gpu
+
+
+def make_system(ctx, seed, nbd, nan, ndh, F, nb):
+    bat = hip.synth_bat(seed, nbd, nan, ndh, F)
+    D = nbd + nan + ndh
+    d_bat = ctx.to_device(bat)
+    d_codes = ctx.alloc((D, F), np.uint8)
+    d_ext = ctx.alloc((D, 4), np.float64)
+    types = np.array([1] * nbd + [2] * nan + [3] * ndh, np.uint8)
+    col = 0
+    for t, n in ((1, nbd), (2, nan), (3, ndh)):
+        if n:
+            ctx.dev_discretize_hist(d_bat.ptr + 8 * col, D, n, F, t, [nb], nb, True,
+                                    d_codes.ptr + col * F, d_ext.ptr + 32 * col)
+        col += n
+    ctx.sync()
+    d_bat.free()
+    return dict(D=D, F=F, codes=d_codes, ext=d_ext, types=ctx.to_device(types))
+
+
+@pytest.mark.parametrize("nb,ests", [(50, (0,)), (30, (1, 0))])
+def test_batch_matches_single_system_path(nb, ests):
+    ctx = hip.Context(0)
+    S = 3
+    rng = np.random.default_rng(nb)
+    sizes = [(int(rng.integers(3, 14)), int(rng.integers(3, 14)), int(rng.integers(3, 14)),
+              int(rng.integers(1000, 3000)) * 4 + 2) for _ in range(6)]
+    systems = [make_system(ctx, 0xCE27E + 200 + k, *sz, nb) for k, sz in enumerate(sizes)]
+    nest = len(ests)
+    want = []
+    for s in systems:
+        D, F = s["D"], s["F"]
+        P = D * (D - 1) // 2
+        plan = hip.PairPlan(ctx, D, nb)
+        d1, d2, dm = (ctx.alloc((D, nest, S), np.float64), ctx.alloc((P, nest, S), np.float64),
+                      ctx.alloc((P, S), np.float64))
+        ctx.dev_pair_entropy(plan, s["codes"].ptr, F, F, s["types"].ptr, s["ext"].ptr, S, list(ests),
+                             True, False, d1.ptr, d2.ptr, dm.ptr)
+        plan.close()
+        want.append((d1.numpy(), d2.numpy(), dm.numpy()))
+    for order in (1, 2, 3, 0):
+        print(f"batch nb={nb} order={order} sizes={sizes}", flush=True)
+        outs, descs = [], []
+        for s in systems:
+            D, F = s["D"], s["F"]
+            P = D * (D - 1) // 2
+            o = (ctx.alloc((D, nest, S), np.float64), ctx.alloc((P, nest, S), np.float64),
+                 ctx.alloc((P, S), np.float64))
+            outs.append(o)
+            descs.append(hip.BatchSystem(s["codes"].ptr, F, D, F, s["types"].ptr, s["ext"].ptr,
+                                         o[0].ptr, o[1].ptr, o[2].ptr))
+        sys_ms, tot, used = ctx.batch_pair_entropy(descs, nb, S, ests, order=order)
+        assert used in (1, 2, 3) and (order == 0 or used == order)
+        assert tot > 0 and np.all(sys_ms > 0) and np.all(sys_ms <= tot * 1.05 + 0.05)
+        for (a1, a2, am), o in zip(want, outs):
+            # S1 comes from the same sequential per-DOF sums: identical.  Pair
+            # sums are FP64 LDS atomics whose order varies from run to run,
+            # so S2 and MI agree to a few ulp.
+            np.testing.assert_array_equal(o[0].numpy(), a1)
+            np.testing.assert_allclose(o[1].numpy(), a2, rtol=1e-13, atol=0)
+            np.testing.assert_allclose(o[2].numpy(), am, rtol=1e-12, atol=1e-15)
+    ctx.close()
