@@ -329,6 +329,9 @@ int centre_pair_plan_create(centre_ctx* ctx, uint32_t n_dof, int nbins,
                            hipMemcpyHostToDevice)) != hipSuccess ||
             (e = hipMalloc(&plan->d_rec2canon, sizeof(uint64_t) * (n_pair ? n_pair : 1))) != hipSuccess ||
             (n_pair && (e = hipMemcpy(plan->d_rec2canon, r2c.data(), sizeof(uint64_t) * n_pair,
+                                      hipMemcpyHostToDevice)) != hipSuccess) ||
+            (e = hipMalloc(&plan->d_rec_ij, sizeof(uint32_t) * 2 * (n_pair ? n_pair : 1))) != hipSuccess ||
+            (n_pair && (e = hipMemcpy(plan->d_rec_ij, pairs, sizeof(uint32_t) * 2 * n_pair,
                                       hipMemcpyHostToDevice)) != hipSuccess)) {
             centre_pair_plan_destroy(plan);
             return fail(ctx, (int)e, std::string("plan upload: ") + hipGetErrorString(e));
@@ -351,6 +354,7 @@ void centre_pair_plan_destroy(centre_pair_plan* plan) {
     if (plan->d_tiles) (void)hipFree(plan->d_tiles);
     if (plan->d_pairmap) (void)hipFree(plan->d_pairmap);
     if (plan->d_rec2canon) (void)hipFree(plan->d_rec2canon);
+    if (plan->d_rec_ij) (void)hipFree(plan->d_rec_ij);
     delete plan;
 }
 
@@ -367,26 +371,15 @@ int centre_pair_plan_tiles(const centre_pair_plan* plan) {
 namespace {
 
 // Canonical (i<j) DOF list of a plan, for the count-epilogue kernel.
-std::vector<uint32_t> canon_pairs(const centre_pair_plan* plan, const int* pm_host) {
+std::vector<uint32_t> canon_pairs(const centre_pair_plan* plan) {
     std::vector<uint32_t> ij(2 * plan->n_canon);
-    if (plan->all_pairs) {
-        uint64_t p = 0;
-        for (uint32_t i = 0; i < plan->n_dof; ++i)
-            for (uint32_t j = i + 1; j < plan->n_dof; ++j) {
-                ij[2 * p] = i;
-                ij[2 * p + 1] = j;
-                ++p;
-            }
-    } else {
-        for (uint32_t i = 0; i < plan->n_dof; ++i)
-            for (uint32_t j = i + 1; j < plan->n_dof; ++j) {
-                const int c = pm_host[(size_t)i * plan->n_dof + j];
-                if (c >= 0) {
-                    ij[2 * (size_t)c] = i;
-                    ij[2 * (size_t)c + 1] = j;
-                }
-            }
-    }
+    uint64_t p = 0;
+    for (uint32_t i = 0; i < plan->n_dof; ++i)
+        for (uint32_t j = i + 1; j < plan->n_dof; ++j) {
+            ij[2 * p] = i;
+            ij[2 * p + 1] = j;
+            ++p;
+        }
     return ij;
 }
 
@@ -723,25 +716,34 @@ extern "C" int centre_dev_pair_entropy(centre_ctx* ctx, const centre_pair_plan* 
     HIPCHK(launch_pair_tiles(a, plan->n_tiles, dump, ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
 
+    if (!all && d_counts)
+        HIPCHK(launch_gather_counts(cnt, plan->d_rec2canon, plan->d_rec_ij, plan->n_rec, nsteps,
+                                    nb, d_counts, ctx->stream));
+
     if (!fused_only) {
-        // every estimator from the materialised counts
-        std::vector<int> pm_host;
-        if (!all) {
-            pm_host.resize((size_t)D * D);
-            HIPCHK(hipMemcpy(pm_host.data(), plan->d_pairmap, sizeof(int) * pm_host.size(),
-                             hipMemcpyDeviceToHost));
-        }
-        std::vector<uint32_t> ij = canon_pairs(plan, pm_host.data());
+        // every estimator from the materialised counts, per output record: a
+        // record keeps its own row/column order (the Chao-Shen Jacobian
+        // depends on it, Estimators.cpp:558,562)
+        const uint64_t NR = all ? P : plan->n_rec;
+        std::vector<uint32_t> ij;
         DevBuf dij, de;
-        HIPCHK(dij.alloc(sizeof(uint32_t) * ij.size()));
+        if (all) {
+            ij = canon_pairs(plan);
+            HIPCHK(dij.alloc(sizeof(uint32_t) * ij.size()));
+            HIPCHK(hipMemcpyAsync(dij.p, ij.data(), sizeof(uint32_t) * ij.size(),
+                                  hipMemcpyHostToDevice, ctx->stream));
+        } else if (d_mi && ests[0] > 1) {
+            ij.resize(2 * NR);
+            HIPCHK(hipMemcpyAsync(ij.data(), plan->d_rec_ij, sizeof(uint32_t) * ij.size(),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+        }
         HIPCHK(de.alloc(sizeof(int) * nest));
-        HIPCHK(hipMemcpyAsync(dij.p, ij.data(), sizeof(uint32_t) * ij.size(),
-                              hipMemcpyHostToDevice, ctx->stream));
         HIPCHK(hipMemcpyAsync(de.p, ests, sizeof(int) * nest, hipMemcpyHostToDevice, ctx->stream));
         CountEntropyArgs c{};
         c.counts = cnt;
-        c.n_pair = P;
-        c.canon_ij = dij.as<uint32_t>();
+        c.n_pair = NR;
+        c.canon_ij = all ? dij.as<uint32_t>() : plan->d_rec_ij;
+        c.rec2canon = all ? nullptr : plan->d_rec2canon;
         c.nb = nb;
         c.nsteps = nsteps;
         c.ss = ss;
@@ -751,32 +753,33 @@ extern "C" int centre_dev_pair_entropy(centre_ctx* ctx, const centre_pair_plan* 
         c.kde = kde;
         c.ests = de.as<int>();
         c.nest = nest;
-        c.s2 = s2c;
+        c.s2 = all ? s2c : d_s2;
         HIPCHK(launch_count_entropy(c, ctx->stream));
         if (d_mi && ests[0] > 1) {
             // MI = S1_i + S1_j - S2 for estimator 0 on the host side of the ABI
-            std::vector<double> hs1((size_t)D * nest * nsteps), hs2((size_t)P * nest * nsteps);
+            std::vector<double> hs1((size_t)D * nest * nsteps), hs2((size_t)NR * nest * nsteps);
             HIPCHK(hipMemcpyAsync(hs1.data(), s1.p, sizeof(double) * hs1.size(),
                                   hipMemcpyDeviceToHost, ctx->stream));
-            HIPCHK(hipMemcpyAsync(hs2.data(), s2c, sizeof(double) * hs2.size(),
+            HIPCHK(hipMemcpyAsync(hs2.data(), c.s2, sizeof(double) * hs2.size(),
                                   hipMemcpyDeviceToHost, ctx->stream));
             HIPCHK(hipStreamSynchronize(ctx->stream));
-            std::vector<double> hmi((size_t)P * nsteps);
-            for (uint64_t p = 0; p < P; ++p)
+            std::vector<double> hmi((size_t)NR * nsteps);
+            for (uint64_t p = 0; p < NR; ++p)
                 for (int s = 0; s < nsteps; ++s)
                     hmi[p * nsteps + s] = hs1[((size_t)ij[2 * p] * nest) * nsteps + s] +
                                           hs1[((size_t)ij[2 * p + 1] * nest) * nsteps + s] -
                                           hs2[(p * nest) * nsteps + s];
-            HIPCHK(hipMemcpyAsync(mic, hmi.data(), sizeof(double) * hmi.size(),
+            HIPCHK(hipMemcpyAsync(all ? mic : d_mi, hmi.data(), sizeof(double) * hmi.size(),
                                   hipMemcpyHostToDevice, ctx->stream));
         }
         HIPCHK(hipStreamSynchronize(ctx->stream));
     }
 
     if (!all) {
-        HIPCHK(launch_gather_rec(s2c, plan->d_rec2canon, plan->n_rec, nest * nsteps, d_s2,
-                                 ctx->stream));
-        if (d_mi)
+        if (fused_only)
+            HIPCHK(launch_gather_rec(s2c, plan->d_rec2canon, plan->n_rec, nest * nsteps, d_s2,
+                                     ctx->stream));
+        if (d_mi && a.mi)
             HIPCHK(launch_gather_rec(mic, plan->d_rec2canon, plan->n_rec, nsteps, d_mi,
                                      ctx->stream));
     }

@@ -27,6 +27,7 @@ struct centre_pair_plan {
     uint64_t n_canon = 0;           // canonical (i<j) pairs computed
     uint64_t n_rec = 0;             // records requested by the caller
     uint64_t* d_rec2canon = nullptr;  // [n_rec] (explicit lists only)
+    uint32_t* d_rec_ij = nullptr;     // [n_rec][2] as given (explicit lists only)
     bool all_pairs = true;
 };
 
@@ -124,9 +125,11 @@ hipError_t launch_dof1d(const Dof1dArgs& a, hipStream_t st);
 
 // estimator epilogue on materialised counts (CS / JS / any estimator)
 struct CountEntropyArgs {
-    const uint32_t* counts;   // [P][S][nb*nb]
-    uint64_t n_pair;
-    const uint32_t* canon_ij; // [P][2] DOF indices
+    const uint32_t* counts;   // [canonical pair][S][nb*nb], row DOF < column DOF
+    uint64_t n_pair;          // output rows
+    const uint32_t* canon_ij; // [n_pair][2] row and column DOF of each output row
+    const uint64_t* rec2canon;// [n_pair] counts index of each row, NULL = identity;
+                              // a row with i > j reads its counts transposed
     int nb, nsteps;
     uint64_t ss;
     const uint8_t* types;
@@ -206,6 +209,11 @@ hipError_t launch_segment_sum(const double* x, const uint64_t* d_beg, int n_seg,
 hipError_t launch_gather_rec(const double* src, const uint64_t* rec2canon,
                              uint64_t n_rec, int width, double* dst,
                              hipStream_t st);
+// dst[r][s][x*nb + y] = counts of record r = (i, j) at step s, row bin x of i,
+// column bin y of j, from canonical counts (row DOF < column DOF).
+hipError_t launch_gather_counts(const uint32_t* src, const uint64_t* rec2canon,
+                                const uint32_t* rec_ij, uint64_t n_rec, int nsteps, int nb,
+                                uint32_t* dst, hipStream_t st);
 
 // ---- learned pair screen (screen.hip) ----------------------------------------
 constexpr int kScreenFeat = 8;     // features per pair

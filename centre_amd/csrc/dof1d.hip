@@ -126,6 +126,11 @@ __global__ __launch_bounds__(64) void count_entropy_kernel(CountEntropyArgs a) {
     const uint64_t p = t / ((uint64_t)nest * S);
     const int est = a.ests[e];
     const uint32_t i = a.canon_ij[2 * p], j = a.canon_ij[2 * p + 1];
+    const uint64_t pc = a.rec2canon ? a.rec2canon[p] : p;
+    // counts are stored with the lower DOF index as rows; the reference's
+    // row is the record's first DOF, which matters to the Chao-Shen Jacobian
+    const bool flip = i > j;
+    const unsigned rs = flip ? 1u : (unsigned)nb, cs = flip ? (unsigned)nb : 1u;
     const int t1 = a.types[i], t2 = a.types[j];
     const double mn1 = a.ext[4 * i], mx1 = a.ext[4 * i + 1];
     const double mn2 = a.ext[4 * j], mx2 = a.ext[4 * j + 1];
@@ -135,7 +140,7 @@ __global__ __launch_bounds__(64) void count_entropy_kernel(CountEntropyArgs a) {
     const bool hj1 = (t1 == 1 || t1 == 2), hj2 = (t2 == 1 || t2 == 2);
     const double area = w1 * w2;
     const double n = (s + 1.0) * a.ss;
-    const uint32_t* f = a.counts + ((size_t)p * S + s) * nb * nb;
+    const uint32_t* f = a.counts + ((size_t)pc * S + s) * nb * nb;
     const unsigned nb2 = (unsigned)nb * nb;
     unsigned blank = 0, single = 0;
     double lam = 0.0, tk = 1.0 / (double)nb;
@@ -157,8 +162,9 @@ __global__ __launch_bounds__(64) void count_entropy_kernel(CountEntropyArgs a) {
         if (hj1) jf1 = jac(t1, mn1 + (r * w1) + hw1);
         double row = 0.0;
         for (int c = 0; c < nb; ++c) {
-            double q = f[r * nb + c] / n;
-            if (est == kCS) q = (1 - (single / n)) * (f[r * nb + c] / n);
+            const uint32_t fc = f[r * rs + c * cs];
+            double q = fc / n;
+            if (est == kCS) q = (1 - (single / n)) * (fc / n);
             else if (est == kJS) q = (lam * tk) + ((1 - lam) * q);
             double pd = q / area;
             if (pd > 0.0) {
@@ -194,7 +200,33 @@ __global__ void gather_rec_kernel(const double* src, const uint64_t* rec2canon,
     dst[t] = src[rec2canon[r] * width + k];
 }
 
+__global__ void gather_counts_kernel(const uint32_t* src, const uint64_t* rec2canon,
+                                     const uint32_t* rec_ij, uint64_t n_rec, int S, int nb,
+                                     uint32_t* dst) {
+    const uint64_t nb2 = (uint64_t)nb * nb;
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_rec * S * nb2) return;
+    const uint64_t q = t % nb2;
+    const uint64_t rs = t / nb2;           // r * S + s
+    const uint64_t r = rs / S;
+    const uint64_t s = rs - r * S;
+    const bool flip = rec_ij[2 * r] > rec_ij[2 * r + 1];
+    const uint64_t x = q / nb, y = q - x * nb;
+    const uint64_t qs = flip ? y * nb + x : q;
+    dst[t] = src[(rec2canon[r] * S + s) * nb2 + qs];
+}
+
 }  // namespace
+
+hipError_t launch_gather_counts(const uint32_t* src, const uint64_t* rec2canon,
+                                const uint32_t* rec_ij, uint64_t n_rec, int nsteps, int nb,
+                                uint32_t* dst, hipStream_t st) {
+    const uint64_t n = n_rec * nsteps * (uint64_t)nb * nb;
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(gather_counts_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                       src, rec2canon, rec_ij, n_rec, nsteps, nb, dst);
+    return hipGetLastError();
+}
 
 hipError_t launch_dof1d(const Dof1dArgs& a, hipStream_t st) {
     if (a.n_dof == 0) return hipSuccess;

@@ -294,6 +294,14 @@ class Context:
             self.h, d_bat, ld, n_dof, n_frame, dof_type, _p(sch), len(sch),
             nbins_ref, int(optimize), d_codes, d_ext))
 
+    def dev_discretize_vmkde(self, d_bat: int, ld: int, n_dof: int, n_frame: int,
+                             nmaxconf: int, kappa: float, d_codes: int, d_minima: int,
+                             d_nfound: int, sdo_step=5, sdo_conv=1e-4, sdo_iter=1000,
+                             grid_intervals=360, d_grid: int = 0):
+        self.check(self._L.centre_dev_discretize_vmkde(
+            self.h, d_bat, ld, n_dof, n_frame, nmaxconf, kappa, sdo_step, sdo_conv,
+            sdo_iter, grid_intervals, d_codes, d_minima, d_nfound, d_grid or None))
+
     def dev_dof_entropy(self, d_codes, stride, n_dof, n_frame, d_types, d_ext,
                         nbins, nsteps, ests, jacobian, kde, d_s1, d_counts=None):
         ests = _c(np.atleast_1d(ests), np.int32)

@@ -50,7 +50,9 @@ void ck(centre_ctx* ctx, int rc, const char* what) {
 struct Dev {
     centre_ctx* ctx = nullptr;
     void* p = nullptr;
-    Dev(centre_ctx* c, size_t bytes) : ctx(c) { ck(ctx, centre_dev_alloc(ctx, bytes, &p), "alloc"); }
+    Dev(centre_ctx* c, size_t bytes) : ctx(c) {
+        if (bytes) ck(ctx, centre_dev_alloc(ctx, bytes, &p), "alloc");
+    }
     ~Dev() { if (p) centre_dev_free(ctx, p); }
     Dev(const Dev&) = delete;
     template <class T> T* as() const { return static_cast<T*>(p); }
@@ -191,6 +193,70 @@ void write_entcontri(const std::string& path, const std::string& content, int or
     w.add_attr("binSchemes", nc::Attr::of<uint8_t>(nc::UBYTE, schemes));
     w.add_var("id", nc::UINT, {dr, di}, ids.data());
     w.add_var("contri_S", nc::DOUBLE, {dr, de, db, dn}, contri.data());
+    w.write(path);
+}
+
+// Bin mids as binData1D / binData2D accumulate them (Estimators.cpp:67-77,
+// 140-160): the offset grows by one bin width per bin, which is not bit for
+// bit min + i*w.
+void append_mids(std::vector<double>& out, double mn, double mx, int nb) {
+    const double w = (mx - mn) / nb, hw = w / 2.0;
+    double o = mn;
+    for (int i = 0; i < nb; ++i) {
+        out.push_back(o + hw);
+        o += w;
+    }
+}
+
+// Count file of one kind (Netcdf_HistUtils.cpp:216-262 constructor clamps,
+// 808-1029 NC_create; records as writeRecords, 714-772).  ndiag shares the
+// attribute name "title" and overwrites it in place (Netcdf_HistUtils.h:26,32).
+struct HistFile {
+    std::string content;
+    int ndims = 1;
+    int storage = 0;                    // TensorType: FULL 0, UPPER 3
+    std::vector<uint64_t> dim_lens;
+    std::vector<uint32_t> ids;          // [rec][ndims]
+    std::vector<double> extrema;        // [rec][ndims][4]
+    std::vector<double> mids;           // [rec][ndims * sum(schemes)]
+    std::vector<uint64_t> freq;         // [rec][steps_eff][sum(schemes^ndims)]
+};
+
+void write_hist(const std::string& path, const HistFile& h, const std::vector<uint8_t>& schemes,
+                int first_step, int nsteps, int stride, int steps_eff) {
+    uint64_t bsum = 0, bpow = 0;
+    for (uint8_t b : schemes) {
+        bsum += b;
+        bpow += h.ndims == 1 ? (uint64_t)b : (uint64_t)b * b;
+    }
+    nc::Writer w(5);
+    const uint64_t nrec = h.ids.size() / h.ndims;
+    const int dr = w.add_dim("records", nrec, true);
+    const int dm = w.add_dim("dimsXbin_sum", bsum * h.ndims);
+    const int dp = w.add_dim("bin_pow_dims_sum", bpow);
+    const int dn = w.add_dim("nsteps_eff", steps_eff);
+    const int di = w.add_dim("ids", h.ndims);
+    const int de = w.add_dim("extremes", 4);
+    w.add_attr("title", nc::Attr::of<uint8_t>(nc::UBYTE, {(uint8_t)(h.storage != 0 ? 1 : 0)}));
+    w.add_attr("appication", nc::Attr::of_text("CENTRE"));
+    w.add_attr("program", nc::Attr::of_text("Real2Int"));
+    w.add_attr("programVersion", nc::Attr::of_text("1.0.0"));
+    w.add_attr("convention", nc::Attr::of_text("CENTRE_HIST"));
+    w.add_attr("conventionVersion", nc::Attr::of_text("1.0"));
+    w.add_attr("content", nc::Attr::of_text(h.content));
+    w.add_attr("order", nc::Attr::of<uint8_t>(nc::UBYTE, {(uint8_t)h.ndims}));
+    w.add_attr("firstStep", nc::Attr::of<uint8_t>(nc::UBYTE, {(uint8_t)first_step}));
+    w.add_attr("nsteps", nc::Attr::of<uint8_t>(nc::UBYTE, {(uint8_t)nsteps}));
+    w.add_attr("stepStride", nc::Attr::of<uint8_t>(nc::UBYTE, {(uint8_t)stride}));
+    w.add_attr("storagetype", nc::Attr::of<uint8_t>(nc::UBYTE, {(uint8_t)h.storage}));
+    w.add_attr("ndims", nc::Attr::of<uint8_t>(nc::UBYTE, {(uint8_t)h.ndims}));
+    w.add_attr("dimLengths", nc::Attr::of<uint64_t>(nc::UINT64, h.dim_lens));
+    w.add_attr("nbinSchemes", nc::Attr::of<uint8_t>(nc::UBYTE, {(uint8_t)schemes.size()}));
+    w.add_attr("binSchemes", nc::Attr::of<uint8_t>(nc::UBYTE, schemes));
+    w.add_var("id", nc::UINT, {dr, di}, h.ids.data());
+    w.add_var("extrema", nc::DOUBLE, {dr, di, de}, h.extrema.data());
+    w.add_var("mid", nc::DOUBLE, {dr, dm}, h.mids.data());
+    w.add_var("freq", nc::UINT64, {dr, dn, dp}, h.freq.data());
     w.write(path);
 }
 
@@ -408,6 +474,30 @@ void Host::run_entropy() {
     dt.put(types.data(), Dall);
     dx.put(ext.data(), ext.size() * 8);
 
+    // count files (EntropyCalculator.h:75-101; the step range clamps as the
+    // Netcdf_HistUtil constructor does, Netcdf_HistUtils.cpp:225-231)
+    const bool wfreq = kde ? in.vm_writefreq : in.hist_writefreq;
+    const uint32_t wset = kde ? in.vm_writeset : in.hist_writeset;
+    auto hist_on = [&](uint32_t set) { return wfreq && (wset & set) != 0; };
+    const int hfirst = std::min(std::max(in.hist_writestepstart, 0), S - 1);
+    const int hstride = std::min(std::max(in.hist_writestepstride, 1), S);
+    const int Se = (S - hfirst + hstride - 1) / hstride;
+    uint64_t bsum = 0, bsq = 0;
+    std::vector<uint64_t> off1(nsch), off2(nsch);
+    for (int sc = 0; sc < nsch; ++sc) {
+        off1[sc] = bsum;
+        off2[sc] = bsq;
+        bsum += schemes[sc];
+        bsq += (uint64_t)schemes[sc] * schemes[sc];
+    }
+    bool any1 = false;
+    std::vector<uint64_t> hist1[3];     // [rec][Se][bsum]
+    for (int ty = 0; ty < 3; ++ty)
+        if (hist_on(ONE_SET[ty]) && !td[ty].subset.empty()) {
+            any1 = true;
+            hist1[ty].assign(td[ty].subset.size() * Se * bsum, 0);
+        }
+
     // outputs gathered over schemes: S1 [dof][est][sch][S], S2 [rec][est][sch][S]
     std::vector<double> s1((size_t)Dall * nest * nsch * S);
     std::vector<std::vector<uint32_t>> ids2(6);
@@ -426,6 +516,9 @@ void Host::run_entropy() {
         }
     }
     std::vector<std::vector<double>> s2(6);
+    std::vector<std::vector<uint64_t>> hist2(6);   // [rec][Se][bsq]
+    for (int k = 0; k < 6; ++k)
+        if (hist_on(KINDS[k].set)) hist2[k].assign(pairs[k].size() / 2 * Se * bsq, 0);
     std::vector<double> kernel_ms;
     for (int sc = 0; sc < nsch; ++sc) {
         std::vector<uint8_t> all((size_t)Dall * Fe);
@@ -436,11 +529,25 @@ void Host::run_entropy() {
                 std::copy(v.begin(), v.end(), all.begin() + (size_t)(base[ty] + k) * Fe);
             }
         dcodes.put(all.data(), all.size());
+        const int nb = schemes[sc];
         Dev ds1(ctx, (size_t)Dall * nest * S * 8);
+        Dev dc1(ctx, any1 ? (size_t)Dall * S * nb * 8 : 0);
         ck(ctx, centre_dev_dof_entropy(ctx, dcodes.as<uint8_t>(), Fe, Dall, Fe, dt.as<uint8_t>(),
-                                       dx.as<double>(), schemes[sc], S, in.estimators.data(), nest,
-                                       in.jacobian, kde, ds1.as<double>(), nullptr),
+                                       dx.as<double>(), nb, S, in.estimators.data(), nest,
+                                       in.jacobian, kde, ds1.as<double>(),
+                                       any1 ? dc1.as<uint64_t>() : nullptr),
            "dof_entropy");
+        if (any1) {
+            std::vector<uint64_t> c1((size_t)Dall * S * nb);
+            dc1.get(c1.data(), c1.size() * 8);
+            for (int ty = 0; ty < 3; ++ty) {
+                if (hist1[ty].empty()) continue;
+                for (size_t r = 0; r < td[ty].subset.size(); ++r)
+                    for (int e = 0; e < Se; ++e)
+                        std::copy_n(&c1[((size_t)(base[ty] + r) * S + hfirst + e * hstride) * nb], nb,
+                                    &hist1[ty][(r * Se + e) * bsum + off1[sc]]);
+            }
+        }
         std::vector<double> h1((size_t)Dall * nest * S);
         ds1.get(h1.data(), h1.size() * 8);
         for (uint32_t d = 0; d < Dall; ++d)
@@ -450,23 +557,42 @@ void Host::run_entropy() {
         for (int k = 0; k < 6; ++k) {
             const uint64_t P = pairs[k].size() / 2;
             if (!P) continue;
-            centre_pair_plan* plan = nullptr;
-            ck(ctx, centre_pair_plan_create(ctx, Dall, schemes[sc], pairs[k].data(), P, &plan), "plan");
-            Dev ds2(ctx, P * nest * S * 8);
-            const int rc = centre_dev_pair_entropy(ctx, plan, dcodes.as<uint8_t>(), Fe, Fe,
-                                                   dt.as<uint8_t>(), dx.as<double>(), S,
-                                                   in.estimators.data(), nest, in.jacobian, kde,
-                                                   nullptr, ds2.as<double>(), nullptr, nullptr);
-            centre_pair_plan_destroy(plan);
-            ck(ctx, rc, "pair_entropy");
-            kernel_ms.push_back(centre_last_pair_kernel_ms(ctx));
-            std::vector<double> h2(P * nest * S);
-            ds2.get(h2.data(), h2.size() * 8);
             s2[k].resize(P * nest * nsch * S);
-            for (uint64_t p = 0; p < P; ++p)
-                for (int e = 0; e < nest; ++e)
-                    for (int s = 0; s < S; ++s)
-                        s2[k][((p * nest + e) * nsch + sc) * S + s] = h2[(p * nest + e) * S + s];
+            // with a count file, records go in chunks of at most 512 MiB of counts
+            const bool hk = !hist2[k].empty();
+            const uint64_t nb2 = (uint64_t)nb * nb;
+            const uint64_t chunk = hk ? std::max<uint64_t>(1, (512ull << 20) / (S * nb2 * 4)) : P;
+            for (uint64_t r0 = 0; r0 < P; r0 += chunk) {
+                const uint64_t n = std::min(chunk, P - r0);
+                centre_pair_plan* plan = nullptr;
+                ck(ctx, centre_pair_plan_create(ctx, Dall, nb, pairs[k].data() + 2 * r0, n, &plan),
+                   "plan");
+                Dev ds2(ctx, n * nest * S * 8);
+                Dev dcn(ctx, hk ? n * S * nb2 * 4 : 0);
+                const int rc = centre_dev_pair_entropy(ctx, plan, dcodes.as<uint8_t>(), Fe, Fe,
+                                                       dt.as<uint8_t>(), dx.as<double>(), S,
+                                                       in.estimators.data(), nest, in.jacobian, kde,
+                                                       nullptr, ds2.as<double>(), nullptr,
+                                                       hk ? dcn.as<uint32_t>() : nullptr);
+                centre_pair_plan_destroy(plan);
+                ck(ctx, rc, "pair_entropy");
+                kernel_ms.push_back(centre_last_pair_kernel_ms(ctx));
+                std::vector<double> h2(n * nest * S);
+                ds2.get(h2.data(), h2.size() * 8);
+                for (uint64_t p = 0; p < n; ++p)
+                    for (int e = 0; e < nest; ++e)
+                        for (int s = 0; s < S; ++s)
+                            s2[k][(((r0 + p) * nest + e) * nsch + sc) * S + s] =
+                                h2[(p * nest + e) * S + s];
+                if (hk) {
+                    std::vector<uint32_t> c2(n * S * nb2);
+                    dcn.get(c2.data(), c2.size() * 4);
+                    for (uint64_t p = 0; p < n; ++p)
+                        for (int e = 0; e < Se; ++e)
+                            std::copy_n(&c2[(p * S + hfirst + e * hstride) * nb2], nb2,
+                                        &hist2[k][((r0 + p) * Se + e) * bsq + off2[sc]]);
+                }
+            }
         }
     }
     // 1-D files
@@ -484,6 +610,46 @@ void Host::run_entropy() {
         write_entcontri(out(std::string("entcontri_") + KINDS[k].tag + "-2d.nc"), KINDS[k].content, 2,
                         same, {td[KINDS[k].a].subset.size(), td[KINDS[k].b].subset.size()}, schemes,
                         S, nest, ids2[k], s2[k]);
+    }
+    // count files (EntropyCalculator.cpp:61-92, 173-214, 362-372, 575-617,
+    // 786-795): 1-D FULL, same-type 2-D FULL, cross-type 2-D UPPER
+    for (int ty = 0; ty < 3; ++ty) {
+        if (hist1[ty].empty()) continue;
+        HistFile h;
+        h.content = ONE_CONTENT[ty];
+        h.ndims = 1;
+        h.dim_lens = {td[ty].subset.size()};
+        h.ids = td[ty].subset;
+        for (size_t r = 0; r < td[ty].subset.size(); ++r) {
+            const double* e = &ext[(size_t)(base[ty] + r) * 4];
+            h.extrema.insert(h.extrema.end(), e, e + 4);
+            for (int sc = 0; sc < nsch; ++sc) append_mids(h.mids, e[0], e[1], schemes[sc]);
+        }
+        h.freq = std::move(hist1[ty]);
+        write_hist(out(std::string("hist_") + ONE_TAG[ty] + "-1d.nc"), h, schemes, hfirst, S,
+                   hstride, Se);
+    }
+    for (int k = 0; k < 6; ++k) {
+        if (!hist_on(KINDS[k].set)) continue;
+        HistFile h;
+        h.content = KINDS[k].content;
+        h.ndims = 2;
+        h.storage = KINDS[k].a == KINDS[k].b ? 0 : 3;
+        h.dim_lens = {td[KINDS[k].a].subset.size(), td[KINDS[k].b].subset.size()};
+        h.ids = ids2[k];
+        for (size_t p = 0; p < pairs[k].size() / 2; ++p) {
+            const double* ea = &ext[(size_t)pairs[k][2 * p] * 4];
+            const double* eb = &ext[(size_t)pairs[k][2 * p + 1] * 4];
+            h.extrema.insert(h.extrema.end(), ea, ea + 4);
+            h.extrema.insert(h.extrema.end(), eb, eb + 4);
+            for (int sc = 0; sc < nsch; ++sc) {
+                append_mids(h.mids, ea[0], ea[1], schemes[sc]);
+                append_mids(h.mids, eb[0], eb[1], schemes[sc]);
+            }
+        }
+        h.freq = std::move(hist2[k]);
+        write_hist(out(std::string("hist_") + KINDS[k].tag + "-2d.nc"), h, schemes, hfirst, S,
+                   hstride, Se);
     }
     times["entropy"] = t.s();
     double kms = 0;

@@ -130,9 +130,10 @@ int centre_pair_plan_tiles(const centre_pair_plan *plan);
  * contractions of one-hot codes; ML and MM are fused into the count pass,
  * CS and JS go through materialised counts.  The marginals of the same
  * frames are computed inside the call; d_s1 (optional) receives them as
- * [n_dof][nest][nsteps].  d_counts (optional, NULL = never materialised;
- * all-pair plans only): [n_pair][nsteps][nbins*nbins] uint32 prefix counts
- * (hist_*-2d). */
+ * [n_dof][nest][nsteps].  d_counts (optional, NULL = never materialised):
+ * [record][nsteps][nbins*nbins] uint32 prefix counts, cell x*nbins + y for
+ * bin x of the record's first DOF and bin y of its second (the binData2D
+ * layout behind hist_*-2d, Estimators.cpp:130-205). */
 int centre_dev_pair_entropy(centre_ctx *ctx, const centre_pair_plan *plan,
                             const uint8_t *d_codes, uint64_t code_stride,
                             uint64_t n_frame, const uint8_t *d_types,

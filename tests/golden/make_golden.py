@@ -12,7 +12,9 @@ and outputs for ``test_examples/example-{1,2}``:
 * ``hist_tor-1d.nc`` (CDF-5): uint64 counts, extrema and bin mids;
 * the convergence CSVs, the MIST tree CSVs and ``vmkde_torsions_minimas.out``;
 * one stored NetCDF-4 file copied as is (``ex1h_entcontri_ang-1d.nc``, 18 KB),
-  the format fixture of the host's NetCDF-4 reader.
+  the format fixture of the host's NetCDF-4 reader;
+* the header of example-2's ``hist_tor-1d.nc`` as JSON
+  (``ex2h_hist_tor-1d_header.json``), the format fixture of the count files.
 
 Usage:  python tests/golden/make_golden.py [/root/reference]
 """
@@ -134,5 +136,28 @@ def main(ref: str) -> None:
           f"{len(out)} arrays)")
 
 
+def write_hist_header(ref: str) -> None:
+    """Header of example-2's stored hist_tor-1d.nc (dimension order and
+    sizes, attribute order, types and values, variable dimensions and types),
+    the format fixture of the host's count-file writer."""
+    import json
+    ds = open_dataset(os.path.join(ref, "test_examples", "example-2", "outfiles-h-ref",
+                                   "hist_tor-1d.nc"))
+    attrs = []
+    for k, v in ds.attrs.items():
+        if isinstance(v, str):
+            attrs.append([k, "text", v])
+        else:
+            a = np.asarray(v)
+            attrs.append([k, a.dtype.name, a.tolist()])
+    hdr = {"version": ds.version, "unlimited": ds.unlimited,
+           "dims": [[k, n] for k, n in ds.dims.items()], "attrs": attrs,
+           "vars": [[v.name, list(v.dims), v.nc_type] for v in ds.variables.values()]}
+    with open(os.path.join(HERE, "ex2h_hist_tor-1d_header.json"), "w") as f:
+        json.dump(hdr, f, indent=1)
+        f.write("\n")
+
+
 if __name__ == "__main__":
     main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
+    write_hist_header(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")

@@ -170,6 +170,41 @@ def test_explicit_pairs_order_and_duplicates(ctx, cfg1):
     assert rel_err(s2[:, 0, :], w2[:, 0, 0, :]) < REL
 
 
+def test_explicit_reversed_pairs_all_estimators_and_counts(ctx, cfg1):
+    """A record (i, j) with i > j keeps the reference's orientation: row =
+    first DOF.  Chao-Shen evaluates the second DOF's Jacobian at the row bin
+    (Estimators.cpp:558,562), so its S2 depends on the order; the counts of
+    such a record are the transpose of (j, i)'s."""
+    nb, S = 30, 4
+    codes, ext, types = oracle_discretize(cfg1, nb, nb)
+    codes = codes[:, 0, :]
+    D, F = codes.shape
+    rng = np.random.default_rng(11)
+    pairs = rng.integers(0, D, size=(120, 2))
+    pairs = pairs[pairs[:, 0] != pairs[:, 1]]
+    pairs = np.concatenate([pairs, pairs[:20, ::-1]]).astype(np.uint32)
+    ests = (2, 0, 1, 3)
+    s1, s2, mi = ctx.pair_entropy(codes, types, ext, nb, S, pairs=pairs, ests=ests)
+    w1 = O.dof_entropies(codes, types, ext, S, [nb], ests)
+    w2 = O.pair_entropies(codes, types, ext, pairs, S, [nb], ests)
+    assert rel_err(s2, w2[:, :, 0, :]) < REL
+    # the swap changes Chao-Shen where both DOFs carry a Jacobian
+    mi_w = w1[pairs[:, 0], 0, 0, :] + w1[pairs[:, 1], 0, 0, :] - w2[:, 0, 0, :]
+    assert np.max(np.abs(mi - mi_w)) < 1e-10
+    plan = hip.PairPlan(ctx, D, nb, pairs)
+    P = len(pairs)
+    dc, dt, dx = ctx.to_device(codes), ctx.to_device(types), ctx.to_device(ext)
+    ds2 = ctx.alloc((P, 1, S), np.float64)
+    cnt = ctx.alloc((P, S, nb * nb), np.uint32)
+    print(f"explicit plan counts: {P} records", flush=True)
+    ctx.dev_pair_entropy(plan, dc.ptr, F, F, dt.ptr, dx.ptr, S, [0], True, False,
+                         0, ds2.ptr, 0, cnt.ptr)
+    plan.close()
+    want = O.pair_counts(codes, nb, pairs, S)
+    assert np.array_equal(cnt.numpy().reshape(P, S, nb, nb), want.astype(np.uint32))
+    assert rel_err(ds2.numpy()[:, 0, :], w2[:, 1, 0, :]) < REL
+
+
 @pytest.mark.parametrize("bad,F", [(30, 100), (255, 100), (255, 4096), (31, 4096)])
 def test_code_out_of_range_is_an_error(ctx, bad, F):
     codes = np.zeros((4, F), np.uint8)

@@ -10,12 +10,14 @@ every 1-D and 2-D entropy contribution (1e-10 relative), the MIE and MIST
 convergence tables (printed with 4 decimals) and the MIST trees of every step.
 """
 import csv
+import json
 import os
 import subprocess
 
 import numpy as np
 import pytest
 
+import oracle as O
 from centre_amd import cdf
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -26,7 +28,7 @@ KIND_WORD = {"bnd": "bond", "ang": "angle", "tor": "torsion",
              "ba": "bacross", "bd": "bdcross", "ad": "adcross"}
 
 
-def write_inputs(d, g):
+def write_inputs(d, g, hist=""):
     out = os.path.join(d, "out")
     os.makedirs(out)
     for t, fname, content, _, _ in TYPES:
@@ -78,7 +80,7 @@ def write_inputs(d, g):
 [histogram]
     referencenbins       = 30
     nbins                = 30
-[entropy]
+{hist}[entropy]
     workset              = 2D
     usesubset            = true
     useneighbor          = true
@@ -144,3 +146,84 @@ def test_host_example1_entropy_and_scoring(tmp_path, golden):
         assert key(got) == key(want), f"tree edges differ at step {s}"
         assert np.all(np.diff(got[:, 4]) <= 0)
         np.testing.assert_allclose(np.sort(got[:, 4]), np.sort(want[:, 4]), rtol=0, atol=1.5e-6)
+
+
+def accumulated_mids(mn, mx, nb):
+    """binData1D's bin mids: the offset grows by one width per bin."""
+    w = (mx - mn) / nb
+    o, out = mn, []
+    for _ in range(nb):
+        out.append(o + w / 2.0)
+        o += w
+    return out
+
+
+@pytest.mark.gpu
+def test_host_example1_count_files(tmp_path, golden):
+    """histogram.writefreq: hist_*-{1d,2d}.nc hold, per record, the ids,
+    extrema, accumulated bin mids and the prefix counts of steps firstStep,
+    firstStep + stepStride, ... (here 1 and 3 of 4).  The layout follows the
+    reference's stored hist_tor-1d.nc header; counts are bit-exact against
+    the oracle's binData2D restatement, in each record's own row/column order."""
+    cfg, out = write_inputs(str(tmp_path), golden,
+                            "    writefreq            = true\n"
+                            "    writeset             = B1D, A1D, D1D, BB2D, AA2D, DD2D, BA2D, BD2D, AD2D\n"
+                            "    writestepstart       = 1\n"
+                            "    writestepstride      = 2\n")
+    print("running host on example-1 with count files", flush=True)
+    r = subprocess.run([HOST, "-i", cfg], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    with open(os.path.join(os.path.dirname(__file__), "golden", "ex2h_hist_tor-1d_header.json")) as f:
+        ref = json.load(f)
+    steps = [1, 3]
+    codes = {t: golden[f"ex1_codes_{t}"] for t in "bat"}
+    ext = {t: golden[f"ex1_extreme_{t}"] for t in "bat"}
+    F = codes["b"].shape[1]
+    ss = F // 4
+    for t, _, _, _, tag in TYPES:
+        ds = cdf.open_dataset(os.path.join(out, f"hist_{tag}-1d.nc"))
+        assert ds.version == ref["version"] and ds.unlimited == ref["unlimited"]
+        assert [k for k, _ in ref["dims"]] == list(ds.dims)
+        assert [a[0] for a in ref["attrs"]] == list(ds.attrs)
+        assert [(v[0], v[1], v[2]) for v in ref["vars"]] == \
+            [(v.name, list(v.dims), v.nc_type) for v in ds.variables.values()]
+        for k, typ, val in ref["attrs"]:
+            got = ds.attrs[k]
+            if typ != "text":
+                assert np.asarray(got).dtype.name == typ, k
+        assert ds.attrs["content"] == {"bnd": "B", "ang": "A", "tor": "D"}[tag]
+        assert list(ds.attrs["firstStep"]) == [1] and list(ds.attrs["stepStride"]) == [2]
+        ids = ds.read("id")[:, 0]
+        np.testing.assert_array_equal(ids, golden[f"ex1_id_{tag}1d"][:, 0])
+        e = ext[t][ids]
+        np.testing.assert_array_equal(ds.read("extrema")[:, 0, :], e)
+        mids = np.array([accumulated_mids(a, b, 30) for a, b in e[:, :2]])
+        np.testing.assert_array_equal(ds.read("mid"), mids)
+        freq = ds.read("freq")
+        for k, d in enumerate(ids):
+            for q, s in enumerate(steps):
+                want = np.bincount(codes[t][d, :(s + 1) * ss], minlength=30)
+                np.testing.assert_array_equal(freq[k, q], want)
+    kinds = {"bnd": ("b", "b", "B/B"), "ang": ("a", "a", "A/A"), "tor": ("t", "t", "D/D"),
+             "ba": ("b", "a", "B/A"), "bd": ("b", "t", "B/D"), "ad": ("a", "t", "A/D")}
+    for tag, (x, y, content) in kinds.items():
+        ds = cdf.open_dataset(os.path.join(out, f"hist_{tag}-2d.nc"))
+        assert ds.attrs["content"] == content
+        assert list(ds.attrs["storagetype"]) == [0 if x == y else 3]
+        assert dict(ds.dims)["bin_pow_dims_sum"] == 900 and dict(ds.dims)["dimsXbin_sum"] == 60
+        ent = cdf.open_dataset(os.path.join(out, f"entcontri_{tag}-2d.nc"))
+        ids = ds.read("id")
+        np.testing.assert_array_equal(ids, ent.read("id"))
+        ex = ds.read("extrema")
+        np.testing.assert_array_equal(ex[:, 0], ext[x][ids[:, 0]])
+        np.testing.assert_array_equal(ex[:, 1], ext[y][ids[:, 1]])
+        mids = ds.read("mid")
+        for k in range(0, len(ids), 97):
+            want = accumulated_mids(*ext[x][ids[k, 0], :2], 30) + accumulated_mids(*ext[y][ids[k, 1], :2], 30)
+            np.testing.assert_array_equal(mids[k], want)
+        freq = ds.read("freq")
+        # the oracle on one concatenated matrix, rows first DOF, columns second
+        both = np.concatenate([codes[x], codes[y]])
+        pr = np.stack([ids[:, 0], ids[:, 1] + len(codes[x])], 1)
+        want = O.pair_counts(both, 30, pr, 4)[:, steps].reshape(len(ids), 2, 900)
+        np.testing.assert_array_equal(freq, want)

@@ -79,10 +79,18 @@ def test_entropy_1d_from_stored_counts_example2(golden):
         s = O.entropy_1d(O.DIHEDRAL, [O.ML], 1, F, ext[k, 0], ext[k, 1], True,
                          False, [30], freq[k])
         assert abs(s[0] - S1[row[int(d)], step]) <= 1e-12 * abs(S1[row[int(d)], step])
-    # stored bin mids follow min + j*w + w/2
+    # stored bin mids are binData1D's, bit for bit: the offset grows by one
+    # bin width per bin (min + j*w + w/2 differs in the last bits)
     w = (ext[:, 1] - ext[:, 0]) / 30
-    mids = ext[:, :1] + np.arange(30) * w[:, None] + w[:, None] / 2
-    assert np.max(np.abs(mids - golden["ex2_hist_mid"])) < 1e-12
+    mids = np.zeros((len(ext), 30))
+    o = ext[:, 0].copy()
+    for j in range(30):
+        mids[:, j] = o + w / 2.0
+        o = o + w
+    np.testing.assert_array_equal(mids, golden["ex2_hist_mid"])
+    for k in range(0, len(ext), 53):
+        f, m = O.bin_data_1d(np.zeros(4, np.uint8), 1, [30], ext[k, 0], ext[k, 1])
+        np.testing.assert_array_equal(m, golden["ex2_hist_mid"][k])
 
 
 def test_counts_match_bruteforce(golden):
