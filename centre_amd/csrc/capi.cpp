@@ -231,6 +231,16 @@ int centre_dev_dof_entropy(centre_ctx* ctx, const uint8_t* d_codes,
 }
 
 // ---------------------------------------------------------------------------
+// Tiles with a full block of R DOFs on both sides go first: the eight-wave
+// kernel takes them; tiles of the last, partial block follow.  Returns how
+// many lead.
+static int full_tiles_first(std::vector<int2>& tiles, uint32_t n_dof, int R) {
+    const int nfull = (int)(n_dof / (uint32_t)R);   // blocks 0..nfull-1 are full
+    auto full = [&](const int2& t) { return t.x < nfull && t.y < nfull; };
+    auto mid = std::stable_partition(tiles.begin(), tiles.end(), full);
+    return (int)(mid - tiles.begin());
+}
+
 int centre_pair_dofs_per_tile(int nbins) {
     return (nbins < 1 || nbins > 255) ? -3 : dofs_per_tile(nbins);
 }
@@ -258,6 +268,7 @@ int centre_pair_plan_from_tiles(centre_ctx* ctx, uint32_t n_dof, int nbins, cons
     plan->all_pairs = true;
     plan->n_canon = (uint64_t)n_dof * (n_dof - 1) / 2;
     plan->n_rec = plan->n_canon;
+    plan->n_full = full_tiles_first(tl, n_dof, R);
     plan->n_tiles = (int)tl.size();
     hipError_t e = hipMalloc(&plan->d_tiles, sizeof(int2) * (tl.size() ? tl.size() : 1));
     if (e == hipSuccess && !tl.empty())
@@ -337,6 +348,7 @@ int centre_pair_plan_create(centre_ctx* ctx, uint32_t n_dof, int nbins,
             return fail(ctx, (int)e, std::string("plan upload: ") + hipGetErrorString(e));
         }
     }
+    plan->n_full = full_tiles_first(tiles, n_dof, R);
     plan->n_tiles = (int)tiles.size();
     hipError_t e = hipMalloc(&plan->d_tiles, sizeof(int2) * (tiles.size() ? tiles.size() : 1));
     if (e == hipSuccess && !tiles.empty())
@@ -455,17 +467,28 @@ struct BatchRun {
     std::vector<double> sys_ms;
 };
 
-// One persistent launch over a prepared tile list.
+// Persistent launches over a prepared tile list: tiles of a system's last,
+// partial block first (four-wave tile), then the full tiles (eight-wave
+// tile), each list in the given order.  Every system's completion stamp is
+// taken by whichever launch finishes its last tile.
 int batch_launch(centre_ctx* ctx, const PairKernelArgs& base, const std::vector<BatchSystem>& bs,
                  const std::vector<BatchTile>& tl, int grid, BatchRun* run) {
     const int n_sys = (int)bs.size();
-    std::vector<int2> tiles(tl.size());
-    std::vector<int> tsys(tl.size()), per(n_sys, 0);
-    for (size_t q = 0; q < tl.size(); ++q) {
-        tiles[q] = make_int2(tl[q].I, tl[q].J);
-        tsys[q] = tl[q].sys;
-        per[tl[q].sys]++;
-    }
+    const bool w8 = pair_tile8_supported(base.R);
+    std::vector<int2> tiles;
+    std::vector<int> tsys, per(n_sys, 0);
+    for (int pass = 0; pass < 2; ++pass)
+        for (const BatchTile& q : tl) {
+            const int nfull = bs[q.sys].n_dof / base.R;
+            const bool full = w8 && q.I < nfull && q.J < nfull;
+            if (full != (pass == 1)) continue;
+            tiles.push_back(make_int2(q.I, q.J));
+            tsys.push_back(q.sys);
+            per[q.sys]++;
+        }
+    int n_part = 0;
+    for (const BatchTile& q : tl)
+        n_part += !(w8 && q.I < bs[q.sys].n_dof / base.R && q.J < bs[q.sys].n_dof / base.R);
     DevBuf dt, ds, dsys, dper, dcnt, dtime, dwork;
     HIPCHK(dt.alloc(sizeof(int2) * tiles.size()));
     HIPCHK(ds.alloc(sizeof(int) * tsys.size()));
@@ -473,26 +496,33 @@ int batch_launch(centre_ctx* ctx, const PairKernelArgs& base, const std::vector<
     HIPCHK(dper.alloc(sizeof(int) * n_sys));
     HIPCHK(dcnt.alloc(sizeof(int) * n_sys));
     HIPCHK(dtime.alloc(sizeof(unsigned long long) * (n_sys + 1)));
-    HIPCHK(dwork.alloc(sizeof(int)));
+    HIPCHK(dwork.alloc(2 * sizeof(int)));
     HIPCHK(hipMemcpyAsync(dt.p, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(ds.p, tsys.data(), sizeof(int) * tsys.size(), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(dsys.p, bs.data(), sizeof(BatchSystem) * n_sys, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(dper.p, per.data(), sizeof(int) * n_sys, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemsetAsync(dcnt.p, 0, sizeof(int) * n_sys, ctx->stream));
     HIPCHK(hipMemsetAsync(dtime.p, 0, sizeof(unsigned long long) * (n_sys + 1), ctx->stream));
-    HIPCHK(hipMemsetAsync(dwork.p, 0, sizeof(int), ctx->stream));
+    HIPCHK(hipMemsetAsync(dwork.p, 0, 2 * sizeof(int), ctx->stream));
     PairKernelArgs a = base;
     a.systems = dsys.as<BatchSystem>();
     a.tiles = dt.as<int2>();
     a.tile_sys = ds.as<int>();
     a.work = dwork.as<int>();
-    a.n_work = (int)tiles.size();
+    a.n_work = n_part;
     a.sys_count = dcnt.as<int>();
     a.sys_tiles = dper.as<int>();
     a.sys_time = dtime.as<unsigned long long>() + 1;
     a.t0 = dtime.as<unsigned long long>();
+    PairKernelArgs f = a;
+    f.tiles = a.tiles + n_part;
+    f.tile_sys = a.tile_sys + n_part;
+    f.work = a.work + 1;
+    f.n_work = (int)tiles.size() - n_part;
+    if (n_part > 0) f.t0 = nullptr;
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
     HIPCHK(launch_pair_persistent(a, grid, ctx->stream));
+    HIPCHK(launch_pair_persistent8(f, grid, ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
     std::vector<unsigned long long> stamps(n_sys + 1);
     HIPCHK(hipMemcpyAsync(stamps.data(), dtime.p, sizeof(unsigned long long) * (n_sys + 1),
@@ -558,9 +588,9 @@ extern "C" int centre_batch_pair_entropy(centre_ctx* ctx, const centre_batch_sys
         const double* s1_first = s1[k].as<double>();
         if (nest > 1) {   // the kernel reads [D][S] of the first estimator
             HIPCHK(s1e0[k].alloc(sizeof(double) * D * nsteps));
-            HIPCHK(hipMemcpy2DAsync(s1e0[k].p, sizeof(double) * nsteps, s1[k].p,
-                                    sizeof(double) * nsteps * nest, sizeof(double) * nsteps, D,
-                                    hipMemcpyDeviceToDevice, ctx->stream));
+            HIPCHK(launch_copy_2d(s1e0[k].p, sizeof(double) * nsteps, s1[k].p,
+                                  sizeof(double) * nsteps * nest, sizeof(double) * nsteps, D,
+                                  ctx->stream));
             s1_first = s1e0[k].as<double>();
         }
         const uint8_t* pc = nullptr;
@@ -705,15 +735,24 @@ extern "C" int centre_dev_pair_entropy(centre_ctx* ctx, const centre_pair_plan* 
     DevBuf s1e0;
     if (nest > 1 && d_mi) {
         HIPCHK(s1e0.alloc(sizeof(double) * D * nsteps));
-        HIPCHK(hipMemcpy2DAsync(s1e0.p, sizeof(double) * nsteps, s1.p,
-                                sizeof(double) * nsteps * nest, sizeof(double) * nsteps, D,
-                                hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHK(launch_copy_2d(s1e0.p, sizeof(double) * nsteps, s1.p,
+                              sizeof(double) * nsteps * nest, sizeof(double) * nsteps, D,
+                              ctx->stream));
         a.dof_S1 = s1e0.as<double>();
     }
     if (!d_mi) a.mi = nullptr;
     if (d_mi && ests[0] > 1) a.mi = nullptr;  // MI for CS/JS computed below
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-    HIPCHK(launch_pair_tiles(a, plan->n_tiles, dump, ctx->stream));
+    if (!dump && pair_tile8_supported(a.R) && plan->n_full > 0 &&
+        ((a.stride | a.sstride | reinterpret_cast<uintptr_t>(a.codes)) & 3) == 0) {
+        // full tiles on the eight-wave kernel, those of a partial block after
+        HIPCHK(launch_pair_tiles8(a, plan->n_full, ctx->stream));
+        PairKernelArgs r = a;
+        r.tiles = plan->d_tiles + plan->n_full;
+        HIPCHK(launch_pair_tiles(r, plan->n_tiles - plan->n_full, false, ctx->stream));
+    } else {
+        HIPCHK(launch_pair_tiles(a, plan->n_tiles, dump, ctx->stream));
+    }
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
 
     if (!all && d_counts)
@@ -1110,6 +1149,18 @@ extern "C" int centre_memcpy_d2d(centre_ctx* ctx, void* d_dst, const void* d_src
     if (!bytes) return 0;
     if (!d_dst || !d_src) return fail(ctx, -3, "null pointer");
     HIPCHK(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+extern "C" int centre_memcpy_d2d_2d(centre_ctx* ctx, void* d_dst, size_t dpitch, const void* d_src,
+                                    size_t spitch, size_t width, size_t height) {
+    if (!ctx) return -3;
+    if (!width || !height) return 0;
+    if (!d_dst || !d_src) return fail(ctx, -3, "null pointer");
+    if (dpitch < width || spitch < width) return fail(ctx, -3, "pitch < width");
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(launch_copy_2d(d_dst, dpitch, d_src, spitch, width, height, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return 0;
 }

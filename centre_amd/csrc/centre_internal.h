@@ -29,6 +29,7 @@ struct centre_pair_plan {
     uint64_t* d_rec2canon = nullptr;  // [n_rec] (explicit lists only)
     uint32_t* d_rec_ij = nullptr;     // [n_rec][2] as given (explicit lists only)
     bool all_pairs = true;
+    int n_full = 0;                   // leading tiles with R DOFs on both sides
 };
 
 // Bins of one-hot rows per tile side (a multiple of 32, the MFMA block).
@@ -96,6 +97,13 @@ struct PairKernelArgs {
 
 
 // Launchers (return hipError_t of the launch).
+// Eight-wave kernel for tiles with R DOFs on both sides and step-aligned codes
+// (no count dump); supported for R with 2R*kGroups <= 512 whose layout keeps
+// the c log c table.
+bool pair_tile8_supported(int R);
+hipError_t launch_pair_tiles8(const PairKernelArgs& a, int n_tiles, hipStream_t st);
+// Persistent form for batches (full tiles only, step-aligned codes).
+hipError_t launch_pair_persistent8(const PairKernelArgs& a, int grid, hipStream_t st);
 hipError_t launch_pair_tiles(const PairKernelArgs& a, int n_tiles, bool dump,
                              hipStream_t st);
 size_t pair_tile_lds_bytes(int R);
@@ -209,6 +217,8 @@ hipError_t launch_segment_sum(const double* x, const uint64_t* d_beg, int n_seg,
 hipError_t launch_gather_rec(const double* src, const uint64_t* rec2canon,
                              uint64_t n_rec, int width, double* dst,
                              hipStream_t st);
+hipError_t launch_copy_2d(void* dst, size_t dpitch, const void* src, size_t spitch, size_t width,
+                          size_t height, hipStream_t st);
 // dst[r][s][x*nb + y] = counts of record r = (i, j) at step s, row bin x of i,
 // column bin y of j, from canonical counts (row DOF < column DOF).
 hipError_t launch_gather_counts(const uint32_t* src, const uint64_t* rec2canon,

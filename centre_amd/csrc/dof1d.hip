@@ -216,7 +216,35 @@ __global__ void gather_counts_kernel(const uint32_t* src, const uint64_t* rec2ca
     dst[t] = src[(rec2canon[r] * S + s) * nb2 + qs];
 }
 
+// Pitched copy of height rows of width bytes, in 8-byte words when every
+// row start is 8-byte aligned (narrow rows: the runtime's 2-D copy issues
+// one copy per row).
+__global__ void copy_2d_kernel(uint8_t* dst, size_t dpitch, const uint8_t* src, size_t spitch,
+                               size_t width, size_t height, int words) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t per = words ? width / 8 : width;
+    if (t >= per * height) return;
+    const uint64_t r = t / per, c = t - r * per;
+    if (words)
+        reinterpret_cast<uint64_t*>(dst + r * dpitch)[c] =
+            reinterpret_cast<const uint64_t*>(src + r * spitch)[c];
+    else
+        dst[r * dpitch + c] = src[r * spitch + c];
+}
+
 }  // namespace
+
+hipError_t launch_copy_2d(void* dst, size_t dpitch, const void* src, size_t spitch, size_t width,
+                          size_t height, hipStream_t st) {
+    const int words = ((width | dpitch | spitch | reinterpret_cast<uintptr_t>(dst) |
+                        reinterpret_cast<uintptr_t>(src)) & 7) == 0;
+    const uint64_t n = (words ? width / 8 : width) * (uint64_t)height;
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(copy_2d_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                       static_cast<uint8_t*>(dst), dpitch, static_cast<const uint8_t*>(src), spitch,
+                       width, height, words);
+    return hipGetLastError();
+}
 
 hipError_t launch_gather_counts(const uint32_t* src, const uint64_t* rec2canon,
                                 const uint32_t* rec_ij, uint64_t n_rec, int nsteps, int nb,

@@ -71,7 +71,7 @@ struct LdsLayout {
     int ntab;      // 0 (no table) or kTab
 };
 
-__host__ __device__ inline LdsLayout lds_layout(int R) {
+__host__ __device__ inline LdsLayout lds_layout(int R, int nwave = 4) {
     LdsLayout L;
     size_t off = (size_t)2 * 2 * kOpBytes;
     L.cl = off;
@@ -87,9 +87,9 @@ __host__ __device__ inline LdsLayout lds_layout(int R) {
     off = (off + 15) & ~(size_t)15;
     L.log = off;
     off += 2 * 256 * sizeof(double);
-    // epilogue parking area of the fast path (R <= 32, i.e. <= 4 staging slots)
+    // epilogue parking area of the fast paths (R <= 32): 4 KB per wave
     L.park = off;
-    if (R <= 32) off += (size_t)4 * 64 * 16 * sizeof(int);
+    if (R <= 32) off += (size_t)nwave * 64 * 16 * sizeof(int);
     L.tab = off;
     L.ntab = off + sizeof(double) * kTab <= kLdsCap ? kTab : 0;
     off += sizeof(double) * L.ntab;
@@ -748,6 +748,327 @@ __global__ __launch_bounds__(256) void pair_persist_kernel(PairKernelArgs a) {
     }
 }
 
+
+// ---- eight-wave kernel for full tiles --------------------------------------------
+// Tiles with R DOFs on both sides (diagonal tiles included: both operands
+// are then the same rows, and only pairs i < j are kept) and 4-byte aligned
+// step starts, with 8 waves in a 2x4 layout: each wave owns 128 rows x 64
+// columns (4x2 MFMA blocks, 128 accumulator registers), so two waves share a
+// SIMD and one wave's fragment reads, staging and epilogue issue while the
+// other one's MFMAs run.  Only the first ceil(live slots/64) waves stage
+// codes (one slot per lane, lanes past the live slots repeat live ones of
+// their own wave); the others only multiply.
+template <int kW>
+__device__ __forceinline__ void epi_block8(const v16i& blk, int row0, int gcol, int lane, int* park,
+                                           const Smem& sm, int nb, int R, int nstr, int ntab) {
+    // the lane holds column gcol and rows row0 + 8*(r/4) + r%4, r = 0..15
+    const int cd = sm.col2dof[gcol];
+    v4i* pk = reinterpret_cast<v4i*>(park) + lane;
+    pk[0] = v4i{blk[0], blk[1], blk[2], blk[3]};
+    pk[64] = v4i{blk[4], blk[5], blk[6], blk[7]};
+    pk[128] = v4i{blk[8], blk[9], blk[10], blk[11]};
+    pk[192] = v4i{blk[12], blk[13], blk[14], blk[15]};
+    asm volatile("" ::: "memory");
+    const int stripe = lane % nstr;
+    if (nb >= 28) {
+        const int dlo = sm.row2dof[row0];
+        const bool ok = (cd != 0xFF) & (dlo != 0xFF);
+        const int kk = (dlo + 1) * nb - row0;
+        double tot = 0.0, hi = 0.0;
+        int ntot = 0, nhi = 0;
+        // four rows at a time keeps few values live (this runs between MFMAs
+        // of the software-pipelined loop)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const v4i c4 = pk[64 * q];
+            int c[4];
+            double tv[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                c[t] = ok ? c4[t] : 0;
+                tv[t] = sm.ctab[min(c[t], ntab - 1)];
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                if (__builtin_amdgcn_ballot_w64(c[t] >= ntab) != 0 && c[t] >= ntab)
+                    tv[t] = (double)c[t] * log_count(c[t], sm.logt, sm.invt);
+                const bool h = 8 * q + t >= kk;
+                const int nz = min(c[t], 1);
+                tot += tv[t];
+                hi += h ? tv[t] : 0.0;
+                ntot += nz;
+                nhi += h ? nz : 0;
+            }
+        }
+        if (!ok) return;
+        if (ntot > nhi) {
+            atomicAdd(&sm.slotT[(dlo * R + cd) * nstr + stripe], tot - hi);
+            atomicAdd(&sm.slotN[(dlo * R + cd) * nstr + stripe], ntot - nhi);
+        }
+        const int dhi = sm.row2dof[row0 + 27];
+        if (nhi && dhi != 0xFF && dhi != dlo) {
+            atomicAdd(&sm.slotT[(dhi * R + cd) * nstr + stripe], hi);
+            atomicAdd(&sm.slotN[(dhi * R + cd) * nstr + stripe], nhi);
+        }
+        return;
+    }
+    if (cd == 0xFF) return;
+#pragma unroll 1
+    for (int q = 0; q < 4; ++q) {
+        const v4i c4 = pk[64 * q];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int rd = sm.row2dof[row0 + 8 * q + t];
+            const int cnt = c4[t];
+            if (rd != 0xFF && cnt > 0) {
+                const double v = cnt < ntab ? sm.ctab[cnt]
+                                            : (double)cnt * log_count(cnt, sm.logt, sm.invt);
+                atomicAdd(&sm.slotT[(rd * R + cd) * nstr + stripe], v);
+                atomicAdd(&sm.slotN[(rd * R + cd) * nstr + stripe], 1);
+            }
+        }
+    }
+}
+
+template <bool kBatch>
+__device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int t) {
+    extern __shared__ __align__(16) uint8_t smem_raw[];
+    PairKernelArgs a = a_in;
+    if constexpr (kBatch) {
+        const BatchSystem& sd = a_in.systems[a_in.tile_sys[t]];
+        a.codes = sd.codes;
+        a.stride = sd.stride;
+        a.n_dof = sd.n_dof;
+        a.ss = sd.ss;
+        a.sstride = sd.sstride;
+        a.s2 = sd.s2;
+        a.mi = sd.mi;
+        a.dof_w = sd.dof_w;
+        a.dof_L = sd.dof_L;
+        a.dof_S1 = sd.dof_S1;
+    }
+    const int R = a.R, nb = a.nb, S = a.nsteps;
+    const LdsLayout lay = lds_layout(R, 8);
+    const int nstr = lay.nstripe, ntab = lay.ntab;
+    Smem sm;
+    sm.oh = smem_raw;
+    sm.cl = reinterpret_cast<uint32_t*>(smem_raw + lay.cl);
+    sm.row2dof = smem_raw + lay.map;
+    sm.col2dof = smem_raw + lay.map + 256;
+    sm.slotT = reinterpret_cast<double*>(smem_raw + lay.T);
+    sm.slotN = reinterpret_cast<int*>(smem_raw + lay.N);
+    sm.logt = reinterpret_cast<double*>(smem_raw + lay.log);
+    sm.invt = sm.logt + 256;
+    sm.ctab = reinterpret_cast<double*>(smem_raw + lay.tab);
+    uint8_t* const trash = smem_raw + lay.map + 512;
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wr = wave >> 2, wc = wave & 3;
+    const int2 tile = a.tiles[t];
+    const int r0 = tile.x * R, c0 = tile.y * R;
+    const bool diag = tile.x == tile.y;
+    const size_t boff = diag ? 0 : kOpBytes;   // column operand
+    const long long D = a.n_dof;
+
+    for (int i = tid; i < 2 * 2 * kOpBytes / 16; i += 512)
+        reinterpret_cast<uint4*>(sm.oh)[i] = make_uint4(0, 0, 0, 0);
+    if (tid < 256) {
+        const int d = tid / nb;
+        sm.row2dof[tid] = (uint8_t)(d < R ? d : 0xFF);
+        sm.col2dof[tid] = (uint8_t)(d < R ? d : 0xFF);
+        const double t = 1.0 + tid * (1.0 / 256.0);
+        sm.logt[tid] = log(t);
+        sm.invt[tid] = 1.0 / t;
+    }
+    for (int i = tid; i < R * R * nstr; i += 512) {
+        sm.slotT[i] = 0.0;
+        sm.slotN[i] = 0;
+    }
+    for (int c = tid; c < ntab; c += 512) sm.ctab[c] = c > 1 ? (double)c * log((double)c) : 0.0;
+    if (tid == 0) *reinterpret_cast<int*>(trash) = *reinterpret_cast<volatile int*>(a.flag);
+    __syncthreads();
+    // codes were range-checked by the marginal pass; with the flag set the
+    // results are discarded
+    if (*reinterpret_cast<const int*>(trash) != 0) return;
+
+    const int nch = (int)((a.ss + kChunk - 1) / kChunk);
+    const int G = S * nch;
+    const int nlive = (diag ? 1 : 2) * R * kGroups;
+    const bool stager = wave * 64 < nlive;   // wave-uniform
+    // this lane's slot: DOF ld, frames k4..k4+3 of each chunk
+    const int q = (wave * 64 + lane) % nlive;
+    const int ld = q / kGroups, k4 = (q - ld * kGroups) * 4;
+    const bool isA = ld < R;
+    const uint8_t* const src = a.codes + (uint64_t)(isA ? r0 + ld : c0 + (ld - R)) * a.stride + k4;
+    const uint32_t dst = (uint32_t)((isA ? 0 : kOpBytes) + (isA ? ld : ld - R) * nb * kLdsRow + k4);
+    uint32_t cur[kSuper], nxt[kSuper], last0 = 0u, last1 = 0u;
+    int ls = 0, lc = 0;
+    auto advance = [](int& s, int& c, int n) __attribute__((always_inline)) {
+        if (++c == n) { c = 0; ++s; }
+    };
+    auto nvalid_of = [&](int s, int c) __attribute__((always_inline)) -> int {
+        return s < S ? (int)min((uint64_t)kChunk, a.ss - (uint64_t)c * kChunk) : 0;
+    };
+    auto load4 = [&](uint32_t (&d)[kSuper]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int c = 0; c < kSuper; ++c) {
+            const uint64_t f0 = (uint64_t)ls * a.sstride + (uint64_t)lc * kChunk;
+            const int nv = nvalid_of(ls, lc);
+            d[c] = *reinterpret_cast<const uint32_t*>(k4 < nv ? src + f0 : a.codes);
+            advance(ls, lc, nch);
+        }
+    };
+    // chunk with nv valid frames from dword cis of cw into buffer b (clear
+    // what this slot set two chunks ago, set the new bytes; frames past nv go
+    // to the trash row)
+    auto stage = [&](int b, int cis, int nv, const uint32_t (&cw)[kSuper]) __attribute__((always_inline)) {
+        uint8_t* const row = sm.oh + (size_t)b * 2 * kOpBytes + dst;
+        const int nvk = nv - k4;
+        const uint32_t vmask = nvk >= 4 ? 0xFFFFFFFFu : (nvk <= 0 ? 0u : 0xFFFFFFFFu >> (8 * (4 - nvk)));
+        uint32_t wv = 0;
+#pragma unroll
+        for (int c = 0; c < kSuper; ++c) wv |= cw[c] & (0u - (uint32_t)(cis == c));
+        wv &= vmask;
+        const uint32_t old = b ? last1 : last0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) row[((old >> (8 * e)) & 0xFFu) * kLdsRow + e] = 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            uint8_t* const srow = nvk > e ? row : trash + 16 + k4;
+            srow[((wv >> (8 * e)) & 0xFFu) * kLdsRow + e] = 1;
+        }
+        last1 = b ? wv : last1;
+        last0 = b ? last0 : wv;
+    };
+    v16i acc[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = v16i{};
+    auto frags = [&](const uint8_t* base, int kk, v4i (&af)[4], v4i (&bf)[2]) __attribute__((always_inline)) {
+        const int koff = kk * 32 + (lane >> 5) * 16;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            af[i] = *reinterpret_cast<const v4i*>(base + (128 * wr + 32 * i + (lane & 31)) * kLdsRow + koff);
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            bf[j] = *reinterpret_cast<const v4i*>(base + boff + (64 * wc + 32 * j + (lane & 31)) * kLdsRow + koff);
+    };
+    auto mfma8 = [&](const v4i (&af)[4], const v4i (&bf)[2]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
+    };
+    int* const park = reinterpret_cast<int*>(smem_raw + lay.park) + wave * 64 * 16;
+
+    int ps = 0, pc = 0;   // chunk to stage next
+    int ms = 0, mc = 0;   // chunk being multiplied
+    auto step_end = [&]() __attribute__((always_inline)) {
+        // step ms ends with chunk g
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                epi_block8<8>(acc[i][j], 128 * wr + 32 * i + 4 * (lane >> 5),
+                              64 * wc + 32 * j + (lane & 31), lane, park, sm, nb, R, nstr, ntab);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        __syncthreads();
+        const double n = (double)(ms + 1) * (double)a.ss;
+        const double logn = log(n);
+        for (int pq = tid; pq < R * R; pq += 512) {
+            const int rd = pq / R, cd = pq - rd * R;
+            const long long gi = r0 + rd, gj = c0 + cd;
+            const long long p = gi >= gj ? -1
+                              : (a.pairmap ? (long long)a.pairmap[gi * D + gj] : tri_index(gi, gj, D));
+            double T = 0.0;
+            int nz = 0;
+            for (int k = 0; k < nstr; ++k) {
+                T += sm.slotT[pq * nstr + k];
+                nz += sm.slotN[pq * nstr + k];
+                sm.slotT[pq * nstr + k] = 0.0;
+                sm.slotN[pq * nstr + k] = 0;
+            }
+            if (p >= 0) {
+                const double L = a.dof_L[gi * S + ms] + a.dof_L[gj * S + ms];
+                const double area = a.dof_w[gi] * a.dof_w[gj];
+                const double ml = (logn + log(area)) - T / n + L / n;
+                const double mm = ml + (double)(unsigned)(nz - 1) / (2.0 * n);
+                if (a.slot_ml >= 0) a.s2[((size_t)p * a.nest + a.slot_ml) * S + ms] = ml;
+                if (a.slot_mm >= 0) a.s2[((size_t)p * a.nest + a.slot_mm) * S + ms] = mm;
+                if (a.mi) {
+                    const double s1 = a.dof_S1[gi * S + ms] + a.dof_S1[gj * S + ms];
+                    a.mi[(size_t)p * S + ms] = s1 - (a.mi_from_mm ? mm : ml);
+                }
+            }
+        }
+    };
+    auto stage_next = [&](int b) __attribute__((always_inline)) {
+        // chunk at (ps, pc) into buffer b
+        if (stager && ps < S) {
+            const int cis = (ps * nch + pc) & (kSuper - 1);
+            if (cis == 0 && !(ps == 0 && pc == 0)) {
+#pragma unroll
+                for (int c = 0; c < kSuper; ++c) cur[c] = nxt[c];
+                load4(nxt);
+            }
+            stage(b, cis, nvalid_of(ps, pc), cur);
+        }
+        advance(ps, pc, nch);
+    };
+    if (stager) {
+        load4(cur);
+        load4(nxt);
+    }
+    // one barrier per chunk: chunk g multiplies from buffer g&1 while the
+    // stager waves write chunk g+1 into the other buffer.  (A variant that
+    // reads chunk g+1's fragments during chunk g's MFMAs needs two sets of
+    // fragment registers next to the 128 accumulators and spills.)
+    stage_next(0);
+    __syncthreads();
+    for (int g = 0; g < G; ++g) {
+        const int b = g & 1;
+        const uint8_t* base = sm.oh + (size_t)b * 2 * kOpBytes;
+        v4i fa0[4], fb0[2], fa1[4], fb1[2];
+        frags(base, 0, fa0, fb0);
+        frags(base, 1, fa1, fb1);
+        mfma8(fa0, fb0);
+        mfma8(fa1, fb1);
+        stage_next(b ^ 1);
+        if (mc == nch - 1) step_end();
+        advance(ms, mc, nch);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(512) void pair_tile8_kernel(PairKernelArgs a) {
+    tile8_body<false>(a, blockIdx.x);
+}
+
+// Persistent batch form of the eight-wave tile (see pair_persist_kernel).
+__global__ __launch_bounds__(512) void pair_persist8_kernel(PairKernelArgs a) {
+    __shared__ int s_t;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && a.t0) *a.t0 = wall_clock64();
+    for (;;) {
+        __syncthreads();   // the previous tile is done with LDS and s_t
+        if (threadIdx.x == 0) s_t = atomicAdd(a.work, 1);
+        __syncthreads();
+        const int t = s_t;
+        if (t >= a.n_work) break;
+        tile8_body<true>(a, t);
+        if (threadIdx.x == 0 && a.sys_count) {
+            const int sys = a.tile_sys[t];
+            __threadfence();
+            if (atomicAdd(&a.sys_count[sys], 1) == a.sys_tiles[sys] - 1)
+                a.sys_time[sys] = wall_clock64();
+        }
+    }
+}
+
 // Copies the ss frames of every step of every DOF to rows where each step
 // starts on a 4-byte boundary: out[d][s * ss4 + f] = codes[d][s * ss + f]
 // (padding bytes 0, never counted).
@@ -811,6 +1132,33 @@ hipError_t launch_pair_persistent(const PairKernelArgs& a, int grid, hipStream_t
     if (need <= 2) return launch_p<2>(a, grid, lds, st);
     if (need <= 4) return launch_p<4>(a, grid, lds, st);
     return launch_p<8>(a, grid, lds, st);
+}
+
+bool pair_tile8_supported(int R) {
+    const LdsLayout L = lds_layout(R, 8);
+    return R <= 32 && 2 * R * kGroups <= 512 && L.ntab > 0 && L.total <= kLdsCap;
+}
+
+hipError_t launch_pair_tiles8(const PairKernelArgs& a, int n_tiles, hipStream_t st) {
+    if (n_tiles <= 0) return hipSuccess;
+    if (!pair_tile8_supported(a.R)) return hipErrorInvalidValue;
+    const size_t lds = lds_layout(a.R, 8).total;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&pair_tile8_kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(pair_tile8_kernel, dim3(n_tiles), dim3(512), lds, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_pair_persistent8(const PairKernelArgs& a, int grid, hipStream_t st) {
+    if (a.n_work <= 0 || grid <= 0) return hipSuccess;
+    if (!pair_tile8_supported(a.R)) return hipErrorInvalidValue;
+    const size_t lds = lds_layout(a.R, 8).total;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&pair_persist8_kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(pair_persist8_kernel, dim3(grid), dim3(512), lds, st, a);
+    return hipGetLastError();
 }
 
 hipError_t launch_pair_tiles(const PairKernelArgs& a, int n_tiles, bool dump,

@@ -24,18 +24,31 @@ namespace {
 
 constexpr double kDegRad = 0.017453292519943295;
 
-__global__ void vm_prep_kernel(VmkdeArgs a) {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t n = (uint64_t)a.n_dof * a.n_frame;
-    if (t >= n) return;
-    // t enumerates [frame][dof] so the reads coalesce
-    const uint64_t f = t / a.n_dof, d = t - f * a.n_dof;
-    const float x = (float)a.bat[f * a.ld + d];
-    a.xt[d * a.n_frame + f] = x;
-    double s, c;
-    sincos((double)x, &s, &c);
-    a.trig[(d * a.n_frame + f) * 2] = c;
-    a.trig[(d * a.n_frame + f) * 2 + 1] = s;
+// 64 frames x 64 torsions per workgroup, transposed through LDS so that both
+// the [frame][dof] reads and the [dof][frame] writes coalesce
+__global__ __launch_bounds__(256) void vm_prep_kernel(VmkdeArgs a) {
+    __shared__ float tile[64][65];
+    const uint64_t f0 = (uint64_t)blockIdx.x * 64;
+    const uint32_t d0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int r = ty; r < 64; r += 4) {
+        const uint64_t f = f0 + r;
+        const uint32_t d = d0 + tx;
+        tile[r][tx] = (f < a.n_frame && d < a.n_dof) ? (float)a.bat[f * a.ld + d] : 0.f;
+    }
+    __syncthreads();
+    for (int r = ty; r < 64; r += 4) {
+        const uint32_t d = d0 + r;
+        const uint64_t f = f0 + tx;
+        if (d < a.n_dof && f < a.n_frame) {
+            const float x = tile[tx][r];
+            const uint64_t o = (uint64_t)d * a.n_frame + f;
+            a.xt[o] = x;
+            double sn, cs;
+            sincos((double)x, &sn, &cs);
+            reinterpret_cast<double2*>(a.trig)[o] = make_double2(cs, sn);
+        }
+    }
 }
 
 constexpr int kPts = 32;      // grid points per workgroup
@@ -184,7 +197,8 @@ hipError_t run_vmkde(const VmkdeArgs& a, hipStream_t st) {
     const double h = centre_vm_bandwidth(a.n_frame, a.kappa);
     const double denom = 6.283185307179586 * (double)a.n_frame * centre_bessel_i(0, h);
     const uint64_t n = (uint64_t)a.n_dof * a.n_frame;
-    hipLaunchKernelGGL(vm_prep_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(vm_prep_kernel, dim3((unsigned)((a.n_frame + 63) / 64), (a.n_dof + 63) / 64),
+                       dim3(256), 0, st, a);
     hipLaunchKernelGGL(vm_grid_kernel, dim3((a.grid + 1 + kPts - 1) / kPts, a.n_dof), dim3(256),
                        0, st, a, h, denom);
     hipLaunchKernelGGL(vm_states_kernel, dim3((a.n_dof + 63) / 64), dim3(64), 0, st, a);

@@ -61,6 +61,7 @@ _SIGS = {
     "centre_memcpy_h2d": ([P, P, P, C.c_size_t], I32),
     "centre_memcpy_d2h": ([P, P, P, C.c_size_t], I32),
     "centre_memcpy_d2d": ([P, P, P, C.c_size_t], I32),
+    "centre_memcpy_d2d_2d": ([P, P, C.c_size_t, P, C.c_size_t, C.c_size_t, C.c_size_t], I32),
     "centre_device_sync": ([P], I32),
 }
 
@@ -218,6 +219,9 @@ class Context:
 
     def copy_d2d(self, dst: int, src: int, nbytes: int):
         self.check(self._L.centre_memcpy_d2d(self.h, dst, src, nbytes))
+
+    def copy_2d(self, dst: int, dpitch: int, src: int, spitch: int, width: int, height: int):
+        self.check(self._L.centre_memcpy_d2d_2d(self.h, dst, dpitch, src, spitch, width, height))
 
     def last_pair_kernel_ms(self) -> float:
         return float(self._L.centre_last_pair_kernel_ms(self.h))

@@ -183,6 +183,11 @@ int centre_memcpy_d2h(centre_ctx *ctx, void *dst, const void *d_src,
                       size_t bytes);
 int centre_memcpy_d2d(centre_ctx *ctx, void *d_dst, const void *d_src,
                       size_t bytes);
+/* Pitched device copy: height rows of width bytes (e.g. one step's MI
+ * column of [pair][step] as MST edge weights). */
+int centre_memcpy_d2d_2d(centre_ctx *ctx, void *d_dst, size_t dpitch,
+                         const void *d_src, size_t spitch, size_t width,
+                         size_t height);
 /* hipDeviceSynchronize on the context's device. */
 int centre_device_sync(centre_ctx *ctx);
 

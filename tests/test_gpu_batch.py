@@ -63,9 +63,10 @@ def test_batch_matches_single_system_path(nb, ests):
         assert tot > 0 and np.all(sys_ms > 0) and np.all(sys_ms <= tot * 1.05 + 0.05)
         for (a1, a2, am), o in zip(want, outs):
             # S1 comes from the same sequential per-DOF sums: identical.  Pair
-            # sums are FP64 LDS atomics whose order varies from run to run,
-            # so S2 and MI agree to a few ulp.
+            # sums are FP64 LDS atomics whose order varies from run to run and
+            # between the batch kernel and the eight-wave tile kernel, so S2
+            # and MI agree to a few ulp of the O(1) terms summed.
             np.testing.assert_array_equal(o[0].numpy(), a1)
-            np.testing.assert_allclose(o[1].numpy(), a2, rtol=1e-13, atol=0)
-            np.testing.assert_allclose(o[2].numpy(), am, rtol=1e-12, atol=1e-15)
+            np.testing.assert_allclose(o[1].numpy(), a2, rtol=1e-12, atol=1e-14)
+            np.testing.assert_allclose(o[2].numpy(), am, rtol=1e-12, atol=1e-14)
     ctx.close()

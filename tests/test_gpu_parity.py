@@ -157,6 +157,33 @@ def test_ragged_shapes_vs_oracle(ctx, nb, D, F, S):
     assert rel_err(s2, w2[:, :, 0, :]) < REL
 
 
+@pytest.mark.parametrize("nb,D,F,S", [(50, 20, 4000, 4), (42, 18, 2500, 5), (36, 14, 3333, 3),
+                                      (50, 23, 6002, 2)])
+def test_full_tiles_vs_oracle(ctx, nb, D, F, S):
+    """Tiles with R DOFs on both sides (diagonal ones included) run on the
+    eight-wave kernel for 33 <= nb <= 51; shapes with whole blocks only, a
+    partial last block, and step lengths that are and are not a multiple of
+    the 64-frame chunk."""
+    rng = np.random.default_rng(nb * 7 + D)
+    codes = rng.integers(0, nb, size=(D, F), dtype=np.uint8)
+    codes[1::3] = (codes[0::3][: len(codes[1::3])] + rng.integers(0, 3, size=F)) % nb
+    codes[2::5] = rng.integers(0, 4, size=(len(codes[2::5]), F))      # peaked
+    types = rng.integers(1, 4, size=D).astype(np.uint8)
+    ext = np.zeros((D, 4))
+    ext[:, 0] = rng.uniform(0.5, 1.0, D)
+    ext[:, 1] = ext[:, 0] + rng.uniform(0.2, 1.5, D)
+    iu, ju = np.triu_indices(D, 1)
+    print(f"full tiles nb={nb} D={D} F={F} S={S}", flush=True)
+    s1, s2, mi = ctx.pair_entropy(codes, types, ext, nb, S, ests=(0, 1))
+    w2 = O.pair_entropies(codes, types, ext, np.stack([iu, ju], 1), S, [nb], (0, 1))
+    assert rel_err(s2, w2[:, :, 0, :]) < REL
+    pairs = np.stack([iu, ju], 1)[rng.permutation(len(iu))[:60]]
+    pairs[::2] = pairs[::2, ::-1]
+    s1, s2, mi = ctx.pair_entropy(codes, types, ext, nb, S, pairs=pairs, ests=(1, 0))
+    w2 = O.pair_entropies(codes, types, ext, pairs, S, [nb], (1, 0))
+    assert rel_err(s2, w2[:, :, 0, :]) < REL
+
+
 def test_explicit_pairs_order_and_duplicates(ctx, cfg1):
     nb, S = 30, 4
     codes, ext, types = oracle_discretize(cfg1, nb, nb)
