@@ -1,5 +1,6 @@
 // extern "C" layer: argument checking, device scratch, plans, launches.
 #include <algorithm>
+#include <numeric>
 #include <cstdio>
 #include <cmath>
 #include <cstring>
@@ -319,24 +320,39 @@ int centre_pair_plan_create(centre_ctx* ctx, uint32_t n_dof, int nbins,
         std::sort(uniq.begin(), uniq.end());
         uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
         plan->n_canon = uniq.size();
-        std::vector<int> pm((size_t)n_dof * n_dof, -1);
-        std::vector<uint8_t> tmark((size_t)nblk * nblk, 0);
+        // canonical index = position in the sorted unique list
+        std::vector<uint64_t> r2c(n_pair);
+        for (uint64_t p = 0; p < n_pair; ++p)
+            r2c[p] = (uint64_t)(std::lower_bound(uniq.begin(), uniq.end(), key[p]) - uniq.begin());
+        // tiles holding at least one pair (block pairs in row-major order)
+        std::vector<uint64_t> tkey;
+        tkey.reserve(uniq.size());
+        for (uint64_t k : uniq)
+            tkey.push_back(((uint64_t)((uint32_t)(k >> 32) / R) << 32) | ((uint32_t)k / R));
+        std::sort(tkey.begin(), tkey.end());
+        tkey.erase(std::unique(tkey.begin(), tkey.end()), tkey.end());
+        for (uint64_t k : tkey) tiles.push_back(make_int2((int)(k >> 32), (int)(uint32_t)k));
+        plan->n_full = full_tiles_first(tiles, n_dof, R);
+        // per-tile map of local (row DOF, column DOF) to canonical index, in
+        // the launch order of the tiles
+        std::vector<uint64_t> tord(tiles.size());
+        for (size_t q = 0; q < tiles.size(); ++q)
+            tord[q] = ((uint64_t)tiles[q].x << 32) | (uint32_t)tiles[q].y;
+        std::vector<size_t> tpos(tiles.size());
+        std::iota(tpos.begin(), tpos.end(), (size_t)0);
+        std::sort(tpos.begin(), tpos.end(), [&](size_t x, size_t y) { return tord[x] < tord[y]; });
+        std::vector<int> tmap(tiles.size() * (size_t)R * R, -1);
         for (size_t c = 0; c < uniq.size(); ++c) {
             const uint32_t i = (uint32_t)(uniq[c] >> 32), j = (uint32_t)uniq[c];
-            pm[(size_t)i * n_dof + j] = (int)c;
-            tmark[(size_t)(i / R) * nblk + j / R] = 1;
+            const uint64_t tk = ((uint64_t)(i / R) << 32) | (j / R);
+            const size_t q = tpos[std::lower_bound(tpos.begin(), tpos.end(), tk,
+                                                   [&](size_t x, uint64_t v) { return tord[x] < v; }) -
+                                  tpos.begin()];
+            tmap[q * R * R + (i % R) * R + (j % R)] = (int)c;
         }
-        std::vector<uint64_t> r2c(n_pair);
-        for (uint64_t p = 0; p < n_pair; ++p) {
-            const uint32_t i = (uint32_t)(key[p] >> 32), j = (uint32_t)key[p];
-            r2c[p] = (uint64_t)pm[(size_t)i * n_dof + j];
-        }
-        for (int I = 0; I < nblk; ++I)
-            for (int J = I; J < nblk; ++J)
-                if (tmark[(size_t)I * nblk + J]) tiles.push_back(make_int2(I, J));
         hipError_t e;
-        if ((e = hipMalloc(&plan->d_pairmap, sizeof(int) * pm.size())) != hipSuccess ||
-            (e = hipMemcpy(plan->d_pairmap, pm.data(), sizeof(int) * pm.size(),
+        if ((e = hipMalloc(&plan->d_tilemap, sizeof(int) * tmap.size())) != hipSuccess ||
+            (e = hipMemcpy(plan->d_tilemap, tmap.data(), sizeof(int) * tmap.size(),
                            hipMemcpyHostToDevice)) != hipSuccess ||
             (e = hipMalloc(&plan->d_rec2canon, sizeof(uint64_t) * (n_pair ? n_pair : 1))) != hipSuccess ||
             (n_pair && (e = hipMemcpy(plan->d_rec2canon, r2c.data(), sizeof(uint64_t) * n_pair,
@@ -348,7 +364,7 @@ int centre_pair_plan_create(centre_ctx* ctx, uint32_t n_dof, int nbins,
             return fail(ctx, (int)e, std::string("plan upload: ") + hipGetErrorString(e));
         }
     }
-    plan->n_full = full_tiles_first(tiles, n_dof, R);
+    if (!pairs) plan->n_full = full_tiles_first(tiles, n_dof, R);
     plan->n_tiles = (int)tiles.size();
     hipError_t e = hipMalloc(&plan->d_tiles, sizeof(int2) * (tiles.size() ? tiles.size() : 1));
     if (e == hipSuccess && !tiles.empty())
@@ -364,7 +380,7 @@ int centre_pair_plan_create(centre_ctx* ctx, uint32_t n_dof, int nbins,
 void centre_pair_plan_destroy(centre_pair_plan* plan) {
     if (!plan) return;
     if (plan->d_tiles) (void)hipFree(plan->d_tiles);
-    if (plan->d_pairmap) (void)hipFree(plan->d_pairmap);
+    if (plan->d_tilemap) (void)hipFree(plan->d_tilemap);
     if (plan->d_rec2canon) (void)hipFree(plan->d_rec2canon);
     if (plan->d_rec_ij) (void)hipFree(plan->d_rec_ij);
     delete plan;
@@ -467,9 +483,9 @@ struct BatchRun {
     std::vector<double> sys_ms;
 };
 
-// Persistent launches over a prepared tile list: tiles of a system's last,
-// partial block first (four-wave tile), then the full tiles (eight-wave
-// tile), each list in the given order.  Every system's completion stamp is
+// Launches over a prepared tile list: tiles of a system's last, partial
+// block first (persistent four-wave tile), then the full tiles (eight-wave
+// tile, one workgroup per tile in list order), each list in the given order.  Every system's completion stamp is
 // taken by whichever launch finishes its last tile.
 int batch_launch(centre_ctx* ctx, const PairKernelArgs& base, const std::vector<BatchSystem>& bs,
                  const std::vector<BatchTile>& tl, int grid, BatchRun* run) {
@@ -522,7 +538,7 @@ int batch_launch(centre_ctx* ctx, const PairKernelArgs& base, const std::vector<
     if (n_part > 0) f.t0 = nullptr;
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
     HIPCHK(launch_pair_persistent(a, grid, ctx->stream));
-    HIPCHK(launch_pair_persistent8(f, grid, ctx->stream));
+    HIPCHK(launch_pair_batch8(f, ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
     std::vector<unsigned long long> stamps(n_sys + 1);
     HIPCHK(hipMemcpyAsync(stamps.data(), dtime.p, sizeof(unsigned long long) * (n_sys + 1),
@@ -718,7 +734,7 @@ extern "C" int centre_dev_pair_entropy(centre_ctx* ctx, const centre_pair_plan* 
     a.nsteps = nsteps;
     a.ss = ss;
     a.tiles = plan->d_tiles;
-    a.pairmap = plan->d_pairmap;
+    a.tilemap = plan->d_tilemap;
     a.dof_w = w.as<double>();
     a.dof_L = L.as<double>();
     a.dof_S1 = s1.as<double>();
@@ -749,6 +765,7 @@ extern "C" int centre_dev_pair_entropy(centre_ctx* ctx, const centre_pair_plan* 
         HIPCHK(launch_pair_tiles8(a, plan->n_full, ctx->stream));
         PairKernelArgs r = a;
         r.tiles = plan->d_tiles + plan->n_full;
+        if (r.tilemap) r.tilemap += (size_t)plan->n_full * a.R * a.R;
         HIPCHK(launch_pair_tiles(r, plan->n_tiles - plan->n_full, false, ctx->stream));
     } else {
         HIPCHK(launch_pair_tiles(a, plan->n_tiles, dump, ctx->stream));
@@ -1023,16 +1040,6 @@ extern "C" int centre_dev_mst(centre_ctx* ctx, const uint32_t* d_src, const uint
     *n_tree = 0;
     if (n_node == 0) return 0;
     HIPCHK(hipSetDevice(ctx->device));
-    // endpoint range check on the host side of the boundary
-    {
-        std::vector<uint32_t> hs(n_edge), hd(n_edge);
-        if (n_edge) {
-            HIPCHK(hipMemcpy(hs.data(), d_src, sizeof(uint32_t) * n_edge, hipMemcpyDeviceToHost));
-            HIPCHK(hipMemcpy(hd.data(), d_dst, sizeof(uint32_t) * n_edge, hipMemcpyDeviceToHost));
-        }
-        for (uint64_t e = 0; e < n_edge; ++e)
-            if (hs[e] >= n_node || hd[e] >= n_node) return fail(ctx, -3, "edge endpoint out of range");
-    }
     DevBuf comp, bw, be, it, ch;
     HIPCHK(comp.alloc(sizeof(uint32_t) * n_node));
     HIPCHK(bw.alloc(sizeof(unsigned long long) * n_node));
@@ -1050,6 +1057,15 @@ extern "C" int centre_dev_mst(centre_ctx* ctx, const uint32_t* d_src, const uint
     a.best_e = be.as<unsigned int>();
     a.in_tree = it.as<uint8_t>();
     a.changed = ch.as<int>();
+    {
+        // endpoint range check before any kernel indexes by an endpoint
+        int bad = 0;
+        HIPCHK(hipMemsetAsync(a.changed, 0, sizeof(int), ctx->stream));
+        HIPCHK(mst_check(a, ctx->stream));
+        HIPCHK(hipMemcpyAsync(&bad, a.changed, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (bad) return fail(ctx, -3, "edge endpoint out of range");
+    }
     HIPCHK(mst_init(a, ctx->stream));
     for (uint32_t round = 0; round < 64; ++round) {
         HIPCHK(run_mst_round(a, ctx->stream));
@@ -1065,9 +1081,21 @@ extern "C" int centre_dev_mst(centre_ctx* ctx, const uint32_t* d_src, const uint
     for (uint64_t e = 0; e < n_edge; ++e)
         if (flags[e]) idx.push_back((uint32_t)e);
     if (idx.size() >= n_node) return fail(ctx, -3, "internal: forest larger than n_node - 1");
+    // weights of the tree edges, gathered on the device
     std::vector<double> wt(idx.size());
-    for (size_t k = 0; k < idx.size(); ++k)
-        HIPCHK(hipMemcpy(&wt[k], d_w + idx[k], sizeof(double), hipMemcpyDeviceToHost));
+    if (!idx.empty()) {
+        std::vector<uint64_t> idx64(idx.begin(), idx.end());
+        DevBuf di, dw;
+        HIPCHK(di.alloc(sizeof(uint64_t) * idx64.size()));
+        HIPCHK(dw.alloc(sizeof(double) * idx64.size()));
+        HIPCHK(hipMemcpyAsync(di.p, idx64.data(), sizeof(uint64_t) * idx64.size(),
+                              hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(launch_gather_rec(d_w, di.as<uint64_t>(), idx64.size(), 1, dw.as<double>(),
+                                 ctx->stream));
+        HIPCHK(hipMemcpyAsync(wt.data(), dw.p, sizeof(double) * wt.size(), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
     std::vector<size_t> ord(idx.size());
     for (size_t k = 0; k < ord.size(); ++k) ord[k] = k;
     std::sort(ord.begin(), ord.end(), [&](size_t x, size_t y) {

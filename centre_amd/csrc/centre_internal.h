@@ -23,7 +23,7 @@ struct centre_pair_plan {
     int dofs_per_tile = 0;          // R: DOFs per tile side
     int n_tiles = 0;
     int2* d_tiles = nullptr;        // (row block, col block)
-    int* d_pairmap = nullptr;       // [D][D] canonical index or -1; null = all i<j
+    int* d_tilemap = nullptr;       // [tile][R*R] canonical index or -1; null = all i<j
     uint64_t n_canon = 0;           // canonical (i<j) pairs computed
     uint64_t n_rec = 0;             // records requested by the caller
     uint64_t* d_rec2canon = nullptr;  // [n_rec] (explicit lists only)
@@ -72,7 +72,8 @@ struct PairKernelArgs {
     uint64_t ss;            // frames per convergence step
     uint64_t sstride;       // code-row offset between step starts (>= ss)
     const int2* tiles;
-    const int* pairmap;     // null => all i<j, canonical index by triangle
+    const int* tilemap;     // [tile][R*R] canonical pair index or -1 of local (row, col)
+                            // DOFs; null => all i<j, canonical index by triangle
     const double* dof_w;    // [D] bin width (1.0 under kde)
     const double* dof_L;    // [D][S] sum_b m_b(s) log J_b (0 if no Jacobian)
     const double* dof_S1;   // [D][S] S1 of the MI estimator, or null
@@ -102,8 +103,9 @@ struct PairKernelArgs {
 // the c log c table.
 bool pair_tile8_supported(int R);
 hipError_t launch_pair_tiles8(const PairKernelArgs& a, int n_tiles, hipStream_t st);
-// Persistent form for batches (full tiles only, step-aligned codes).
-hipError_t launch_pair_persistent8(const PairKernelArgs& a, int grid, hipStream_t st);
+// Batch form (full tiles only, step-aligned codes): one workgroup per entry
+// of a.tiles[0 .. a.n_work), dispatched in list order.
+hipError_t launch_pair_batch8(const PairKernelArgs& a, hipStream_t st);
 hipError_t launch_pair_tiles(const PairKernelArgs& a, int n_tiles, bool dump,
                              hipStream_t st);
 size_t pair_tile_lds_bytes(int R);
@@ -210,6 +212,7 @@ struct MstArgs {
 };
 hipError_t run_mst_round(const MstArgs& a, hipStream_t st);
 hipError_t mst_init(const MstArgs& a, hipStream_t st);
+hipError_t mst_check(const MstArgs& a, hipStream_t st);   // sets *changed on a bad endpoint
 
 // ---- reductions ------------------------------------------------------------------
 hipError_t launch_segment_sum(const double* x, const uint64_t* d_beg, int n_seg,

@@ -106,6 +106,18 @@ __global__ __launch_bounds__(256) void segsum_kernel(const double* x, const uint
 
 }  // namespace
 
+// *a.changed |= 1 when an edge endpoint is not below n_node
+__global__ void mst_check_kernel(MstArgs a) {
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < a.n_edge && (a.src[e] >= a.n_node || a.dst[e] >= a.n_node)) atomicOr(a.changed, 1);
+}
+
+hipError_t mst_check(const MstArgs& a, hipStream_t st) {
+    if (!a.n_edge) return hipSuccess;
+    hipLaunchKernelGGL(mst_check_kernel, dim3((unsigned)((a.n_edge + 255) / 256)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
 hipError_t mst_init(const MstArgs& a, hipStream_t st) {
     const uint64_t n = a.n_edge > a.n_node ? a.n_edge : a.n_node;
     hipLaunchKernelGGL(mst_init_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);

@@ -116,15 +116,15 @@ __device__ inline long long tri_index(long long i, long long j, long long D) {
 }
 
 // One tile: all DOF pairs of row block tile.x x column block tile.y of
-// tile list entry t, over every frame and step.  With a batch (a_in.systems)
-// the entry belongs to system a_in.tile_sys[t], whose codes, sizes and
+// tile list entry tix, over every frame and step.  With a batch (a_in.systems)
+// the entry belongs to system a_in.tile_sys[tix], whose codes, sizes and
 // outputs replace those of a_in.
 template <bool kDump, int kSlots, bool kBatch>
-__device__ __forceinline__ void tile_body(const PairKernelArgs& a_in, int t) {
+__device__ __forceinline__ void tile_body(const PairKernelArgs& a_in, int tix) {
     extern __shared__ __align__(16) uint8_t smem_raw[];
     PairKernelArgs a = a_in;
     if constexpr (kBatch) {
-        const BatchSystem& sd = a_in.systems[a_in.tile_sys[t]];
+        const BatchSystem& sd = a_in.systems[a_in.tile_sys[tix]];
         a.codes = sd.codes;
         a.stride = sd.stride;
         a.n_dof = sd.n_dof;
@@ -154,7 +154,7 @@ __device__ __forceinline__ void tile_body(const PairKernelArgs& a_in, int t) {
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform
     const int wr = wave >> 1, wc = wave & 1;
-    const int2 tile = a.tiles[t];
+    const int2 tile = a.tiles[tix];
     const int r0 = tile.x * R, c0 = tile.y * R;
     const int nr = min(R, a.n_dof - r0), nc = min(R, a.n_dof - c0);
     const bool diag = tile.x == tile.y;
@@ -421,7 +421,7 @@ __device__ __forceinline__ void tile_body(const PairKernelArgs& a_in, int t) {
                         if (rd != 0xFF) {
                             const long long gi = r0 + rd, gj = c0 + cd;
                             if (gi < gj) {
-                                const long long p = a.pairmap ? (long long)a.pairmap[gi * D + gj]
+                                const long long p = a.tilemap ? (long long)a.tilemap[(size_t)tix * R * R + rd * R + cd]
                                                               : tri_index(gi, gj, D);
                                 if (p >= 0) {
                                     const int bi_ = grow - rd * nb, bj_ = gcol - cd * nb;
@@ -449,7 +449,7 @@ __device__ __forceinline__ void tile_body(const PairKernelArgs& a_in, int t) {
             const int rd = q / nc, cd = q - rd * nc;
             const long long gi = r0 + rd, gj = c0 + cd;
             if (gi < gj) {
-                const long long p = a.pairmap ? (long long)a.pairmap[gi * D + gj]
+                const long long p = a.tilemap ? (long long)a.tilemap[(size_t)tix * R * R + rd * R + cd]
                                               : tri_index(gi, gj, D);
                 if (p >= 0) {
                     double T = 0.0;
@@ -831,22 +831,16 @@ __device__ __forceinline__ void epi_block8(const v16i& blk, int row0, int gcol, 
 }
 
 template <bool kBatch>
-__device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int t) {
+__device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) {
     extern __shared__ __align__(16) uint8_t smem_raw[];
-    PairKernelArgs a = a_in;
-    if constexpr (kBatch) {
-        const BatchSystem& sd = a_in.systems[a_in.tile_sys[t]];
-        a.codes = sd.codes;
-        a.stride = sd.stride;
-        a.n_dof = sd.n_dof;
-        a.ss = sd.ss;
-        a.sstride = sd.sstride;
-        a.s2 = sd.s2;
-        a.mi = sd.mi;
-        a.dof_w = sd.dof_w;
-        a.dof_L = sd.dof_L;
-        a.dof_S1 = sd.dof_S1;
-    }
+    const PairKernelArgs& a = a_in;
+    // a batch entry takes codes, sizes and outputs from its system; the
+    // output pointers are read at each step end rather than kept live
+    const BatchSystem* const sdp = kBatch ? &a.systems[a.tile_sys[tix]] : nullptr;
+    const uint8_t* const codes = kBatch ? sdp->codes : a.codes;
+    const uint64_t cstride = kBatch ? sdp->stride : a.stride;
+    const uint64_t ss = kBatch ? sdp->ss : a.ss;
+    const uint64_t sstride = kBatch ? sdp->sstride : a.sstride;
     const int R = a.R, nb = a.nb, S = a.nsteps;
     const LdsLayout lay = lds_layout(R, 8);
     const int nstr = lay.nstripe, ntab = lay.ntab;
@@ -865,11 +859,11 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int t) {
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wr = wave >> 2, wc = wave & 3;
-    const int2 tile = a.tiles[t];
+    const int2 tile = a.tiles[tix];
     const int r0 = tile.x * R, c0 = tile.y * R;
     const bool diag = tile.x == tile.y;
     const size_t boff = diag ? 0 : kOpBytes;   // column operand
-    const long long D = a.n_dof;
+    const long long D = kBatch ? sdp->n_dof : a.n_dof;
 
     for (int i = tid; i < 2 * 2 * kOpBytes / 16; i += 512)
         reinterpret_cast<uint4*>(sm.oh)[i] = make_uint4(0, 0, 0, 0);
@@ -892,7 +886,7 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int t) {
     // results are discarded
     if (*reinterpret_cast<const int*>(trash) != 0) return;
 
-    const int nch = (int)((a.ss + kChunk - 1) / kChunk);
+    const int nch = (int)((ss + kChunk - 1) / kChunk);
     const int G = S * nch;
     const int nlive = (diag ? 1 : 2) * R * kGroups;
     const bool stager = wave * 64 < nlive;   // wave-uniform
@@ -900,7 +894,7 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int t) {
     const int q = (wave * 64 + lane) % nlive;
     const int ld = q / kGroups, k4 = (q - ld * kGroups) * 4;
     const bool isA = ld < R;
-    const uint8_t* const src = a.codes + (uint64_t)(isA ? r0 + ld : c0 + (ld - R)) * a.stride + k4;
+    const uint8_t* const src = codes + (uint64_t)(isA ? r0 + ld : c0 + (ld - R)) * cstride + k4;
     const uint32_t dst = (uint32_t)((isA ? 0 : kOpBytes) + (isA ? ld : ld - R) * nb * kLdsRow + k4);
     uint32_t cur[kSuper], nxt[kSuper], last0 = 0u, last1 = 0u;
     int ls = 0, lc = 0;
@@ -908,14 +902,14 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int t) {
         if (++c == n) { c = 0; ++s; }
     };
     auto nvalid_of = [&](int s, int c) __attribute__((always_inline)) -> int {
-        return s < S ? (int)min((uint64_t)kChunk, a.ss - (uint64_t)c * kChunk) : 0;
+        return s < S ? (int)min((uint64_t)kChunk, ss - (uint64_t)c * kChunk) : 0;
     };
     auto load4 = [&](uint32_t (&d)[kSuper]) __attribute__((always_inline)) {
 #pragma unroll
         for (int c = 0; c < kSuper; ++c) {
-            const uint64_t f0 = (uint64_t)ls * a.sstride + (uint64_t)lc * kChunk;
+            const uint64_t f0 = (uint64_t)ls * sstride + (uint64_t)lc * kChunk;
             const int nv = nvalid_of(ls, lc);
-            d[c] = *reinterpret_cast<const uint32_t*>(k4 < nv ? src + f0 : a.codes);
+            d[c] = *reinterpret_cast<const uint32_t*>(k4 < nv ? src + f0 : codes);
             advance(ls, lc, nch);
         }
     };
@@ -978,13 +972,19 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int t) {
                 __builtin_amdgcn_sched_barrier(0);
             }
         __syncthreads();
-        const double n = (double)(ms + 1) * (double)a.ss;
+        const double n = (double)(ms + 1) * (double)ss;
         const double logn = log(n);
+        asm volatile("" ::: "memory");   // read the output pointers here
+        double* const s2 = kBatch ? sdp->s2 : a.s2;
+        double* const mi = kBatch ? sdp->mi : a.mi;
+        const double* const dof_w = kBatch ? sdp->dof_w : a.dof_w;
+        const double* const dof_L = kBatch ? sdp->dof_L : a.dof_L;
+        const double* const dof_S1 = kBatch ? sdp->dof_S1 : a.dof_S1;
         for (int pq = tid; pq < R * R; pq += 512) {
             const int rd = pq / R, cd = pq - rd * R;
             const long long gi = r0 + rd, gj = c0 + cd;
             const long long p = gi >= gj ? -1
-                              : (a.pairmap ? (long long)a.pairmap[gi * D + gj] : tri_index(gi, gj, D));
+                              : (a.tilemap ? (long long)a.tilemap[(size_t)tix * R * R + pq] : tri_index(gi, gj, D));
             double T = 0.0;
             int nz = 0;
             for (int k = 0; k < nstr; ++k) {
@@ -994,15 +994,15 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int t) {
                 sm.slotN[pq * nstr + k] = 0;
             }
             if (p >= 0) {
-                const double L = a.dof_L[gi * S + ms] + a.dof_L[gj * S + ms];
-                const double area = a.dof_w[gi] * a.dof_w[gj];
+                const double L = dof_L[gi * S + ms] + dof_L[gj * S + ms];
+                const double area = dof_w[gi] * dof_w[gj];
                 const double ml = (logn + log(area)) - T / n + L / n;
                 const double mm = ml + (double)(unsigned)(nz - 1) / (2.0 * n);
-                if (a.slot_ml >= 0) a.s2[((size_t)p * a.nest + a.slot_ml) * S + ms] = ml;
-                if (a.slot_mm >= 0) a.s2[((size_t)p * a.nest + a.slot_mm) * S + ms] = mm;
-                if (a.mi) {
-                    const double s1 = a.dof_S1[gi * S + ms] + a.dof_S1[gj * S + ms];
-                    a.mi[(size_t)p * S + ms] = s1 - (a.mi_from_mm ? mm : ml);
+                if (a.slot_ml >= 0) s2[((size_t)p * a.nest + a.slot_ml) * S + ms] = ml;
+                if (a.slot_mm >= 0) s2[((size_t)p * a.nest + a.slot_mm) * S + ms] = mm;
+                if (mi) {
+                    const double s1 = dof_S1[gi * S + ms] + dof_S1[gj * S + ms];
+                    mi[(size_t)p * S + ms] = s1 - (a.mi_from_mm ? mm : ml);
                 }
             }
         }
@@ -1024,10 +1024,11 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int t) {
         load4(cur);
         load4(nxt);
     }
-    // one barrier per chunk: chunk g multiplies from buffer g&1 while the
-    // stager waves write chunk g+1 into the other buffer.  (A variant that
-    // reads chunk g+1's fragments during chunk g's MFMAs needs two sets of
-    // fragment registers next to the 128 accumulators and spills.)
+    // One barrier per chunk: chunk g multiplies from buffer g&1 while the
+    // stager waves write chunk g+1 into the other buffer.  (Reading chunk
+    // g+1's fragments during chunk g's MFMAs needs a second set of fragment
+    // registers next to the 128 accumulators; with two waves per SIMD that
+    // spills and runs 8x slower.)
     stage_next(0);
     __syncthreads();
     for (int g = 0; g < G; ++g) {
@@ -1049,23 +1050,20 @@ __global__ __launch_bounds__(512) void pair_tile8_kernel(PairKernelArgs a) {
     tile8_body<false>(a, blockIdx.x);
 }
 
-// Persistent batch form of the eight-wave tile (see pair_persist_kernel).
-__global__ __launch_bounds__(512) void pair_persist8_kernel(PairKernelArgs a) {
-    __shared__ int s_t;
-    if (blockIdx.x == 0 && threadIdx.x == 0 && a.t0) *a.t0 = wall_clock64();
-    for (;;) {
-        __syncthreads();   // the previous tile is done with LDS and s_t
-        if (threadIdx.x == 0) s_t = atomicAdd(a.work, 1);
-        __syncthreads();
-        const int t = s_t;
-        if (t >= a.n_work) break;
-        tile8_body<true>(a, t);
-        if (threadIdx.x == 0 && a.sys_count) {
-            const int sys = a.tile_sys[t];
-            __threadfence();
-            if (atomicAdd(&a.sys_count[sys], 1) == a.sys_tiles[sys] - 1)
-                a.sys_time[sys] = wall_clock64();
-        }
+// Batch form of the eight-wave tile: one workgroup per entry of the ordered
+// batch tile list.  Workgroups are dispatched in list order as CUs free up,
+// which is the work queue of the persistent form; a loop around the tile
+// body would push it past the two-waves-per-SIMD register budget.  The
+// workgroup finishing a system's last tile stamps the system's completion.
+__global__ __launch_bounds__(512) void pair_batch8_kernel(PairKernelArgs a) {
+    const int t = blockIdx.x;
+    if (t == 0 && threadIdx.x == 0 && a.t0) *a.t0 = wall_clock64();
+    tile8_body<true>(a, t);
+    if (threadIdx.x == 0 && a.sys_count) {
+        const int sys = a.tile_sys[t];
+        __threadfence();
+        if (atomicAdd(&a.sys_count[sys], 1) == a.sys_tiles[sys] - 1)
+            a.sys_time[sys] = wall_clock64();
     }
 }
 
@@ -1150,14 +1148,14 @@ hipError_t launch_pair_tiles8(const PairKernelArgs& a, int n_tiles, hipStream_t 
     return hipGetLastError();
 }
 
-hipError_t launch_pair_persistent8(const PairKernelArgs& a, int grid, hipStream_t st) {
-    if (a.n_work <= 0 || grid <= 0) return hipSuccess;
+hipError_t launch_pair_batch8(const PairKernelArgs& a, hipStream_t st) {
+    if (a.n_work <= 0) return hipSuccess;
     if (!pair_tile8_supported(a.R)) return hipErrorInvalidValue;
     const size_t lds = lds_layout(a.R, 8).total;
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&pair_persist8_kernel),
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&pair_batch8_kernel),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(pair_persist8_kernel, dim3(grid), dim3(512), lds, st, a);
+    hipLaunchKernelGGL(pair_batch8_kernel, dim3(a.n_work), dim3(512), lds, st, a);
     return hipGetLastError();
 }
 

@@ -276,6 +276,17 @@ def test_mst_vs_oracle(ctx, seed):
     assert got.tolist() == kruskal_set(n, src.tolist(), dst.tolist(), w.tolist())
 
 
+def test_mst_endpoint_out_of_range_is_an_error(ctx):
+    """An endpoint >= n_node is refused (-3) before any round runs; the
+    context stays usable."""
+    src = np.array([0, 1, 2], np.uint32)
+    dst = np.array([1, 2, 5], np.uint32)
+    with pytest.raises(hip.CentreError) as e:
+        ctx.mst(5, src, dst, np.ones(3))
+    assert e.value.code == -3
+    assert sorted(ctx.mst(6, src, dst, np.ones(3)).tolist()) == [0, 1, 2]
+
+
 def test_mst_example2_trees(ctx, golden):
     from test_oracle_scorer import score
     for step in range(4):
