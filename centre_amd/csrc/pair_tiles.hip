@@ -85,8 +85,8 @@ __host__ __device__ inline LdsLayout lds_layout(int R, int nwave = 4) {
     L.N = off;
     off += sizeof(int) * R * R * L.nstripe;
     off = (off + 15) & ~(size_t)15;
-    L.log = off;
-    off += 2 * 256 * sizeof(double);
+    L.log = off;   // log(1 + j/256), 1/(1 + j/256), then 8 series coefficients
+    off += (2 * 256 + 8) * sizeof(double);
     // epilogue parking area of the fast paths (R <= 32): 4 KB per wave
     L.park = off;
     if (R <= 32) off += (size_t)nwave * 64 * 16 * sizeof(int);
@@ -109,6 +109,21 @@ __device__ inline double log_count(int c, const double* logt, const double* invt
     const double r = (f - j * (1.0 / 256.0)) * invt[j];
     const double p = r * (1.0 + r * (-0.5 + r * (1.0 / 3.0 + r * (-0.25 + r * (0.2 + r * (-1.0 / 6.0))))));
     return (double)(e - 1) * 0.6931471805599453 + logt[j] + p;
+}
+
+// The same for any positive finite x, with the series coefficients and log 2
+// read from LDS (coef = {1, -1/2, 1/3, -1/4, 1/5, -1/6, log 2}): as
+// literals they would be materialised in registers for the whole kernel.
+__device__ inline double log_pos_lds(double x, const double* logt, const double* invt,
+                                     const double* coef) {
+    int e;
+    const double m2 = 2.0 * frexp(x, &e);
+    const double f = m2 - 1.0;
+    const int j = (int)(f * 256.0);
+    const double r = (f - j * (1.0 / 256.0)) * invt[j];
+    const double p =
+        r * (coef[0] + r * (coef[1] + r * (coef[2] + r * (coef[3] + r * (coef[4] + r * coef[5])))));
+    return (double)(e - 1) * coef[6] + logt[j] + p;
 }
 
 __device__ inline long long tri_index(long long i, long long j, long long D) {
@@ -770,7 +785,7 @@ __device__ __forceinline__ void epi_block8(const v16i& blk, int row0, int gcol, 
     pk[192] = v4i{blk[12], blk[13], blk[14], blk[15]};
     asm volatile("" ::: "memory");
     const int stripe = lane % nstr;
-    if (nb >= 28) {
+    {   // nb >= 28: the lane's 16 rows touch at most two DOFs
         const int dlo = sm.row2dof[row0];
         const bool ok = (cd != 0xFF) & (dlo != 0xFF);
         const int kk = (dlo + 1) * nb - row0;
@@ -791,7 +806,7 @@ __device__ __forceinline__ void epi_block8(const v16i& blk, int row0, int gcol, 
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 if (__builtin_amdgcn_ballot_w64(c[t] >= ntab) != 0 && c[t] >= ntab)
-                    tv[t] = (double)c[t] * log_count(c[t], sm.logt, sm.invt);
+                    tv[t] = (double)c[t] * log_pos_lds((double)c[t], sm.logt, sm.invt, sm.invt + 256);
                 const bool h = 8 * q + t >= kk;
                 const int nz = min(c[t], 1);
                 tot += tv[t];
@@ -809,23 +824,6 @@ __device__ __forceinline__ void epi_block8(const v16i& blk, int row0, int gcol, 
         if (nhi && dhi != 0xFF && dhi != dlo) {
             atomicAdd(&sm.slotT[(dhi * R + cd) * nstr + stripe], hi);
             atomicAdd(&sm.slotN[(dhi * R + cd) * nstr + stripe], nhi);
-        }
-        return;
-    }
-    if (cd == 0xFF) return;
-#pragma unroll 1
-    for (int q = 0; q < 4; ++q) {
-        const v4i c4 = pk[64 * q];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int rd = sm.row2dof[row0 + 8 * q + t];
-            const int cnt = c4[t];
-            if (rd != 0xFF && cnt > 0) {
-                const double v = cnt < ntab ? sm.ctab[cnt]
-                                            : (double)cnt * log_count(cnt, sm.logt, sm.invt);
-                atomicAdd(&sm.slotT[(rd * R + cd) * nstr + stripe], v);
-                atomicAdd(&sm.slotN[(rd * R + cd) * nstr + stripe], 1);
-            }
         }
     }
 }
@@ -874,6 +872,11 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
         const double t = 1.0 + tid * (1.0 / 256.0);
         sm.logt[tid] = log(t);
         sm.invt[tid] = 1.0 / t;
+        if (tid < 8) {
+            const double cf[8] = {1.0, -0.5, 1.0 / 3.0, -0.25, 0.2, -1.0 / 6.0,
+                                  0.6931471805599453, 0.0};
+            sm.invt[256 + tid] = cf[tid];
+        }
     }
     for (int i = tid; i < R * R * nstr; i += 512) {
         sm.slotT[i] = 0.0;
@@ -972,8 +975,9 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
                 __builtin_amdgcn_sched_barrier(0);
             }
         __syncthreads();
+        const double* const coef = sm.invt + 256;
         const double n = (double)(ms + 1) * (double)ss;
-        const double logn = log(n);
+        const double logn = log_pos_lds(n, sm.logt, sm.invt, coef);
         asm volatile("" ::: "memory");   // read the output pointers here
         double* const s2 = kBatch ? sdp->s2 : a.s2;
         double* const mi = kBatch ? sdp->mi : a.mi;
@@ -996,7 +1000,7 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
             if (p >= 0) {
                 const double L = dof_L[gi * S + ms] + dof_L[gj * S + ms];
                 const double area = dof_w[gi] * dof_w[gj];
-                const double ml = (logn + log(area)) - T / n + L / n;
+                const double ml = (logn + log_pos_lds(area, sm.logt, sm.invt, coef)) - T / n + L / n;
                 const double mm = ml + (double)(unsigned)(nz - 1) / (2.0 * n);
                 if (a.slot_ml >= 0) s2[((size_t)p * a.nest + a.slot_ml) * S + ms] = ml;
                 if (a.slot_mm >= 0) s2[((size_t)p * a.nest + a.slot_mm) * S + ms] = mm;
@@ -1029,6 +1033,11 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     // g+1's fragments during chunk g's MFMAs needs a second set of fragment
     // registers next to the 128 accumulators; with two waves per SIMD that
     // spills and runs 8x slower.)
+    // One barrier per chunk: chunk g multiplies from buffer g&1 while the
+    // stager waves write chunk g+1 into the other buffer.  Reading chunk
+    // g+1's fragments during chunk g's MFMAs (half a chunk ahead) needs 24
+    // more registers than the two-waves-per-SIMD budget leaves next to the
+    // 128 accumulators: that variant spills and ran 2.5x slower.
     stage_next(0);
     __syncthreads();
     for (int g = 0; g < G; ++g) {
@@ -1133,8 +1142,9 @@ hipError_t launch_pair_persistent(const PairKernelArgs& a, int grid, hipStream_t
 }
 
 bool pair_tile8_supported(int R) {
+    // R <= 8 means nb >= 29: a lane's 16 rows of a block touch at most two DOFs
     const LdsLayout L = lds_layout(R, 8);
-    return R <= 32 && 2 * R * kGroups <= 512 && L.ntab > 0 && L.total <= kLdsCap;
+    return R <= 8 && 2 * R * kGroups <= 512 && L.ntab > 0 && L.total <= kLdsCap;
 }
 
 hipError_t launch_pair_tiles8(const PairKernelArgs& a, int n_tiles, hipStream_t st) {
