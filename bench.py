@@ -38,12 +38,15 @@ def parse():
     ap.add_argument("--cpu-pairs", type=int, default=0,
                     help="pairs in the CPU-baseline sample (0 = all pairs)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="pairs", choices=["pairs", "screen", "batch", "vmkde"],
+    ap.add_argument("--workload", default="pairs",
+                    choices=["pairs", "screen", "batch", "vmkde", "host"],
                     help="pairs: the headline full pairwise pair stage (default); "
                          "screen: learned pair screen vs the plain path at 6,000 DOF; "
                          "batch: 64 uneven systems in one persistent launch; "
                          "vmkde: von Mises KDE discretisation of 3,000 torsions x 50k "
-                         "frames, then A-MIE / A-MIST over index neighbours")
+                         "frames, then A-MIE / A-MIST over index neighbours; "
+                         "host: the C++ host end to end with file I/O on the headline "
+                         "workload (BAT file in, bin/entcontri/CSV files out)")
     ap.add_argument("--batch-systems", type=int, default=64, help="systems (batch)")
     ap.add_argument("--batch-dof", type=int, nargs=2, default=[200, 2000],
                     help="DOF range of the batch systems (default 200 2000)")
@@ -88,6 +91,8 @@ def main():
         return batch_workload(args, hip)
     if args.workload == "vmkde":
         return vmkde_workload(args, hip)
+    if args.workload == "host":
+        return host_workload(args, hip)
     nb, S, F = args.nbins, args.nsteps, args.frames
     nbd, nan, ndh = args.n_bond, args.n_angle, args.n_dih
     D = nbd + nan + ndh
@@ -604,6 +609,91 @@ def vmkde_workload(args, hip):
             "frames_differing_with_oracle_grid": frames_diff, "frames_checked": sample * F,
             "s2_max_rel_err": s2_err, "pairs_checked": ps}
         out["speedup_end_to_end_vs_cpu_1core"] = cpu_s / (td + tp + tm)
+    print(json.dumps(out), flush=True)
+
+
+def host_workload(args, hip):
+    """The C++ host (centre_amd/centre_hip) end to end on the headline
+    workload: a CDF-5 BAT file of 600 DOF x 100,000 frames is written, then
+    one run discretises it, computes every 1-D and 2-D contribution (all
+    179,700 pairs, 10 steps) and the MIE / MIST tables, reading and writing
+    files as centre_OMP does (one JSON line)."""
+    import shutil
+    import subprocess
+    import tempfile
+    from centre_amd import cdf
+    host = os.path.join(os.path.dirname(os.path.abspath(__file__)), "centre_amd", "centre_hip")
+    if not os.path.exists(host):
+        raise SystemExit("centre_amd/centre_hip is not built (run __graft_entry__.build())")
+    nbd, nan, ndh, F, nb, S = args.n_bond, args.n_angle, args.n_dih, args.frames, args.nbins, args.nsteps
+    D = nbd + nan + ndh
+    P = D * (D - 1) // 2
+    work = tempfile.mkdtemp(prefix="centre_host_")
+    try:
+        bat = hip.synth_bat(0xCE27E + 1, nbd, nan, ndh, F)
+        t0 = time.perf_counter()
+        cdf.write_dataset(
+            os.path.join(work, "bat.nc"),
+            [("frame", F), ("bond", nbd), ("angle", nan), ("dihedral", ndh)],
+            [cdf.VarSpec("bond", ("frame", "bond"), np.ascontiguousarray(bat[:, :nbd]),
+                         {"units": "angstrom"}),
+             cdf.VarSpec("angle", ("frame", "angle"), np.ascontiguousarray(bat[:, nbd:nbd + nan]),
+                         {"units": "radian"}),
+             cdf.VarSpec("dihedral", ("frame", "dihedral"), np.ascontiguousarray(bat[:, nbd + nan:]),
+                         {"units": "radian"})],
+            attrs={"convention": "CENTRE"}, version=5, unlimited="frame")
+        t_write = time.perf_counter() - t0
+        del bat
+        os.makedirs(os.path.join(work, "out"))
+        cfg = os.path.join(work, "centre.cfg")
+        with open(cfg, "w") as f:
+            f.write(f"""[control]
+    infilepath           = {work}/
+    outfilepath          = {work}/out/
+    infofile             = progress.info
+    nsteps               = {S}
+    discretize           = true
+    calcentropy          = true
+    genconvgdata         = true
+[discretization]
+    fname                = bat.nc
+    pdfmethod            = histogram
+[histogram]
+    referencenbins       = {nb}
+    nbins                = {nb}
+[entropy]
+    workset              = 2D
+    usesubset            = false
+    useneighbor          = false
+    jacobian             = true
+    scoringmethod        = MIST
+    estimator            = ML
+""")
+        t0 = time.perf_counter()
+        r = subprocess.run([host, "-i", cfg, "-O"], capture_output=True, text=True)
+        t_run = time.perf_counter() - t0
+        if r.returncode != 0:
+            raise SystemExit(f"host failed ({r.returncode}): {r.stdout[-2000:]} {r.stderr[-2000:]}")
+        stages = {}
+        for line in r.stdout.splitlines():
+            if line.startswith("TIME: "):
+                k, v = line[6:].rsplit(" ", 2)[:2]
+                stages[k] = float(v)
+        outs = sorted(os.listdir(os.path.join(work, "out")))
+        sizes = sum(os.path.getsize(os.path.join(work, "out", o)) for o in outs)
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    out = {
+        "metric": "pair entropies/sec end to end through the C++ host, file I/O included",
+        "value": P / t_run, "unit": "pair entropies/s", "higher_is_better": True,
+        "n_gpus": 1, "scaling": "weak", "vs_baseline": None, "dtype": "int8",
+        "data": "synthetic (seeded Philox BAT generator, latent-coupled), CDF-5 BAT file",
+        "config": {"workload": f"centre_hip -i centre.cfg: discretise + entropy + MIE/MIST, {D} DOF "
+                               f"({nbd}B/{nan}A/{ndh}T) x {F} frames, {nb} bins, {S} steps, ML",
+                   "n_pair": P},
+        "seconds": t_run, "stages_s": stages, "bat_file_write_s": t_write,
+        "output_files": len(outs), "output_bytes": sizes,
+    }
     print(json.dumps(out), flush=True)
 
 

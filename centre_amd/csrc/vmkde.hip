@@ -11,7 +11,7 @@
 //            sum_f exp(h cos(X - a_f)) * {-sin(X - a_f), h sin^2 - cos},
 //            with cos/sin(X - a) by the angle-difference identities from the
 //            per-frame and per-point cos/sin.  FP64 throughout; frames are
-//            staged in LDS and shared by 32 grid points per workgroup.
+//            staged in LDS and shared by the grid points of a workgroup.
 //   states   one lane per torsion: maxima (gradient sign change with second
 //            derivative < -1e-4, at most 9), the two selection sorts, the
 //            initial minima, steepest descent on the linearly interpolated
@@ -51,19 +51,21 @@ __global__ __launch_bounds__(256) void vm_prep_kernel(VmkdeArgs a) {
     }
 }
 
-constexpr int kPts = 32;      // grid points per workgroup
-constexpr int kLanesF = 8;    // frame lanes per grid point
 constexpr int kStage = 1024;  // frames staged per LDS pass
 
-__global__ __launch_bounds__(256) void vm_grid_kernel(VmkdeArgs a, double h, double denom) {
+// A workgroup covers np grid points of one torsion with 256 / np frame lanes
+// per point (threads past np * lanes only stage frames); see grid_split.
+__global__ __launch_bounds__(256) void vm_grid_kernel(VmkdeArgs a, double h, double denom, int np) {
     __shared__ double2 fr[kStage];
-    __shared__ double red[kLanesF][kPts][2];
+    __shared__ double red[256][2];
     const int G = a.grid;
     const uint32_t d = blockIdx.y;
-    const int p = blockIdx.x * kPts + (threadIdx.x % kPts);
-    const int fl = threadIdx.x / kPts;
+    const int nl = 256 / np;                       // frame lanes per point
+    const int pl = threadIdx.x % np, fl = threadIdx.x / np;
+    const int p = blockIdx.x * np + pl;
+    const bool live = fl < nl && p <= G;
     double cx = 0, sx = 0;
-    if (p <= G) sincos((p * (360.0 / G)) * kDegRad, &sx, &cx);
+    if (live) sincos((p * (360.0 / G)) * kDegRad, &sx, &cx);
     double g = 0.0, sd = 0.0;
     const double2* trig = reinterpret_cast<const double2*>(a.trig) + (uint64_t)d * a.n_frame;
     for (uint64_t f0 = 0; f0 < a.n_frame; f0 += kStage) {
@@ -71,8 +73,8 @@ __global__ __launch_bounds__(256) void vm_grid_kernel(VmkdeArgs a, double h, dou
         __syncthreads();
         for (int i = threadIdx.x; i < nf; i += 256) fr[i] = trig[f0 + i];
         __syncthreads();
-        if (p <= G) {
-            for (int i = fl; i < nf; i += kLanesF) {
+        if (live) {
+            for (int i = fl; i < nf; i += nl) {
                 const double ca = fr[i].x, sa = fr[i].y;
                 const double c = cx * ca + sx * sa;      // cos(X - a)
                 const double s = sx * ca - cx * sa;      // sin(X - a)
@@ -82,19 +84,37 @@ __global__ __launch_bounds__(256) void vm_grid_kernel(VmkdeArgs a, double h, dou
             }
         }
     }
-    red[fl][threadIdx.x % kPts][0] = g;
-    red[fl][threadIdx.x % kPts][1] = sd;
+    red[threadIdx.x][0] = g;
+    red[threadIdx.x][1] = sd;
     __syncthreads();
     if (fl == 0 && p <= G) {
         double gg = 0.0, ss = 0.0;
-        for (int k = 0; k < kLanesF; ++k) {
-            gg += red[k][threadIdx.x][0];
-            ss += red[k][threadIdx.x][1];
+        for (int k = 0; k < nl; ++k) {
+            gg += red[k * np + pl][0];
+            ss += red[k * np + pl][1];
         }
         double* out = a.gridbuf + (uint64_t)d * 2 * (G + 1);
         out[p] = gg * h / denom;
         out[G + 1 + p] = ss * h / denom;
     }
+}
+
+// Points per workgroup for G+1 grid points: the fewest frame passes per
+// torsion, ceil((G+1)/np) workgroups x F/floor(256/np) frames each, with at
+// most 32 workgroups per torsion (each re-reads the torsion's frames).
+int grid_split(int G) {
+    int best = 32;
+    double cost = 1e300;
+    for (int np = 8; np <= 128; ++np) {
+        const int nwg = (G + 1 + np - 1) / np;
+        if (nwg > 32) continue;
+        const double c = (double)nwg / (double)(256 / np);
+        if (c < cost - 1e-12) {
+            cost = c;
+            best = np;
+        }
+    }
+    return best;
 }
 
 __global__ void vm_states_kernel(VmkdeArgs a) {
@@ -199,8 +219,9 @@ hipError_t run_vmkde(const VmkdeArgs& a, hipStream_t st) {
     const uint64_t n = (uint64_t)a.n_dof * a.n_frame;
     hipLaunchKernelGGL(vm_prep_kernel, dim3((unsigned)((a.n_frame + 63) / 64), (a.n_dof + 63) / 64),
                        dim3(256), 0, st, a);
-    hipLaunchKernelGGL(vm_grid_kernel, dim3((a.grid + 1 + kPts - 1) / kPts, a.n_dof), dim3(256),
-                       0, st, a, h, denom);
+    const int np = grid_split(a.grid);
+    hipLaunchKernelGGL(vm_grid_kernel, dim3((a.grid + 1 + np - 1) / np, a.n_dof), dim3(256), 0, st,
+                       a, h, denom, np);
     hipLaunchKernelGGL(vm_states_kernel, dim3((a.n_dof + 63) / 64), dim3(64), 0, st, a);
     hipLaunchKernelGGL(vm_assign_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
     return hipGetLastError();

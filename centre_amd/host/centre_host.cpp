@@ -520,6 +520,16 @@ void Host::run_entropy() {
     for (int k = 0; k < 6; ++k)
         if (hist_on(KINDS[k].set)) hist2[k].assign(pairs[k].size() / 2 * Se * bsq, 0);
     std::vector<double> kernel_ms;
+    // Every pair of the concatenated subset DOFs (all six kinds, no neighbour
+    // list, no 2-D count files, fused estimators only): one all-pairs launch,
+    // whose triangle order then gives each kind's records.  Otherwise one
+    // explicit plan per kind.
+    uint64_t n_req = 0;
+    for (int k = 0; k < 6; ++k) n_req += pairs[k].size() / 2;
+    bool allpairs = !(in.usesubset && in.useneighbor) &&
+                    n_req == (uint64_t)Dall * (Dall - 1) / 2 && Dall >= 2;
+    for (int k = 0; k < 6; ++k) allpairs = allpairs && hist2[k].empty();
+    for (int e : in.estimators) allpairs = allpairs && e <= 1;   // ML / MM: counts never stored
     for (int sc = 0; sc < nsch; ++sc) {
         std::vector<uint8_t> all((size_t)Dall * Fe);
         for (int ty = 0; ty < 3; ++ty)
@@ -554,6 +564,34 @@ void Host::run_entropy() {
             for (int e = 0; e < nest; ++e)
                 for (int s = 0; s < S; ++s)
                     s1[(((size_t)d * nest + e) * nsch + sc) * S + s] = h1[((size_t)d * nest + e) * S + s];
+        if (allpairs) {
+            const uint64_t PA = (uint64_t)Dall * (Dall - 1) / 2;
+            centre_pair_plan* plan = nullptr;
+            ck(ctx, centre_pair_plan_create(ctx, Dall, nb, nullptr, 0, &plan), "plan");
+            Dev ds2(ctx, PA * nest * S * 8);
+            const int rc = centre_dev_pair_entropy(ctx, plan, dcodes.as<uint8_t>(), Fe, Fe,
+                                                   dt.as<uint8_t>(), dx.as<double>(), S,
+                                                   in.estimators.data(), nest, in.jacobian, kde,
+                                                   nullptr, ds2.as<double>(), nullptr, nullptr);
+            centre_pair_plan_destroy(plan);
+            ck(ctx, rc, "pair_entropy");
+            kernel_ms.push_back(centre_last_pair_kernel_ms(ctx));
+            std::vector<double> h2(PA * nest * S);
+            ds2.get(h2.data(), h2.size() * 8);
+            for (int k = 0; k < 6; ++k) {
+                const uint64_t P = pairs[k].size() / 2;
+                s2[k].resize(P * nest * nsch * S);
+                for (uint64_t p = 0; p < P; ++p) {
+                    // i < j: rows come from the earlier type (or the earlier DOF)
+                    const uint64_t i = pairs[k][2 * p], j = pairs[k][2 * p + 1];
+                    const uint64_t t = i * Dall - i * (i + 1) / 2 + (j - i - 1);
+                    for (int e = 0; e < nest; ++e)
+                        for (int s = 0; s < S; ++s)
+                            s2[k][((p * nest + e) * nsch + sc) * S + s] = h2[(t * nest + e) * S + s];
+                }
+            }
+            continue;
+        }
         for (int k = 0; k < 6; ++k) {
             const uint64_t P = pairs[k].size() / 2;
             if (!P) continue;

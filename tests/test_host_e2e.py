@@ -18,7 +18,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from centre_amd import cdf
+from centre_amd import cdf, hip
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HOST = os.path.join(ROOT, "centre_amd", "centre_hip")
@@ -227,3 +227,74 @@ def test_host_example1_count_files(tmp_path, golden):
         pr = np.stack([ids[:, 0], ids[:, 1] + len(codes[x])], 1)
         want = O.pair_counts(both, 30, pr, 4)[:, steps].reshape(len(ids), 2, 900)
         np.testing.assert_array_equal(freq, want)
+
+
+@pytest.mark.gpu
+def test_host_all_pairs_from_bat_file(tmp_path):
+    """Discretise a BAT file and compute every pair (no subset, no neighbour
+    list: the host's single all-pairs launch), with ML and MM; every 1-D and
+    2-D record agrees with the oracle run on the host's own bin files."""
+    nbd, nan, ndh, F, nb, S = 9, 8, 10, 6000, 30, 3
+    bat = hip.synth_bat(0xCE27E + 77, nbd, nan, ndh, F)
+    d = str(tmp_path)
+    out = os.path.join(d, "out")
+    os.makedirs(out)
+    cdf.write_dataset(os.path.join(d, "bat.nc"),
+                      [("frame", F), ("bond", nbd), ("angle", nan), ("dihedral", ndh)],
+                      [cdf.VarSpec("bond", ("frame", "bond"), np.ascontiguousarray(bat[:, :nbd])),
+                       cdf.VarSpec("angle", ("frame", "angle"),
+                                   np.ascontiguousarray(bat[:, nbd:nbd + nan])),
+                       cdf.VarSpec("dihedral", ("frame", "dihedral"),
+                                   np.ascontiguousarray(bat[:, nbd + nan:]))],
+                      attrs={"convention": "CENTRE"}, version=5, unlimited="frame")
+    cfg = os.path.join(d, "centre.cfg")
+    with open(cfg, "w") as f:
+        f.write(f"""[control]
+    infilepath           = {d}/
+    outfilepath          = {out}/
+    nsteps               = {S}
+    discretize           = true
+    calcentropy          = true
+    genconvgdata         = false
+[discretization]
+    fname                = bat.nc
+    pdfmethod            = histogram
+[histogram]
+    referencenbins       = {nb}
+    nbins                = {nb}
+[entropy]
+    workset              = 2D
+    usesubset            = false
+    useneighbor          = false
+    jacobian             = true
+    estimator            = ML, MM
+""")
+    print("running host all-pairs", flush=True)
+    r = subprocess.run([HOST, "-i", cfg], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    codes, ext = {}, {}
+    for t, fname, _, _, _ in TYPES:
+        ds = cdf.open_dataset(os.path.join(out, fname))
+        codes[t] = ds.read("coord")[:, 0, :]
+        ext[t] = ds.read("extreme")
+    tnum = {"b": 1, "a": 2, "t": 3}
+    for tag, t in (("bnd", "b"), ("ang", "a"), ("tor", "t")):
+        ds = cdf.open_dataset(os.path.join(out, f"entcontri_{tag}-1d.nc"))
+        want = O.dof_entropies(codes[t], np.full(len(codes[t]), tnum[t], np.uint8), ext[t], S,
+                               [nb], (0, 1))
+        np.testing.assert_allclose(ds.read("contri_S")[:, :, 0, :], want[:, :, 0, :],
+                                   rtol=1e-10, atol=1e-12)
+    kinds = {"bnd": "bb", "ang": "aa", "tor": "tt", "ba": "ba", "bd": "bt", "ad": "at"}
+    for tag, (x, y) in kinds.items():
+        ds = cdf.open_dataset(os.path.join(out, f"entcontri_{tag}-2d.nc"))
+        ids = ds.read("id").astype(np.int64)
+        n_x, n_y = len(codes[x]), len(codes[y])
+        want_n = n_x * (n_x - 1) // 2 if x == y else n_x * n_y
+        assert len(ids) == want_n, tag
+        both = np.concatenate([codes[x], codes[y]])
+        types = np.array([tnum[x]] * n_x + [tnum[y]] * n_y, np.uint8)
+        e = np.concatenate([ext[x], ext[y]])
+        pr = np.stack([ids[:, 0], ids[:, 1] + n_x], 1)
+        want = O.pair_entropies(both, types, e, pr, S, [nb], (0, 1))
+        np.testing.assert_allclose(ds.read("contri_S")[:, :, 0, :], want[:, :, 0, :],
+                                   rtol=1e-10, atol=1e-12, err_msg=tag)
