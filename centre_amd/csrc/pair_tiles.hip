@@ -892,8 +892,11 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     const int nch = (int)((ss + kChunk - 1) / kChunk);
     const int G = S * nch;
     const int nlive = (diag ? 1 : 2) * R * kGroups;
+    // this lane's slot: DOF ld, frames k4..k4+3 of each chunk.  The first
+    // waves stage one slot per lane (lanes past nlive repeat live slots of
+    // their own wave); spreading the slots over all eight waves instead
+    // (nlive/8 lanes each) ran 20% slower.
     const bool stager = wave * 64 < nlive;   // wave-uniform
-    // this lane's slot: DOF ld, frames k4..k4+3 of each chunk
     const int q = (wave * 64 + lane) % nlive;
     const int ld = q / kGroups, k4 = (q - ld * kGroups) * 4;
     const bool isA = ld < R;
