@@ -773,7 +773,6 @@ __global__ __launch_bounds__(256) void pair_persist_kernel(PairKernelArgs a) {
 // other one's MFMAs run.  Only the first ceil(live slots/64) waves stage
 // codes (one slot per lane, lanes past the live slots repeat live ones of
 // their own wave); the others only multiply.
-template <int kW>
 __device__ __forceinline__ void epi_block8(const v16i& blk, int row0, int gcol, int lane, int* park,
                                            const Smem& sm, int nb, int R, int nstr, int ntab) {
     // the lane holds column gcol and rows row0 + 8*(r/4) + r%4, r = 0..15
@@ -964,22 +963,26 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     };
     int* const park = reinterpret_cast<int*>(smem_raw + lay.park) + wave * 64 * 16;
 
-    int ps = 0, pc = 0;   // chunk to stage next
-    int ms = 0, mc = 0;   // chunk being multiplied
-    auto step_end = [&]() __attribute__((always_inline)) {
-        // step ms ends with chunk g
-        __syncthreads();
+    // Step-end epilogue, all waves: each block's c log c and nonzero bins
+    // into the per-pair LDS slots.
+    auto epilogue = [&]() __attribute__((always_inline)) {
+        __syncthreads();   // the pending finalize has read and cleared the slots
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
-                epi_block8<8>(acc[i][j], 128 * wr + 32 * i + 4 * (lane >> 5),
-                              64 * wc + 32 * j + (lane & 31), lane, park, sm, nb, R, nstr, ntab);
+                epi_block8(acc[i][j], 128 * wr + 32 * i + 4 * (lane >> 5),
+                           64 * wc + 32 * j + (lane & 31), lane, park, sm, nb, R, nstr, ntab);
                 __builtin_amdgcn_sched_barrier(0);
             }
-        __syncthreads();
+    };
+    // Slot sums of step fs -> S2 / MI, by wave 7 (never a stager) one
+    // iteration after the epilogue, so that it overlaps the next chunk; the
+    // iteration barrier orders it after the atomics, and the next epilogue's
+    // opening barrier after it.
+    auto finalize = [&](int fs) __attribute__((always_inline)) {
         const double* const coef = sm.invt + 256;
-        const double n = (double)(ms + 1) * (double)ss;
+        const double n = (double)(fs + 1) * (double)ss;
         const double logn = log_pos_lds(n, sm.logt, sm.invt, coef);
         asm volatile("" ::: "memory");   // read the output pointers here
         double* const s2 = kBatch ? sdp->s2 : a.s2;
@@ -987,7 +990,7 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
         const double* const dof_w = kBatch ? sdp->dof_w : a.dof_w;
         const double* const dof_L = kBatch ? sdp->dof_L : a.dof_L;
         const double* const dof_S1 = kBatch ? sdp->dof_S1 : a.dof_S1;
-        for (int pq = tid; pq < R * R; pq += 512) {
+        for (int pq = lane; pq < R * R; pq += 64) {
             const int rd = pq / R, cd = pq - rd * R;
             const long long gi = r0 + rd, gj = c0 + cd;
             const long long p = gi >= gj ? -1
@@ -1001,61 +1004,88 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
                 sm.slotN[pq * nstr + k] = 0;
             }
             if (p >= 0) {
-                const double L = dof_L[gi * S + ms] + dof_L[gj * S + ms];
+                const double L = dof_L[gi * S + fs] + dof_L[gj * S + fs];
                 const double area = dof_w[gi] * dof_w[gj];
                 const double ml = (logn + log_pos_lds(area, sm.logt, sm.invt, coef)) - T / n + L / n;
                 const double mm = ml + (double)(unsigned)(nz - 1) / (2.0 * n);
-                if (a.slot_ml >= 0) s2[((size_t)p * a.nest + a.slot_ml) * S + ms] = ml;
-                if (a.slot_mm >= 0) s2[((size_t)p * a.nest + a.slot_mm) * S + ms] = mm;
+                if (a.slot_ml >= 0) s2[((size_t)p * a.nest + a.slot_ml) * S + fs] = ml;
+                if (a.slot_mm >= 0) s2[((size_t)p * a.nest + a.slot_mm) * S + fs] = mm;
                 if (mi) {
-                    const double s1 = dof_S1[gi * S + ms] + dof_S1[gj * S + ms];
-                    mi[(size_t)p * S + ms] = s1 - (a.mi_from_mm ? mm : ml);
+                    const double s1 = dof_S1[gi * S + fs] + dof_S1[gj * S + fs];
+                    mi[(size_t)p * S + fs] = s1 - (a.mi_from_mm ? mm : ml);
                 }
             }
         }
     };
-    auto stage_next = [&](int b) __attribute__((always_inline)) {
-        // chunk at (ps, pc) into buffer b
-        if (stager && ps < S) {
-            const int cis = (ps * nch + pc) & (kSuper - 1);
-            if (cis == 0 && !(ps == 0 && pc == 0)) {
-#pragma unroll
-                for (int c = 0; c < kSuper; ++c) cur[c] = nxt[c];
-                load4(nxt);
-            }
-            stage(b, cis, nvalid_of(ps, pc), cur);
+
+    // One barrier per chunk: chunk g multiplies from buffer g&1 while the
+    // stager waves write chunk g+1 into the other buffer, their staging
+    // instructions spread between their MFMAs (stager waves and the others
+    // run separate copies of the loop; both pass the same barriers).
+    // Reading chunk g+1's fragments during chunk g's MFMAs (half a chunk
+    // ahead) needs 24 more registers than the two-waves-per-SIMD budget
+    // leaves next to the 128 accumulators: that variant spills and ran 2.5x
+    // slower.
+    int ps = 0, pc = 0;   // chunk to stage next
+    int ms = 0, mc = 0;   // chunk being multiplied
+    int fin = -1;         // step whose finalize is pending
+    auto run = [&](auto stg) __attribute__((always_inline)) {
+        constexpr bool kStg = decltype(stg)::value;
+        if constexpr (kStg) {
+            load4(cur);
+            load4(nxt);
+            stage(0, 0, nvalid_of(0, 0), cur);
         }
         advance(ps, pc, nch);
-    };
-    if (stager) {
-        load4(cur);
-        load4(nxt);
-    }
-    // One barrier per chunk: chunk g multiplies from buffer g&1 while the
-    // stager waves write chunk g+1 into the other buffer.  (Reading chunk
-    // g+1's fragments during chunk g's MFMAs needs a second set of fragment
-    // registers next to the 128 accumulators; with two waves per SIMD that
-    // spills and runs 8x slower.)
-    // One barrier per chunk: chunk g multiplies from buffer g&1 while the
-    // stager waves write chunk g+1 into the other buffer.  Reading chunk
-    // g+1's fragments during chunk g's MFMAs (half a chunk ahead) needs 24
-    // more registers than the two-waves-per-SIMD budget leaves next to the
-    // 128 accumulators: that variant spills and ran 2.5x slower.
-    stage_next(0);
-    __syncthreads();
-    for (int g = 0; g < G; ++g) {
-        const int b = g & 1;
-        const uint8_t* base = sm.oh + (size_t)b * 2 * kOpBytes;
-        v4i fa0[4], fb0[2], fa1[4], fb1[2];
-        frags(base, 0, fa0, fb0);
-        frags(base, 1, fa1, fb1);
-        mfma8(fa0, fb0);
-        mfma8(fa1, fb1);
-        stage_next(b ^ 1);
-        if (mc == nch - 1) step_end();
-        advance(ms, mc, nch);
         __syncthreads();
-    }
+        for (int g = 0; g < G; ++g) {
+            const int b = g & 1;
+            const uint8_t* base = sm.oh + (size_t)b * 2 * kOpBytes;
+            const int cis = (ps * nch + pc) & (kSuper - 1);   // chunk g+1 in its super-chunk
+            if constexpr (kStg) {
+                if (cis == 0) {
+#pragma unroll
+                    for (int c = 0; c < kSuper; ++c) cur[c] = nxt[c];
+                    load4(nxt);
+                }
+            }
+            v4i fa0[4], fb0[2], fa1[4], fb1[2];
+            frags(base, 0, fa0, fb0);
+            frags(base, 1, fa1, fb1);
+            mfma8(fa0, fb0);
+            mfma8(fa1, fb1);
+            if constexpr (kStg) {
+                // past the last chunk nvalid is 0: everything goes to the trash row
+                stage(b ^ 1, cis, nvalid_of(ps, pc), cur);
+                __builtin_amdgcn_sched_group_barrier(0x100, 6, 0);
+#pragma unroll
+                for (int i = 0; i < 6; ++i) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                    __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+                }
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                    __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);
+                    __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
+                }
+                __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+            } else {
+                if (fin >= 0 && wave == 7) finalize(fin);
+            }
+            advance(ps, pc, nch);
+            fin = -1;
+            if (mc == nch - 1) {
+                epilogue();
+                fin = ms;
+            }
+            advance(ms, mc, nch);
+            __syncthreads();
+        }
+    };
+    if (stager) run(std::true_type{});
+    else run(std::false_type{});
+    if (wave == 7 && fin >= 0) finalize(fin);
 }
 
 __global__ __launch_bounds__(512) void pair_tile8_kernel(PairKernelArgs a) {
