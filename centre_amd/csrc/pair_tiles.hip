@@ -87,7 +87,9 @@ __host__ __device__ inline LdsLayout lds_layout(int R, int nwave = 4) {
     off = (off + 15) & ~(size_t)15;
     L.log = off;   // log(1 + j/256), 1/(1 + j/256), then 8 series coefficients
     off += (2 * 256 + 8) * sizeof(double);
-    // epilogue parking area of the fast paths (R <= 32): 4 KB per wave
+    // epilogue parking area of the four-wave tile's fast path (R <= 32):
+    // 4 KB per wave; nwave = 0 for the eight-wave tile, which reads its
+    // accumulators in place
     L.park = off;
     if (R <= 32) off += (size_t)nwave * 64 * 16 * sizeof(int);
     L.tab = off;
@@ -773,16 +775,12 @@ __global__ __launch_bounds__(256) void pair_persist_kernel(PairKernelArgs a) {
 // other one's MFMAs run.  Only the first ceil(live slots/64) waves stage
 // codes (one slot per lane, lanes past the live slots repeat live ones of
 // their own wave); the others only multiply.
-__device__ __forceinline__ void epi_block8(const v16i& blk, int row0, int gcol, int lane, int* park,
+__device__ __forceinline__ void epi_block8(const v16i& blk, int row0, int gcol, int lane,
                                            const Smem& sm, int nb, int R, int nstr, int ntab) {
-    // the lane holds column gcol and rows row0 + 8*(r/4) + r%4, r = 0..15
+    // the lane holds column gcol and rows row0 + 8*(r/4) + r%4, r = 0..15;
+    // with two waves per SIMD the accumulators are ordinary VGPRs and are
+    // read in place (the four-wave tile parks its AGPRs in LDS instead)
     const int cd = sm.col2dof[gcol];
-    v4i* pk = reinterpret_cast<v4i*>(park) + lane;
-    pk[0] = v4i{blk[0], blk[1], blk[2], blk[3]};
-    pk[64] = v4i{blk[4], blk[5], blk[6], blk[7]};
-    pk[128] = v4i{blk[8], blk[9], blk[10], blk[11]};
-    pk[192] = v4i{blk[12], blk[13], blk[14], blk[15]};
-    asm volatile("" ::: "memory");
     const int stripe = lane % nstr;
     {   // nb >= 28: the lane's 16 rows touch at most two DOFs
         const int dlo = sm.row2dof[row0];
@@ -790,16 +788,14 @@ __device__ __forceinline__ void epi_block8(const v16i& blk, int row0, int gcol, 
         const int kk = (dlo + 1) * nb - row0;
         double tot = 0.0, hi = 0.0;
         int ntot = 0, nhi = 0;
-        // four rows at a time keeps few values live (this runs between MFMAs
-        // of the software-pipelined loop)
+        // four rows at a time keeps few values live
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const v4i c4 = pk[64 * q];
             int c[4];
             double tv[4];
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
-                c[t] = ok ? c4[t] : 0;
+                c[t] = ok ? blk[4 * q + t] : 0;
                 tv[t] = sm.ctab[min(c[t], ntab - 1)];
             }
 #pragma unroll
@@ -839,7 +835,7 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     const uint64_t ss = kBatch ? sdp->ss : a.ss;
     const uint64_t sstride = kBatch ? sdp->sstride : a.sstride;
     const int R = a.R, nb = a.nb, S = a.nsteps;
-    const LdsLayout lay = lds_layout(R, 8);
+    const LdsLayout lay = lds_layout(R, 0);
     const int nstr = lay.nstripe, ntab = lay.ntab;
     Smem sm;
     sm.oh = smem_raw;
@@ -961,8 +957,6 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
             for (int j = 0; j < 2; ++j)
                 acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
     };
-    int* const park = reinterpret_cast<int*>(smem_raw + lay.park) + wave * 64 * 16;
-
     // Step-end epilogue, all waves: each block's c log c and nonzero bins
     // into the per-pair LDS slots.
     auto epilogue = [&]() __attribute__((always_inline)) {
@@ -972,7 +966,7 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
                 epi_block8(acc[i][j], 128 * wr + 32 * i + 4 * (lane >> 5),
-                           64 * wc + 32 * j + (lane & 31), lane, park, sm, nb, R, nstr, ntab);
+                           64 * wc + 32 * j + (lane & 31), lane, sm, nb, R, nstr, ntab);
                 __builtin_amdgcn_sched_barrier(0);
             }
     };
@@ -1175,15 +1169,16 @@ hipError_t launch_pair_persistent(const PairKernelArgs& a, int grid, hipStream_t
 }
 
 bool pair_tile8_supported(int R) {
-    // R <= 8 means nb >= 29: a lane's 16 rows of a block touch at most two DOFs
-    const LdsLayout L = lds_layout(R, 8);
+    // R <= 8 means nb >= 29: a lane's 16 rows of a block touch at most two DOFs.
+    // The eight-wave tile needs no park area (its accumulators are VGPRs).
+    const LdsLayout L = lds_layout(R, 0);
     return R <= 8 && 2 * R * kGroups <= 512 && L.ntab > 0 && L.total <= kLdsCap;
 }
 
 hipError_t launch_pair_tiles8(const PairKernelArgs& a, int n_tiles, hipStream_t st) {
     if (n_tiles <= 0) return hipSuccess;
     if (!pair_tile8_supported(a.R)) return hipErrorInvalidValue;
-    const size_t lds = lds_layout(a.R, 8).total;
+    const size_t lds = lds_layout(a.R, 0).total;
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&pair_tile8_kernel),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
@@ -1194,7 +1189,7 @@ hipError_t launch_pair_tiles8(const PairKernelArgs& a, int n_tiles, hipStream_t 
 hipError_t launch_pair_batch8(const PairKernelArgs& a, hipStream_t st) {
     if (a.n_work <= 0) return hipSuccess;
     if (!pair_tile8_supported(a.R)) return hipErrorInvalidValue;
-    const size_t lds = lds_layout(a.R, 8).total;
+    const size_t lds = lds_layout(a.R, 0).total;
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&pair_batch8_kernel),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
