@@ -99,7 +99,7 @@ struct PairKernelArgs {
 
 // Launchers (return hipError_t of the launch).
 // Eight-wave kernel for tiles with R DOFs on both sides and step-aligned codes
-// (no count dump); supported for R with 2R*kGroups <= 512 whose layout keeps
+// (no count dump); supported for R <= 8 (nb >= 29) when its LDS layout keeps
 // the c log c table.
 bool pair_tile8_supported(int R);
 hipError_t launch_pair_tiles8(const PairKernelArgs& a, int n_tiles, hipStream_t st);

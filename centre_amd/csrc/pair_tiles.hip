@@ -36,6 +36,12 @@
 // (4x4 MFMA 32x32 blocks = 256 int32 accumulators per lane).  Blocks with no
 // wanted pair (padding rows/cols, the lower triangle of diagonal tiles) are
 // skipped by a wave-uniform mask.
+//
+// Tiles with R DOFs on both sides and 29 <= nb take the eight-wave form
+// below (pair_tile8_kernel): two waves per SIMD, 128 accumulators each, so
+// that one wave's staging and epilogue overlap the other's MFMAs.  The
+// four-wave form keeps the rest: tiles of a partial block, count dumps and
+// smaller bin counts.
 #include "centre_internal.h"
 
 #include <type_traits>
