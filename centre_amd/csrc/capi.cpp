@@ -351,9 +351,9 @@ int centre_pair_plan_create(centre_ctx* ctx, uint32_t n_dof, int nbins,
             tmap[q * R * R + (i % R) * R + (j % R)] = (int)c;
         }
         hipError_t e;
-        if ((e = hipMalloc(&plan->d_tilemap, sizeof(int) * tmap.size())) != hipSuccess ||
-            (e = hipMemcpy(plan->d_tilemap, tmap.data(), sizeof(int) * tmap.size(),
-                           hipMemcpyHostToDevice)) != hipSuccess ||
+        if ((e = hipMalloc(&plan->d_tilemap, sizeof(int) * (tmap.size() ? tmap.size() : 1))) != hipSuccess ||
+            (!tmap.empty() && (e = hipMemcpy(plan->d_tilemap, tmap.data(), sizeof(int) * tmap.size(),
+                                             hipMemcpyHostToDevice)) != hipSuccess) ||
             (e = hipMalloc(&plan->d_rec2canon, sizeof(uint64_t) * (n_pair ? n_pair : 1))) != hipSuccess ||
             (n_pair && (e = hipMemcpy(plan->d_rec2canon, r2c.data(), sizeof(uint64_t) * n_pair,
                                       hipMemcpyHostToDevice)) != hipSuccess) ||

@@ -195,9 +195,17 @@ def entropy_2d(t1, t2, ests, nsteps, step_size, mn1, mx1, mn2, mx2, jacobian,
     return out
 
 
+def _check_steps(nsteps):
+    # the C side takes nsteps as uint8_t, the reference's hbin_t
+    # (configcentre.h:24, Estimators.h:54); 256 would wrap to 0
+    if not 1 <= int(nsteps) <= 255:
+        raise ValueError("nsteps must be in 1..255")
+
+
 def dof_entropies(codes, types, ext, nsteps, schemes=(30,), ests=(ML,),
                   jacobian=True, kde=False, nthreads=0):
     """codes [D, nsch, F] (or [D, F]) -> S1 [D, nest, nsch, nsteps]."""
+    _check_steps(nsteps)
     sch = _c(np.atleast_1d(schemes), np.uint8)
     codes = _c(codes, np.uint8)
     D = codes.shape[0]
@@ -215,6 +223,7 @@ def dof_entropies(codes, types, ext, nsteps, schemes=(30,), ests=(ML,),
 def pair_entropies(codes, types, ext, pairs, nsteps, schemes=(30,),
                    ests=(ML,), jacobian=True, kde=False, nthreads=0):
     """codes [D, nsch, F] -> S2 [npair, nest, nsch, nsteps]."""
+    _check_steps(nsteps)
     sch = _c(np.atleast_1d(schemes), np.uint8)
     codes = _c(codes, np.uint8)
     D = codes.shape[0]
@@ -234,6 +243,7 @@ def pair_entropies(codes, types, ext, pairs, nsteps, schemes=(30,),
 
 
 def pair_counts(codes, nb, pairs, nsteps):
+    _check_steps(nsteps)
     codes = _c(codes, np.uint8)
     D = codes.shape[0]
     F = codes.shape[-1]

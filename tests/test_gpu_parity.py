@@ -314,3 +314,42 @@ def test_vmkde_grid_and_states(ctx):
             assert nf[d] == wn
             assert np.array_equal(mins[d, :wn], wm)
             assert np.array_equal(codes[d], wc)
+
+
+# ---------------------------------------------------------------------------
+# edge cases: one bin, one frame per step, two DOFs, empty pair lists, the
+# count-materialisation limit
+def test_single_bin_and_single_frame_steps(ctx):
+    rng = np.random.default_rng(21)
+    for nb, D, F, S in ((1, 5, 1000, 3), (7, 2, 64, 64), (50, 2, 255, 255)):
+        codes = rng.integers(0, nb, size=(D, F), dtype=np.uint8)
+        types = rng.integers(1, 4, size=D).astype(np.uint8)
+        ext = np.zeros((D, 4))
+        ext[:, 0] = rng.uniform(0.5, 1.0, D)
+        ext[:, 1] = ext[:, 0] + rng.uniform(0.2, 1.5, D)
+        iu, ju = np.triu_indices(D, 1)
+        print(f"edge nb={nb} D={D} F={F} S={S}", flush=True)
+        s1, s2, mi = ctx.pair_entropy(codes, types, ext, nb, S, ests=(0, 1, 2, 3))
+        w1 = O.dof_entropies(codes, types, ext, S, [nb], (0, 1, 2, 3))
+        w2 = O.pair_entropies(codes, types, ext, np.stack([iu, ju], 1), S, [nb], (0, 1, 2, 3))
+        assert rel_err(s1, w1[:, :, 0, :]) < REL
+        assert rel_err(s2, w2[:, :, 0, :]) < REL
+
+
+def test_empty_pair_list(ctx):
+    codes = np.zeros((4, 100), np.uint8)
+    s1, s2, mi = ctx.pair_entropy(codes, np.ones(4, np.uint8), np.tile([1.0, 2.0, 0, 0], (4, 1)),
+                                  10, 2, pairs=np.zeros((0, 2), np.uint32))
+    assert s2.shape == (0, 1, 2) and mi.shape == (0, 2)
+    assert np.all(np.isfinite(s1))
+
+
+def test_count_materialisation_limit_is_an_error(ctx):
+    """Chao-Shen and James-Stein go through materialised counts; a plan whose
+    counts would exceed 16 GiB is refused (-3) instead of allocated."""
+    D, F, nb, S = 3000, 64, 50, 10
+    codes = np.zeros((D, F), np.uint8)
+    with pytest.raises(hip.CentreError) as e:
+        ctx.pair_entropy(codes, np.ones(D, np.uint8), np.tile([1.0, 2.0, 0, 0], (D, 1)), nb, S,
+                         ests=(2,), want_s1=False, want_mi=False)
+    assert e.value.code == -3

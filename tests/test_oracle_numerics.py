@@ -228,3 +228,15 @@ def test_shuffle_swap_semantics():
     for i, j in enumerate(idx):
         ref[i], ref[int(j)] = ref[int(j)], ref[i]
     assert got.tolist() == ref
+
+
+def test_step_count_outside_uint8_is_refused():
+    """Step counts are unsigned char in the reference; 0 and 256 are refused
+    rather than wrapped."""
+    codes = np.zeros((2, 256), np.uint8)
+    ext = np.tile([1.0, 2.0, 0.0, 0.0], (2, 1))
+    for bad in (0, 256):
+        with pytest.raises(ValueError):
+            O.dof_entropies(codes, np.ones(2, np.uint8), ext, bad, [4])
+        with pytest.raises(ValueError):
+            O.pair_counts(codes, 4, [[0, 1]], bad)
