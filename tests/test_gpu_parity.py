@@ -300,7 +300,9 @@ def test_mst_example2_trees(ctx, golden):
 # ---------------------------------------------------------------------------
 def test_vmkde_grid_and_states(ctx):
     bat = hip.synth_bat(0xCE27E + 2, 0, 0, 12, 3000)
-    for G, kappa, nmax in ((360, 1.0, 3), (64, 10.0, 5)):
+    # even grids pair points half a turn apart; odd ones evaluate each point
+    for G, kappa, nmax in ((360, 1.0, 3), (64, 10.0, 5), (45, 4.0, 4), (2, 1.0, 2)):
+        print(f"vmkde G={G} kappa={kappa}", flush=True)
         codes, mins, nf, grid = ctx.discretize_vmkde(bat, nmax, kappa, 5, 1e-4, 1000, G,
                                                      want_grid=True)
         for d in range(bat.shape[1]):
