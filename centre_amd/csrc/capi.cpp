@@ -758,8 +758,41 @@ extern "C" int centre_dev_pair_entropy(centre_ctx* ctx, const centre_pair_plan* 
     }
     if (!d_mi) a.mi = nullptr;
     if (d_mi && ests[0] > 1) a.mi = nullptr;  // MI for CS/JS computed below
+    // Frame split for plans with few tiles at small nb (ML / MM only): the
+    // four-wave kernel is bound by one-hot staging there, so K workgroups per
+    // tile each count a K-th of every step into their own count dump and a
+    // second kernel sums the dumps and evaluates the entropies.
+    int nsplit = 1;
+    uint64_t split_len = 0;
+    if (!dump && nb <= 16 && plan->n_tiles > 0) {
+        int ncu = 0;
+        HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+        int k = std::min<int>(8, ncu / plan->n_tiles);
+        k = (int)std::min<uint64_t>((uint64_t)k, ss / 512);
+        const uint64_t cbytes = sizeof(uint32_t) * P * nsteps * (size_t)((nb * nb + 3) & ~3);
+        if (k >= 2 && cbytes * k <= ((size_t)1 << 30)) {
+            split_len = ((ss + k - 1) / k + 63) / 64 * 64;   // whole chunks, 4-byte aligned
+            nsplit = (int)((ss + split_len - 1) / split_len);
+        }
+    }
+    DevBuf split_counts;
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-    if (!dump && pair_tile8_supported(a.R) && plan->n_full > 0 &&
+    if (nsplit > 1) {
+        const int rec = (nb * nb + 3) & ~3;
+        HIPCHK(split_counts.alloc(sizeof(uint32_t) * P * nsteps * (size_t)rec * nsplit));
+        PairKernelArgs sp = a;
+        sp.counts = split_counts.as<uint32_t>();
+        sp.nsplit = nsplit;
+        sp.split_tiles = plan->n_tiles;
+        sp.split_len = split_len;
+        sp.split_rec = rec;
+        sp.split_cstride = P * nsteps * (uint64_t)rec;
+        PairKernelArgs dumpargs = sp;   // counts only
+        dumpargs.slot_ml = dumpargs.slot_mm = -1;
+        dumpargs.mi = nullptr;
+        HIPCHK(launch_pair_tiles(dumpargs, plan->n_tiles, true, ctx->stream));
+        HIPCHK(launch_split_entropy(sp, plan->n_tiles, ctx->stream));
+    } else if (!dump && pair_tile8_supported(a.R) && plan->n_full > 0 &&
         ((a.stride | a.sstride | reinterpret_cast<uintptr_t>(a.codes)) & 3) == 0) {
         // full tiles on the eight-wave kernel, those of a partial block after
         HIPCHK(launch_pair_tiles8(a, plan->n_full, ctx->stream));

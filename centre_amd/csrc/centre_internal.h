@@ -84,6 +84,15 @@ struct PairKernelArgs {
     double* mi;             // [P][S] or null
     int mi_from_mm;         // MI uses the MM S2 (when MM is the first estimator)
     uint32_t* counts;       // dump mode: [P][S][nb*nb] uint32 prefix counts
+    // frame split (dump mode only, nsplit > 1): workgroup w counts tile
+    // w % split_tiles over frames [k*split_len, k*split_len + len_k) of every
+    // step, k = w / split_tiles (the last part takes the rest of the step),
+    // into counts + k*split_cstride
+    int nsplit;
+    int split_tiles;
+    uint64_t split_len;
+    uint64_t split_cstride;
+    int split_rec;          // words per (pair, step) count record; 0 = nb*nb
     int* flag;              // error word: bit0 = code >= nbins
     // batch (persistent launch): per-entry system of the tile list
     const BatchSystem* systems;   // null = single system in the fields above
@@ -108,6 +117,10 @@ hipError_t launch_pair_tiles8(const PairKernelArgs& a, int n_tiles, hipStream_t 
 hipError_t launch_pair_batch8(const PairKernelArgs& a, hipStream_t st);
 hipError_t launch_pair_tiles(const PairKernelArgs& a, int n_tiles, bool dump,
                              hipStream_t st);
+// S2 (ML / MM slots) and MI of every pair of n_tiles tiles from the
+// a.nsplit partial count dumps at a.counts (stride a.split_cstride), with the
+// same formulas as the fused step epilogue
+hipError_t launch_split_entropy(const PairKernelArgs& a, int n_tiles, hipStream_t st);
 size_t pair_tile_lds_bytes(int R);
 hipError_t launch_pair_persistent(const PairKernelArgs& a, int grid, hipStream_t st);
 hipError_t launch_repack(const uint8_t* codes, uint64_t stride, uint32_t D, uint64_t ss, int S,

@@ -448,8 +448,9 @@ __device__ __forceinline__ void tile_body(const PairKernelArgs& a_in, int tix) {
                                                               : tri_index(gi, gj, D);
                                 if (p >= 0) {
                                     const int bi_ = grow - rd * nb, bj_ = gcol - cd * nb;
-                                    a.counts[((size_t)p * S + s) * nb * nb +
-                                             (size_t)bi_ * nb + bj_] = (uint32_t)cnt;
+                                    const int rec = a.split_rec ? a.split_rec : nb * nb;
+                                    a.counts[((size_t)p * S + s) * rec + (size_t)bi_ * nb + bj_] =
+                                        (uint32_t)cnt;
                                 }
                             }
                         }
@@ -745,7 +746,72 @@ __device__ __forceinline__ void tile_body(const PairKernelArgs& a_in, int tix) {
 
 template <bool kDump, int kSlots>
 __global__ __launch_bounds__(256) void pair_tile_kernel(PairKernelArgs a) {
-    tile_body<kDump, kSlots, false>(a, blockIdx.x);
+    int tix = blockIdx.x;
+    if constexpr (kDump) {
+        if (a.nsplit > 1) {
+            const int k = tix / a.split_tiles;
+            tix -= k * a.split_tiles;
+            a.codes += (uint64_t)k * a.split_len;
+            a.ss = (k == a.nsplit - 1) ? a.ss - (uint64_t)k * a.split_len : a.split_len;
+            a.counts += (size_t)k * a.split_cstride;
+        }
+    }
+    tile_body<kDump, kSlots, false>(a, tix);
+}
+
+// Workgroups (tile, slice of 256 (pair, step) items), one thread per item: sum the partial
+// counts, then S2 = log n + log(w_i w_j) - sum c log c / n + (L_i + L_j) / n,
+// the MM correction (nonzero bins - 1) / 2n and MI = S1_i + S1_j - S2, as the
+// step epilogue of the tile kernels computes them.
+__global__ __launch_bounds__(256) void split_entropy_kernel(PairKernelArgs a) {
+    const int tix = blockIdx.x;
+    const int R = a.R, nb = a.nb, S = a.nsteps;
+    const int2 tile = a.tiles[tix];
+    const int r0 = tile.x * R, c0 = tile.y * R;
+    const int nr = min(R, a.n_dof - r0), nc = min(R, a.n_dof - c0);
+    const long long D = a.n_dof;
+    const int nbb = nb * nb;
+    const int item = blockIdx.y * 256 + threadIdx.x;
+    if (item < nr * nc * S) {
+        const int q = item / S, s = item - q * S;
+        const int rd = q / nc, cd = q - rd * nc;
+        const long long gi = r0 + rd, gj = c0 + cd;
+        if (gi >= gj) return;
+        const long long p = a.tilemap ? (long long)a.tilemap[(size_t)tix * R * R + rd * R + cd]
+                                      : tri_index(gi, gj, D);
+        if (p < 0) return;
+        // records of split_rec words (nb^2 rounded up to a multiple of 4,
+        // 16-byte aligned); the padding words are never written
+        const uint4* c0p = reinterpret_cast<const uint4*>(a.counts + ((size_t)p * S + s) * a.split_rec);
+        const size_t kstride = a.split_cstride / 4;
+        double T = 0.0;
+        int nz = 0;
+        for (int b4 = 0; b4 < a.split_rec / 4; ++b4) {
+            uint4 c = c0p[b4];
+            for (int k = 1; k < a.nsplit; ++k) {
+                const uint4 v = c0p[(size_t)k * kstride + b4];
+                c.x += v.x; c.y += v.y; c.z += v.z; c.w += v.w;
+            }
+            const uint32_t cv[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                if (4 * b4 + e >= nbb) break;
+                nz += cv[e] > 0;
+                if (cv[e] > 1) T += (double)cv[e] * log((double)cv[e]);
+            }
+        }
+        const double n = (double)(s + 1) * (double)a.ss;
+        const double L = a.dof_L[gi * S + s] + a.dof_L[gj * S + s];
+        const double area = a.dof_w[gi] * a.dof_w[gj];
+        const double ml = (log(n) + log(area)) - T / n + L / n;
+        const double mm = ml + (double)(unsigned)(nz - 1) / (2.0 * n);
+        if (a.slot_ml >= 0) a.s2[((size_t)p * a.nest + a.slot_ml) * S + s] = ml;
+        if (a.slot_mm >= 0) a.s2[((size_t)p * a.nest + a.slot_mm) * S + s] = mm;
+        if (a.mi) {
+            const double s1 = a.dof_S1[gi * S + s] + a.dof_S1[gj * S + s];
+            a.mi[(size_t)p * S + s] = s1 - (a.mi_from_mm ? mm : ml);
+        }
+    }
 }
 
 // Persistent batch: every workgroup takes tile list entries from a shared
@@ -1141,7 +1207,15 @@ static hipError_t launch_t(const PairKernelArgs& a, int n_tiles, size_t lds, hip
         reinterpret_cast<const void*>(&pair_tile_kernel<kDump, kSlots>),
         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((pair_tile_kernel<kDump, kSlots>), dim3(n_tiles), dim3(256), lds, st, a);
+    const int grid = (kDump && a.nsplit > 1) ? n_tiles * a.nsplit : n_tiles;
+    hipLaunchKernelGGL((pair_tile_kernel<kDump, kSlots>), dim3(grid), dim3(256), lds, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_split_entropy(const PairKernelArgs& a, int n_tiles, hipStream_t st) {
+    if (n_tiles <= 0) return hipSuccess;
+    const int ny = (a.R * a.R * a.nsteps + 255) / 256;
+    hipLaunchKernelGGL(split_entropy_kernel, dim3(n_tiles, ny), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 

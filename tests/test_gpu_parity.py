@@ -355,3 +355,35 @@ def test_count_materialisation_limit_is_an_error(ctx):
         ctx.pair_entropy(codes, np.ones(D, np.uint8), np.tile([1.0, 2.0, 0, 0], (D, 1)), nb, S,
                          ests=(2,), want_s1=False, want_mi=False)
     assert e.value.code == -3
+
+
+@pytest.mark.parametrize("nb,D,F,S,ests,explicit", [
+    (5, 40, 20000, 3, (0,), False),      # one tile, eight parts, short last part
+    (16, 50, 9000, 2, (1, 0), True),     # ten tiles, MI from MM
+    (3, 20, 4096, 4, (0, 1), True),      # two parts of 512 frames
+])
+def test_frame_split_vs_oracle(ctx, nb, D, F, S, ests, explicit):
+    """Plans with few tiles at small nb split every step's frames over several
+    workgroups per tile and sum their count dumps before the entropies."""
+    rng = np.random.default_rng(nb * 31 + D)
+    codes = rng.integers(0, nb, size=(D, F), dtype=np.uint8)
+    codes[1::2] = (codes[0::2][: len(codes[1::2])] + rng.integers(0, 2, size=F)) % nb
+    types = rng.integers(1, 4, size=D).astype(np.uint8)
+    ext = np.zeros((D, 4))
+    ext[:, 0] = rng.uniform(0.5, 1.0, D)
+    ext[:, 1] = ext[:, 0] + rng.uniform(0.2, 1.5, D)
+    iu, ju = np.triu_indices(D, 1)
+    pairs = np.stack([iu, ju], 1)
+    if explicit:
+        pick = rng.choice(len(pairs), len(pairs) // 3, replace=False)
+        pairs = pairs[pick]
+        flip = rng.random(len(pairs)) < 0.4
+        pairs[flip] = pairs[flip][:, ::-1]
+    print(f"frame split nb={nb} D={D} F={F} S={S} ests={ests} explicit={explicit}", flush=True)
+    s1, s2, mi = ctx.pair_entropy(codes, types, ext, nb, S, pairs=pairs if explicit else None,
+                                  ests=ests)
+    w1 = O.dof_entropies(codes, types, ext, S, [nb], ests)
+    w2 = O.pair_entropies(codes, types, ext, pairs, S, [nb], ests)
+    assert rel_err(s2, w2[:, :, 0, :]) < REL
+    mi_w = w1[pairs[:, 0], 0, 0, :] + w1[pairs[:, 1], 0, 0, :] - w2[:, 0, 0, :]
+    assert np.max(np.abs(mi - mi_w)) < 1e-10
