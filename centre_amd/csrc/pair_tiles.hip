@@ -865,15 +865,23 @@ __device__ __forceinline__ void epi_block8(const v16i& blk, int row0, int gcol, 
         for (int q = 0; q < 4; ++q) {
             int c[4];
             double tv[4];
+            // counts 0 and 1 add nothing: only lanes with c > 1 read the
+            // table, which thins the bank traffic of the lookups
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 c[t] = ok ? blk[4 * q + t] : 0;
-                tv[t] = sm.ctab[min(c[t], ntab - 1)];
+                tv[t] = 0.0;
+                if (c[t] > 1) tv[t] = sm.ctab[min(c[t], ntab - 1)];
+            }
+            const int mx = max(max(c[0], c[1]), max(c[2], c[3]));
+            if (__builtin_amdgcn_ballot_w64(mx >= ntab) != 0) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    if (c[t] >= ntab)
+                        tv[t] = (double)c[t] * log_pos_lds((double)c[t], sm.logt, sm.invt, sm.invt + 256);
             }
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
-                if (__builtin_amdgcn_ballot_w64(c[t] >= ntab) != 0 && c[t] >= ntab)
-                    tv[t] = (double)c[t] * log_pos_lds((double)c[t], sm.logt, sm.invt, sm.invt + 256);
                 const bool h = 8 * q + t >= kk;
                 const int nz = min(c[t], 1);
                 tot += tv[t];
