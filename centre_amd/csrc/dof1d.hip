@@ -2,11 +2,12 @@
 //
 // dof1d_kernel replaces binData1D + entropy1D (Estimators.cpp:56-118,
 // 217-389) for all DOFs at once: one workgroup per DOF histograms each step
-// chunk in LDS, keeps the prefix counts, and one lane evaluates every
-// requested estimator in the reference's loop order so S1 matches the
-// source's arithmetic term for term.  It also leaves, per DOF and step, the
-// marginal Jacobian sum L = sum_b m_b log J_b and the bin width that the
-// pair epilogue needs.
+// chunk in LDS, keeps the prefix counts, and evaluates every requested
+// estimator with bin b on thread b: the reference's per-bin terms, summed in
+// a fixed tree order (so S1 is the same from run to run and within a few ulp
+// of the sequential sum).  It also leaves, per DOF and step, the marginal
+// Jacobian sum L = sum_b m_b log J_b and the bin width that the pair
+// epilogue needs.
 //
 // count_entropy_kernel evaluates entropy2D (Estimators.cpp:391-651) for any
 // estimator from prefix counts written by the pair kernel's dump mode: one
@@ -26,53 +27,70 @@ __device__ inline double jac(int type, double mid) {
     return 1.0;
 }
 
-// entropy1D for one step and one estimator.  fr: scratch [nb].
-__device__ double ent1d(int type, int est, int nb, const uint64_t* f, double n,
-                        double w, const double* mid, int jacobian, double* fr) {
-    for (int j = 0; j < nb; ++j) fr[j] = f[j] / n;
-    double sum = 0.0;
-    uint8_t blank = 0, single = 0;
+// Sum over the 256 threads of a workgroup in a fixed order (wave shuffles,
+// then the four wave totals in order), so the result does not vary from run
+// to run; every thread gets the total.  red: 4 doubles of LDS.
+__device__ inline double block_sum256(double v, double* red) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    __syncthreads();   // red is free (previous use finished)
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// entropy1D for one step and one estimator with bin b on thread b (threads
+// b >= nb contribute nothing).  The bin statistics the estimator needs
+// (empty and singleton bins, the James-Stein sums) are block sums first;
+// then the per-bin terms and their block sum.  Same terms as the
+// reference's loops (Estimators.cpp:217-389), summed in tree order.
+__device__ double ent1d_par(int type, int est, int nb, uint64_t fb, bool live, double n,
+                            double w, double midb, int jacobian, double* red) {
+    double frb = live ? fb / n : 0.0;
+    int blank = 0;
     if (est == kMM) {
-        for (int j = 0; j < nb; ++j)
-            if (f[j] == 0) ++blank;
+        blank = (int)block_sum256((live && fb == 0) ? 1.0 : 0.0, red);
     } else if (est == kCS) {
-        for (int j = 0; j < nb; ++j)
-            if (f[j] == 1) ++single;
-        for (int j = 0; j < nb; ++j) fr[j] = (1 - (single / n)) * (f[j] / n);
+        const double single = block_sum256((live && fb == 1) ? 1.0 : 0.0, red);
+        frb = live ? (1 - (single / n)) * (fb / n) : 0.0;
     } else if (est == kJS) {
         const double tk = 1.0 / (double)nb;
-        double s2 = 0, tks = 0;
-        for (int j = 0; j < nb; ++j) {
-            s2 += fr[j] * fr[j];
-            tks += ((tk - fr[j]) * (tk - fr[j]));
-        }
+        const double s2 = block_sum256(frb * frb, red);
+        const double tks = block_sum256(live ? (tk - frb) * (tk - frb) : 0.0, red);
         const double lam = (1 - s2) / ((n - 1) * tks);
-        for (int j = 0; j < nb; ++j) fr[j] = (lam * tk) + ((1 - lam) * fr[j]);
+        frb = live ? (lam * tk) + ((1 - lam) * frb) : 0.0;
     }
     const bool hasj = (type == 1 || type == 2);
-    for (int j = 0; j < nb; ++j) {
-        double pd = fr[j] / w;
-        if (pd > 0.0) {
-            const double denom = (est == kCS) ? 1 - pow((1 - fr[j]), n) : 1.0;
-            if (jacobian && hasj) {
-                const double jf = jac(type, mid[j]);
-                pd = pd / jf;
-                sum += (est == kCS) ? (jf * pd * log(pd) / denom) : (jf * pd * log(pd));
-            } else {
-                sum += (est == kCS) ? (pd * log(pd) / denom) : (pd * log(pd));
-            }
+    double term = 0.0;
+    double pd = frb / w;
+    if (live && pd > 0.0) {
+        const double denom = (est == kCS) ? 1 - pow((1 - frb), n) : 1.0;
+        if (jacobian && hasj) {
+            const double jf = jac(type, midb);
+            pd = pd / jf;
+            term = (est == kCS) ? (jf * pd * log(pd) / denom) : (jf * pd * log(pd));
+        } else {
+            term = (est == kCS) ? (pd * log(pd) / denom) : (pd * log(pd));
         }
     }
+    double sum = block_sum256(term, red);
     sum *= -w;
     if (est == kMM) sum += ((nb - blank) - 1) / (2.0 * n);
     return sum;
 }
 
+// Up to kPrivBins bins each thread counts into its own LDS row (an odd row
+// length keeps the 64 lanes of a wave in different banks); more bins share
+// one histogram through LDS atomics, which collide less the more bins there
+// are.
+constexpr int kPrivBins = 32;
+
 __global__ __launch_bounds__(256) void dof1d_kernel(Dof1dArgs a) {
     __shared__ uint32_t h[256];
+    __shared__ uint32_t hp[256 * (kPrivBins + 1)];
     __shared__ uint64_t cum[256];
-    __shared__ double fr[256];
     __shared__ double mid[256];
+    __shared__ double red[4];
     const int d = blockIdx.x, tid = threadIdx.x;
     const int nb = a.nb, S = a.nsteps;
     const uint8_t* row = a.codes + (uint64_t)d * a.stride;
@@ -86,31 +104,65 @@ __global__ __launch_bounds__(256) void dof1d_kernel(Dof1dArgs a) {
         cum[b] = 0;
         mid[b] = (hasj && b < nb) ? mn + b * w + hw : 0.0;
     }
+    const bool priv = nb <= kPrivBins;
+    const int nbp = nb | 1;                        // private row length, odd
+    uint32_t* const mine = hp + tid * nbp;
+    auto add = [&](uint32_t c) __attribute__((always_inline)) {
+        if (c < (uint32_t)nb) {
+            if (priv) atomicAdd(&mine[c], 1u);     // private: never contended
+            else atomicAdd(&h[c], 1u);
+        } else {
+            atomicOr(a.flag, 1);
+        }
+    };
     for (int s = 0; s < S; ++s) {
-        for (int b = tid; b < 256; b += 256) h[b] = 0;
-        __syncthreads();
-        const uint64_t f0 = (uint64_t)s * a.ss, f1 = f0 + a.ss;
-        for (uint64_t f = f0 + tid; f < f1; f += 256) {
-            const uint32_t c = row[f];
-            if (c < (uint32_t)nb) atomicAdd(&h[c], 1u);
-            else atomicOr(a.flag, 1);
+        if (priv) {
+            for (int b = 0; b < nbp; ++b) mine[b] = 0;
+        } else {
+            h[tid] = 0;
         }
         __syncthreads();
+        // the step's codes: bytes up to a 4-byte boundary, dwords, then the
+        // remaining bytes
+        const uint8_t* const p = row + (uint64_t)s * a.ss;
+        const uint64_t len = a.ss;
+        const uint64_t head = min((uint64_t)((4 - (reinterpret_cast<uintptr_t>(p) & 3)) & 3), len);
+        if ((uint64_t)tid < head) add(p[tid]);
+        const uint32_t* const p4 = reinterpret_cast<const uint32_t*>(p + head);
+        const uint64_t n4 = (len - head) >> 2;
+        for (uint64_t i = tid; i < n4; i += 256) {
+            const uint32_t v = p4[i];
+            add(v & 0xFFu);
+            add((v >> 8) & 0xFFu);
+            add((v >> 16) & 0xFFu);
+            add(v >> 24);
+        }
+        for (uint64_t f = head + 4 * n4 + tid; f < len; f += 256) add(p[f]);
+        __syncthreads();
         if (tid < nb) {
-            cum[tid] += h[tid];
+            uint32_t tot = 0;
+            if (priv) {
+                for (int t = 0; t < 256; ++t) tot += hp[t * nbp + tid];
+            } else {
+                tot = h[tid];
+            }
+            cum[tid] += tot;
             if (a.counts) a.counts[((uint64_t)d * S + s) * nb + tid] = cum[tid];
         }
         __syncthreads();
-        if (tid == 0) {
+        {   // every estimator and the Jacobian sum, bin b on thread b
             const double n = (s + 1.0) * a.ss;
-            for (int e = 0; e < a.nest; ++e)
-                a.s1[((uint64_t)d * a.nest + e) * S + s] =
-                    ent1d(type, a.ests[e], nb, cum, n, w, mid, a.jacobian, fr);
+            const bool live = tid < nb;
+            const uint64_t fb = live ? cum[tid] : 0;
+            const double midb = live ? mid[tid] : 0.0;
+            for (int e = 0; e < a.nest; ++e) {
+                const double v = ent1d_par(type, a.ests[e], nb, fb, live, n, w, midb, a.jacobian, red);
+                if (tid == 0) a.s1[((uint64_t)d * a.nest + e) * S + s] = v;
+            }
             double L = 0.0;
             if (a.jacobian && hasj)
-                for (int b = 0; b < nb; ++b)
-                    if (cum[b]) L += (double)cum[b] * log(jac(type, mid[b]));
-            a.L[(uint64_t)d * S + s] = L;
+                L = block_sum256((live && fb) ? (double)fb * log(jac(type, midb)) : 0.0, red);
+            if (tid == 0) a.L[(uint64_t)d * S + s] = L;
         }
         __syncthreads();
     }
