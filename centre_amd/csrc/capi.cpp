@@ -445,36 +445,78 @@ struct BatchTile {
 
 // order: 1 = longest tiles first; 2 = system by system (row-major tiles);
 // 3 = systems by total cost, largest first, each system's tiles together.
+// Orders 1 and 3 are stable sorts of the row-major list; both reduce to
+// ordering groups of equal-cost tiles (a system's diagonal tiles, its
+// off-diagonal tiles) and emitting each group in row-major order, which is
+// linear in the number of tiles.  sample < 1 keeps an evenly spread share of
+// each system's tiles (calibration) without generating the rest.
 std::vector<BatchTile> batch_tiles(const centre_batch_system* sys, int n_sys, int R, int nsteps,
                                    int order, uint64_t max_frames_per_step, double sample) {
-    std::vector<BatchTile> out;
-    std::vector<double> sys_cost(n_sys, 0.0);
+    struct Group {
+        int sys;
+        bool diag;
+        double cost;   // per tile
+    };
+    std::vector<int> nblk(n_sys);
+    std::vector<double> frames(n_sys), sys_cost(n_sys, 0.0);
+    std::vector<uint64_t> ntile(n_sys), nkeep(n_sys);
+    size_t total = 0;
     for (int k = 0; k < n_sys; ++k) {
-        const int nblk = (int)((sys[k].n_dof + R - 1) / R);
-        const double frames =
-            (double)std::min<uint64_t>(sys[k].n_frame / nsteps, max_frames_per_step) * nsteps;
-        const size_t first = out.size();
-        for (int I = 0; I < nblk; ++I)
-            for (int J = I; J < nblk; ++J)
-                out.push_back({I, J, k, frames * (I == J ? 0.5 : 1.0)});
-        if (sample < 1.0) {   // calibration: an evenly spread share of the system's tiles
-            std::vector<BatchTile> keep;
-            const size_t n = out.size() - first;
-            const size_t m = std::max<size_t>(1, (size_t)(n * sample));
-            for (size_t q = 0; q < m; ++q) keep.push_back(out[first + q * n / m]);
-            out.resize(first);
-            out.insert(out.end(), keep.begin(), keep.end());
-        }
-        for (size_t q = first; q < out.size(); ++q) sys_cost[k] += out[q].cost;
+        nblk[k] = (int)((sys[k].n_dof + R - 1) / R);
+        frames[k] = (double)std::min<uint64_t>(sys[k].n_frame / nsteps, max_frames_per_step) * nsteps;
+        ntile[k] = (uint64_t)nblk[k] * (nblk[k] + 1) / 2;
+        nkeep[k] = sample < 1.0 ? std::max<uint64_t>(1, (uint64_t)(ntile[k] * sample)) : ntile[k];
+        total += nkeep[k];
     }
-    if (order == 1)
-        std::stable_sort(out.begin(), out.end(),
-                         [](const BatchTile& x, const BatchTile& y) { return x.cost > y.cost; });
-    else if (order == 3)
-        std::stable_sort(out.begin(), out.end(), [&](const BatchTile& x, const BatchTile& y) {
-            return sys_cost[x.sys] > sys_cost[y.sys] ||
-                   (sys_cost[x.sys] == sys_cost[y.sys] && x.sys < y.sys);
+    // tiles of system k, in row-major order, that the sample keeps; with
+    // only = 0 / 1 just the off-diagonal / diagonal ones
+    auto emit = [&](int k, int only, std::vector<BatchTile>& out) {
+        const uint64_t n = ntile[k], m = nkeep[k];
+        uint64_t lin = 0, q = 0, next = 0;   // next kept linear index: q * n / m
+        for (int I = 0; I < nblk[k] && q < m; ++I)
+            for (int J = I; J < nblk[k] && q < m; ++J, ++lin) {
+                if (lin != next) continue;
+                ++q;
+                next = q * n / m;
+                const bool d = I == J;
+                if (only < 0 || (only == 1) == d)
+                    out.push_back({I, J, k, frames[k] * (d ? 0.5 : 1.0)});
+            }
+    };
+    // system cost of the kept tiles
+    {
+        std::vector<BatchTile> tmp;
+        for (int k = 0; k < n_sys; ++k) {
+            if (sample < 1.0) {
+                tmp.clear();
+                emit(k, -1, tmp);
+                for (const BatchTile& t : tmp) sys_cost[k] += t.cost;
+            } else {
+                sys_cost[k] = frames[k] * ((double)ntile[k] - 0.5 * nblk[k]);
+            }
+        }
+    }
+    std::vector<BatchTile> out;
+    out.reserve(total);
+    if (order == 1) {
+        std::vector<Group> g;
+        for (int k = 0; k < n_sys; ++k) {
+            g.push_back({k, false, frames[k]});
+            g.push_back({k, true, 0.5 * frames[k]});
+        }
+        // equal costs keep the row-major list's order: system, then position
+        std::stable_sort(g.begin(), g.end(), [](const Group& x, const Group& y) {
+            return x.cost > y.cost || (x.cost == y.cost && x.sys < y.sys);
         });
+        for (const Group& q : g) emit(q.sys, q.diag ? 1 : 0, out);
+    } else {
+        std::vector<int> ord(n_sys);
+        std::iota(ord.begin(), ord.end(), 0);
+        if (order == 3)
+            std::stable_sort(ord.begin(), ord.end(),
+                             [&](int x, int y) { return sys_cost[x] > sys_cost[y]; });
+        for (int k : ord) emit(k, -1, out);
+    }
     return out;
 }
 
@@ -630,17 +672,19 @@ extern "C" int centre_batch_pair_entropy(centre_ctx* ctx, const centre_batch_sys
     int dev_cu = 0;
     HIPCHK(hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, ctx->device));
     const int grid = dev_cu;   // one workgroup per CU: the tile's LDS fills most of a CU
-    // autotune: time each order on a calibration share of the batch
+    // autotune: time each order on a calibration share of the batch (2% of
+    // each system's tiles, at most 1,024 frames per step: about 0.6% of the
+    // full run's work for the three orders together)
     int chosen = order;
     if (order == 0) {
         double best = 1e300;
         for (int cand = 1; cand <= 3; ++cand) {
-            auto tl = batch_tiles(sys, n_sys, base.R, nsteps, cand, 4096, 0.05);
+            auto tl = batch_tiles(sys, n_sys, base.R, nsteps, cand, 1024, 0.02);
             // calibration outputs land in the real output buffers and are
             // overwritten by the full run below
             BatchRun run;
             std::vector<BatchSystem> cbs = bs;
-            for (auto& c : cbs) c.ss = std::min<uint64_t>(c.ss, 4096);
+            for (auto& c : cbs) c.ss = std::min<uint64_t>(c.ss, 1024);
             if (int r = batch_launch(ctx, base, cbs, tl, grid, &run)) return r;
             if (run.total_ms < best) {
                 best = run.total_ms;
