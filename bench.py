@@ -9,8 +9,8 @@ in HBM: marginal counts and S1, the one-hot int8 MFMA joint counts with the
 fused FP64 entropy epilogue, and MI.  Discretisation and MIST run once
 outside the timed region and are reported as separate stage times.
 
-Prints ONE JSON line (rank 0).  --gpus N > 1 is refused: this build targets
-a single GPU.
+Prints ONE JSON line (rank 0).  --gpus N > 1 is refused: the library runs
+on a single GPU.
 """
 from __future__ import annotations
 
@@ -78,7 +78,7 @@ def traffic_from_profile(workload_is_headline: bool):
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU only (the pair stage "
+        print("bench.py: the library runs on one GPU only (the pair stage "
               "shards by pair tiles across GPUs by design, but no multi-GPU "
               "path is built); run with --gpus 1", file=sys.stderr)
         sys.exit(2)

@@ -146,7 +146,7 @@ Inputs inputs_from_symbols(const Symbols& sym) {
                 in.estimators.push_back(e);
         }
     }
-    // options of this build
+    // options of the MI355X host (no counterpart in the reference's cfg)
     i1("mi355x.threadsfororder", in.threads_for_order);
     i1("mi355x.device", in.device);
 
