@@ -287,6 +287,31 @@ __device__ __forceinline__ void tile_body(const PairKernelArgs& a_in, int tix) {
                 const bool isA = ld < nr;
                 const int rbase = (isA ? ld : ld - nr) * nb;
                 uint8_t* op = sm.oh + (size_t)(b * 2 + (isA ? 0 : 1)) * kOpBytes + k4;
+                if (nb <= 8) {   // workgroup-uniform
+                    // Whole dwords: the slot's 4 frames in each of the DOF's
+                    // nb rows are rewritten, so nothing from two chunks ago
+                    // needs clearing.  Byte e of row r is 1 where code e == r
+                    // (exact per-byte zero test of code ^ r, no carries
+                    // between bytes); frames outside the chunk read as 0xFF,
+                    // which matches no row.
+                    const int nv = nvalid - k4;
+                    const uint32_t vm = nv >= 4 ? 0xFFFFFFFFu : (nv <= 0 ? 0u : 0xFFFFFFFFu >> (8 * (4 - nv)));
+                    const uint32_t wx = wv | ~vm;
+                    bool bad = false;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        bad |= (e < nv) && ((wx >> (8 * e)) & 0xFFu) >= (uint32_t)nb;
+#pragma unroll
+                    for (int r = 0; r < 8; ++r) {
+                        if (r < nb) {
+                            const uint32_t x = wx ^ (0x01010101u * (uint32_t)r);
+                            const uint32_t z = ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x | 0x7F7F7F7Fu);
+                            *reinterpret_cast<uint32_t*>(op + (rbase + r) * kLdsRow) = z >> 7;
+                        }
+                    }
+                    if (bad) atomicOr(a.flag, 1);
+                    continue;
+                }
                 uint32_t* slot = sm.cl + ((size_t)b * 2 * R + ld) * kGroups + (k4 >> 2);
                 const uint32_t old = *slot;
 #pragma unroll
