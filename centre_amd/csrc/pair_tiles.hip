@@ -371,6 +371,8 @@ __device__ __forceinline__ void tile_body(const PairKernelArgs& a_in, int tix) {
     // park: 4 KB of LDS per wave where one block's counts are parked so the
     // accumulators are read once, straight from AGPRs; with rezero_g >= 0 the
     // park area lies in buffer rezero_g & 1, which is cleared afterwards.
+    // S2 / MI wanted from this launch (a count-only dump wants neither)
+    const bool fused = a.slot_ml >= 0 || a.slot_mm >= 0 || a.mi != nullptr;
     auto epilogue = [&](int s, int* park, int rezero_g) __attribute__((always_inline)) {
         // One 32x32 block: the lane holds column (lane & 31) and rows
         // 8*(r/4) + 4*(lane/32) + r%4 for r = 0..15.
@@ -450,7 +452,8 @@ __device__ __forceinline__ void tile_body(const PairKernelArgs& a_in, int tix) {
                 return;
             }
             if (cd == 0xFF) return;
-            // general: one LDS atomic per nonzero count
+            // general: one LDS atomic per nonzero count (none when the launch
+            // only dumps counts)
 #pragma unroll 1
             for (int q = 0; q < 4; ++q) {
                 const v4i c4 = pk[64 * q];
@@ -459,7 +462,7 @@ __device__ __forceinline__ void tile_body(const PairKernelArgs& a_in, int tix) {
                     const int grow = row0 + 8 * q + t;
                     const int rd = sm.row2dof[grow];
                     const int cnt = c4[t];
-                    if (rd != 0xFF && cnt > 0) {
+                    if (fused && rd != 0xFF && cnt > 0) {
                         const double v = cnt < ntab ? sm.ctab[cnt]
                                                     : (double)cnt * log_count(cnt, sm.logt, sm.invt);
                         atomicAdd(&sm.slotT[(rd * R + cd) * nstr + lane % nstr], v);
@@ -494,7 +497,7 @@ __device__ __forceinline__ void tile_body(const PairKernelArgs& a_in, int tix) {
         __syncthreads();
         const double n = (double)(s + 1) * (double)a.ss;
         const double logn = log(n);
-        for (int q = tid; q < nr * nc; q += 256) {
+        for (int q = fused ? tid : nr * nc; q < nr * nc; q += 256) {
             const int rd = q / nc, cd = q - rd * nc;
             const long long gi = r0 + rd, gj = c0 + cd;
             if (gi < gj) {
