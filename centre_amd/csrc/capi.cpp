@@ -235,11 +235,38 @@ int centre_dev_dof_entropy(centre_ctx* ctx, const uint8_t* d_codes,
 // Tiles with a full block of R DOFs on both sides go first: the eight-wave
 // kernel takes them; tiles of the last, partial block follow.  Returns how
 // many lead.
+// Launch order of the first n tiles for the L2s of the 8 XCDs, which take
+// workgroups round-robin (workgroup w runs on XCD w % 8).  The tiles are put
+// in 4 x 8 blocks of (row block, column block), the blocked list is cut into
+// 8 contiguous runs, and run x goes to positions x, x + 8, x + 16, ...: the
+// ~32 tiles an XCD has in flight then share 4 row and 8 column code blocks
+// instead of one row block and 32 column blocks.  Results do not depend on
+// the order.
+static void xcd_blocked_order(std::vector<int2>& t, size_t n) {
+    constexpr size_t kXcd = 8;
+    constexpr int kBI = 4, kBJ = 8;
+    if (n < 2 * kXcd) return;
+    std::stable_sort(t.begin(), t.begin() + n, [](const int2& a, const int2& b) {
+        if (a.x / kBI != b.x / kBI) return a.x / kBI < b.x / kBI;
+        if (a.y / kBJ != b.y / kBJ) return a.y / kBJ < b.y / kBJ;
+        return a.x != b.x ? a.x < b.x : a.y < b.y;
+    });
+    std::vector<int2> out(n);
+    const size_t per = n / kXcd, rem = n % kXcd;
+    for (size_t x = 0; x < kXcd; ++x) {
+        const size_t start = x * per + std::min(x, rem), cnt = per + (x < rem ? 1 : 0);
+        for (size_t q = 0; q < cnt; ++q) out[q * kXcd + x] = t[start + q];
+    }
+    std::copy(out.begin(), out.end(), t.begin());
+}
+
 static int full_tiles_first(std::vector<int2>& tiles, uint32_t n_dof, int R) {
     const int nfull = (int)(n_dof / (uint32_t)R);   // blocks 0..nfull-1 are full
     auto full = [&](const int2& t) { return t.x < nfull && t.y < nfull; };
     auto mid = std::stable_partition(tiles.begin(), tiles.end(), full);
-    return (int)(mid - tiles.begin());
+    const int n = (int)(mid - tiles.begin());
+    xcd_blocked_order(tiles, (size_t)n);
+    return n;
 }
 
 int centre_pair_dofs_per_tile(int nbins) {
