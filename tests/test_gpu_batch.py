@@ -27,7 +27,7 @@ def make_system(ctx, seed, nbd, nan, ndh, F, nb):
     return dict(D=D, F=F, codes=d_codes, ext=d_ext, types=ctx.to_device(types))
 
 
-@pytest.mark.parametrize("nb,ests", [(50, (0,)), (30, (1, 0))])
+@pytest.mark.parametrize("nb,ests", [(50, (0,)), (30, (1, 0)), (5, (0, 1))])
 def test_batch_matches_single_system_path(nb, ests):
     ctx = hip.Context(0)
     S = 3
