@@ -318,6 +318,33 @@ def test_vmkde_grid_and_states(ctx):
             assert np.array_equal(codes[d], wc)
 
 
+def test_vmkde_eight_states():
+    """An eight-mode torsion mixture: eight minima descend on eight lanes of
+    the state kernel (nmaxconf 9), on even grids and an odd one."""
+    rng = np.random.default_rng(77)
+    F, D = 4000, 3
+    centers = np.arange(8) * (np.pi / 4) - np.pi + 0.2
+    bat = np.zeros((F, D))
+    for d in range(D):
+        k = rng.integers(0, 8, F)
+        bat[:, d] = np.angle(np.exp(1j * (centers[k] + rng.normal(0, 0.08, F))))
+    with hip.Context(0) as ctx:
+        for G, kappa in ((360, 10.0), (120, 6.0), (91, 2.0)):
+            print(f"vmkde eight states G={G} kappa={kappa}", flush=True)
+            codes, mins, nf, grid = ctx.discretize_vmkde(bat, 9, kappa, 5, 1e-4, 1000, G,
+                                                         want_grid=True)
+            for d in range(D):
+                x = bat[:, d].astype(np.float32)
+                wg = O.vmkde_grid(x, kappa, G)
+                g = grid[d].reshape(-1)
+                scale = np.max(np.abs(wg[: G + 1]))
+                assert np.max(np.abs(g[: G + 1] - wg[: G + 1])) <= 1e-11 * scale
+                wc, wm, wn = O.vmkde_states(x, g, 9, 5, 1e-4, 1000, G)
+                assert wn >= 6 and nf[d] == wn
+                assert np.array_equal(mins[d, :wn], wm)
+                assert np.array_equal(codes[d], wc)
+
+
 # ---------------------------------------------------------------------------
 # edge cases: one bin, one frame per step, two DOFs, empty pair lists, the
 # count-materialisation limit
