@@ -276,6 +276,24 @@ def test_mst_vs_oracle(ctx, seed):
     assert got.tolist() == kruskal_set(n, src.tolist(), dst.tolist(), w.tolist())
 
 
+def test_mst_neighbour_graph_scale(ctx):
+    """The neighbour-list graph of the vmKDE workload's size (3,000 nodes,
+    pairs within 30 of each other, ~90k edges) with weights from a small set
+    (heavy ties): the same forest as the oracle's Boruvka and Kruskal."""
+    rng = np.random.default_rng(99)
+    n = 3000
+    i, j = np.triu_indices(n, 1)
+    keep = (j - i) <= 30
+    src, dst = i[keep], j[keep]
+    w = rng.integers(0, 50, len(src)).astype(float) / 7.0
+    print(f"mst n={n} m={len(src)}", flush=True)
+    got = ctx.mst(n, src, dst, w)
+    want = O.boruvka(n, src, dst, w)
+    assert len(got) == n - 1
+    assert sorted(got.tolist()) == sorted(want.tolist())
+    assert got.tolist() == kruskal_set(n, src.tolist(), dst.tolist(), w.tolist())
+
+
 def test_mst_endpoint_out_of_range_is_an_error(ctx):
     """An endpoint >= n_node is refused (-3) before any round runs; the
     context stays usable."""
