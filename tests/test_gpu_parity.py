@@ -432,3 +432,57 @@ def test_frame_split_vs_oracle(ctx, nb, D, F, S, ests, explicit):
     assert rel_err(s2, w2[:, :, 0, :]) < REL
     mi_w = w1[pairs[:, 0], 0, 0, :] + w1[pairs[:, 1], 0, 0, :] - w2[:, 0, 0, :]
     assert np.max(np.abs(mi - mi_w)) < 1e-10
+
+
+# ---------------------------------------------------------------------------
+# the headline workload at full size: 600 DOF (200 B / 200 A / 200 T) x
+# 100,000 frames, 50 bins, 10 steps, all 179,700 pairs
+def test_full_size_headline_properties():
+    """At full size the checks are size-independent: the plug-in (ML) MI is a
+    Kullback-Leibler divergence (the bin widths and Jacobians cancel between
+    S1_i + S1_j and S2), so it is >= 0 for every pair and step up to
+    rounding; S2 is symmetric in the pair's order; and an evenly strided
+    sample of pairs matches the oracle's entropy2D within 1e-10."""
+    nbd = nan = ndh = 200
+    D, F, nb, S = 600, 100000, 50, 10
+    P = D * (D - 1) // 2
+    bat = hip.synth_bat(0xCE27E + 1, nbd, nan, ndh, F)
+    types = np.array([1] * nbd + [2] * nan + [3] * ndh, np.uint8)
+    print(f"full size D={D} F={F} nb={nb} S={S}", flush=True)
+    with hip.Context(0) as ctx:
+        d_bat = ctx.to_device(bat)
+        d_codes = ctx.alloc((D, F), np.uint8)
+        d_ext = ctx.alloc((D, 4), np.float64)
+        d_types = ctx.to_device(types)
+        col = 0
+        for t, n in ((1, nbd), (2, nan), (3, ndh)):
+            ctx.dev_discretize_hist(d_bat.ptr + 8 * col, D, n, F, t, [nb], nb, True,
+                                    d_codes.ptr + col * F, d_ext.ptr + 32 * col)
+            col += n
+        ctx.sync()
+        d_bat.free()
+        plan = hip.PairPlan(ctx, D, nb)
+        d_s1 = ctx.alloc((D, 1, S), np.float64)
+        d_s2 = ctx.alloc((P, 1, S), np.float64)
+        d_mi = ctx.alloc((P, S), np.float64)
+        ctx.dev_pair_entropy(plan, d_codes.ptr, F, F, d_types.ptr, d_ext.ptr, S, [0], True,
+                             False, d_s1.ptr, d_s2.ptr, d_mi.ptr)
+        s1, s2, mi = d_s1.numpy(), d_s2.numpy(), d_mi.numpy()
+        codes, ext = d_codes.numpy(), d_ext.numpy()
+        plan.close()
+        for buf in (d_s1, d_s2, d_mi, d_codes, d_ext, d_types):
+            buf.free()
+        # reversed explicit records give the same S2 (ML is symmetric)
+        rng = np.random.default_rng(5)
+        iu, ju = np.triu_indices(D, 1)
+        pick = np.sort(rng.choice(P, 2000, replace=False))
+        rev = np.stack([ju[pick], iu[pick]], 1)
+        _, s2r, _ = ctx.pair_entropy(codes, types, ext, nb, S, pairs=rev, ests=(0,))
+    assert np.all(np.isfinite(s2)) and np.all(np.isfinite(mi))
+    assert mi.min() > -1e-11
+    assert rel_err(s2r, s2[pick]) < REL
+    sample = np.arange(0, P, P // 48)[:48]
+    w2 = O.pair_entropies(codes, types, ext, np.stack([iu[sample], ju[sample]], 1), S, [nb], (0,))
+    assert rel_err(s2[sample], w2[:, :, 0, :]) < REL
+    w1 = O.dof_entropies(codes, types, ext, S, [nb], (0,))
+    assert rel_err(s1, w1[:, :, 0, :]) < REL
