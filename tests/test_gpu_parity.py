@@ -486,3 +486,56 @@ def test_full_size_headline_properties():
     assert rel_err(s2[sample], w2[:, :, 0, :]) < REL
     w1 = O.dof_entropies(codes, types, ext, S, [nb], (0,))
     assert rel_err(s1, w1[:, :, 0, :]) < REL
+
+
+# ---------------------------------------------------------------------------
+# the vmKDE workload at full size: 3,000 torsions x 50,000 frames, kappa 10,
+# 64-interval grid, at most 5 states, pairs within 30 DOF, 10 steps
+def test_full_size_vmkde_properties():
+    """Size-independent checks at full size: every torsion's state count is
+    in 0..5 and its codes below it (or all 0); ML MI >= 0 on every
+    neighbour pair and step; a sample of torsions matches the oracle's
+    states given the device grid, and a strided sample of pairs its
+    entropy2D (this plan has 118 tiles, so it runs the frame split)."""
+    D, F, G, kappa, nmax, S, cut = 3000, 50000, 64, 10.0, 5, 10, 30
+    bat = hip.synth_bat(0xCE27E + 7, 0, 0, D, F)
+    ii = np.concatenate([np.full(min(D, i + cut + 1) - i - 1, i) for i in range(D)])
+    jj = np.concatenate([np.arange(i + 1, min(D, i + cut + 1)) for i in range(D)])
+    pairs = np.stack([ii, jj], 1).astype(np.uint32)
+    P = len(pairs)
+    types = np.full(D, 3, np.uint8)
+    print(f"full size vmkde D={D} F={F} P={P}", flush=True)
+    with hip.Context(0) as ctx:
+        d_bat = ctx.to_device(bat)
+        d_codes = ctx.alloc((D, F), np.uint8)
+        d_min = ctx.alloc((D, 9), np.float64)
+        d_nf = ctx.alloc(D, np.uint8)
+        d_grid = ctx.alloc((D, 2, G + 1), np.float64)
+        ctx.dev_discretize_vmkde(d_bat.ptr, D, D, F, nmax, kappa, d_codes.ptr, d_min.ptr,
+                                 d_nf.ptr, grid_intervals=G, d_grid=d_grid.ptr)
+        ctx.sync()
+        d_bat.free()
+        d_types, d_ext = ctx.to_device(types), ctx.to_device(np.zeros((D, 4)))
+        plan = hip.PairPlan(ctx, D, nmax, pairs)
+        d_s1 = ctx.alloc((D, 1, S), np.float64)
+        d_s2 = ctx.alloc((P, 1, S), np.float64)
+        d_mi = ctx.alloc((P, S), np.float64)
+        ctx.dev_pair_entropy(plan, d_codes.ptr, F, F, d_types.ptr, d_ext.ptr, S, [0], True, True,
+                             d_s1.ptr, d_s2.ptr, d_mi.ptr)
+        codes, mins, nf, grid = d_codes.numpy(), d_min.numpy(), d_nf.numpy(), d_grid.numpy()
+        s2, mi = d_s2.numpy(), d_mi.numpy()
+        plan.close()
+        for buf in (d_codes, d_min, d_nf, d_grid, d_types, d_ext, d_s1, d_s2, d_mi):
+            buf.free()
+    assert nf.max() <= nmax
+    assert np.all(codes.max(axis=1) < np.maximum(nf, 1))
+    assert np.all(np.isfinite(s2)) and mi.min() > -1e-11
+    for d in (0, 1, 777, 1500, 2998, 2999):
+        x = bat[:, d].astype(np.float32)
+        wc, wm, wn = O.vmkde_states(x, grid[d].reshape(-1), nmax, 5, 1e-4, 1000, G)
+        assert nf[d] == wn and np.array_equal(mins[d, :wn], wm)
+        assert np.array_equal(codes[d], wc)
+    sample = np.arange(0, P, P // 500)[:500]
+    w2 = O.pair_entropies(codes, types, np.zeros((D, 4)), pairs[sample], S, [nmax], (0,),
+                          jacobian=True, kde=True)
+    assert rel_err(s2[sample], w2[:, :, 0, :]) < REL
