@@ -29,8 +29,10 @@
 // MFMA work) ahead of use so their latency is hidden.  The one-hot rows are
 // written into LDS by setting one byte per (DOF, frame) and clearing the
 // byte the same slot set two chunks earlier, so a chunk costs a few byte
-// writes per thread instead of rebuilding 32 KB of zeros.  Two LDS buffers
-// alternate: chunk g+1 is staged while chunk g feeds the MFMAs.
+// writes per thread instead of rebuilding 32 KB of zeros.  With nb <= 8 the
+// four-wave general path writes each slot's nb rows as whole dwords instead
+// (nothing to clear).  Two LDS buffers alternate: chunk g+1 is staged while
+// chunk g feeds the MFMAs.
 //
 // Waves: 4 per workgroup in a 2x2 layout, each owning a 128x128 quadrant
 // (4x4 MFMA 32x32 blocks = 256 int32 accumulators per lane).  Blocks with no
@@ -42,6 +44,11 @@
 // that one wave's staging and epilogue overlap the other's MFMAs.  The
 // four-wave form keeps the rest: tiles of a partial block, count dumps and
 // smaller bin counts.
+//
+// Frame split.  A plan with few tiles at small nb can run K workgroups per
+// tile, each counting a K-th of every step into its own count dump
+// (pair_tile_kernel in dump mode with nsplit = K); split_entropy_kernel then
+// sums the K dumps and evaluates S2 and MI with the epilogue's formulas.
 #include "centre_internal.h"
 
 #include <type_traits>
