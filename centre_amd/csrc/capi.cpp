@@ -36,6 +36,27 @@ struct DevBuf {
     T* as() const { return static_cast<T*>(p); }
 };
 
+// Scratch slot `slot` of the context with at least `bytes` bytes (grown, never
+// shrunk; the old buffer is freed first, which waits for the device).
+hipError_t ctx_scratch(centre_ctx* ctx, int slot, size_t bytes, void** out) {
+    if (ctx->scratch_bytes[slot] < bytes || !ctx->scratch[slot]) {
+        if (ctx->scratch[slot]) {
+            hipError_t e = hipFree(ctx->scratch[slot]);
+            ctx->scratch[slot] = nullptr;
+            ctx->scratch_bytes[slot] = 0;
+            if (e != hipSuccess) return e;
+        }
+        hipError_t e = hipMalloc(&ctx->scratch[slot], bytes ? bytes : 16);
+        if (e != hipSuccess) {
+            ctx->scratch[slot] = nullptr;
+            return e;
+        }
+        ctx->scratch_bytes[slot] = bytes;
+    }
+    *out = ctx->scratch[slot];
+    return hipSuccess;
+}
+
 int check_flag(centre_ctx* ctx) {
     int f = 0;
     HIPCHK(hipMemcpyAsync(&f, ctx->d_flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -82,6 +103,8 @@ void centre_destroy(centre_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (void* p : ctx->scratch)
+        if (p) (void)hipFree(p);
     if (ctx->d_flag) (void)hipFree(ctx->d_flag);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -846,14 +869,14 @@ extern "C" int centre_dev_pair_entropy(centre_ctx* ctx, const centre_pair_plan* 
             nsplit = (int)((ss + split_len - 1) / split_len);
         }
     }
-    DevBuf split_counts;   // allocated before the timed region
+    void* split_counts = nullptr;   // allocated before the timed region
     const int rec = (nb * nb + 3) & ~3;
     if (nsplit > 1)
-        HIPCHK(split_counts.alloc(sizeof(uint32_t) * P * nsteps * (size_t)rec * nsplit));
+        HIPCHK(ctx_scratch(ctx, 2, sizeof(uint32_t) * P * nsteps * (size_t)rec * nsplit, &split_counts));
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
     if (nsplit > 1) {
         PairKernelArgs sp = a;
-        sp.counts = split_counts.as<uint32_t>();
+        sp.counts = static_cast<uint32_t*>(split_counts);
         sp.nsplit = nsplit;
         sp.split_tiles = plan->n_tiles;
         sp.split_len = split_len;
@@ -1073,14 +1096,15 @@ extern "C" int centre_dev_discretize_vmkde(centre_ctx* ctx, const double* d_bat,
     if (ld < n_dof || n_frame == 0) return fail(ctx, -3, "ld < n_dof or no frames");
     if (n_dof == 0) return 0;
     HIPCHK(hipSetDevice(ctx->device));
-    DevBuf grid, xt, trig;
+    DevBuf grid;
     double* g = d_grid;
     if (!g) {
         HIPCHK(grid.alloc(sizeof(double) * 2 * (grid_intervals + 1) * (size_t)n_dof));
         g = grid.as<double>();
     }
-    HIPCHK(xt.alloc(sizeof(float) * n_frame * (size_t)n_dof));
-    HIPCHK(trig.alloc(sizeof(double) * 2 * n_frame * (size_t)n_dof));
+    void *xt = nullptr, *trig = nullptr;
+    HIPCHK(ctx_scratch(ctx, 0, sizeof(float) * n_frame * (size_t)n_dof, &xt));
+    HIPCHK(ctx_scratch(ctx, 1, sizeof(double) * 2 * n_frame * (size_t)n_dof, &trig));
     VmkdeArgs a{};
     a.bat = d_bat;
     a.ld = ld;
@@ -1096,8 +1120,8 @@ extern "C" int centre_dev_discretize_vmkde(centre_ctx* ctx, const double* d_bat,
     a.minima = d_minima;
     a.nfound = d_nfound;
     a.codes = d_codes;
-    a.xt = xt.as<float>();
-    a.trig = trig.as<double>();
+    a.xt = static_cast<float*>(xt);
+    a.trig = static_cast<double*>(trig);
     HIPCHK(run_vmkde(a, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return 0;
