@@ -1169,23 +1169,27 @@ extern "C" int centre_dev_mst(centre_ctx* ctx, const uint32_t* d_src, const uint
     *n_tree = 0;
     if (n_node == 0) return 0;
     HIPCHK(hipSetDevice(ctx->device));
-    DevBuf comp, bw, be, it, ch;
-    HIPCHK(comp.alloc(sizeof(uint32_t) * n_node));
-    HIPCHK(bw.alloc(sizeof(unsigned long long) * n_node));
-    HIPCHK(be.alloc(sizeof(unsigned int) * n_node));
-    HIPCHK(it.alloc(n_edge ? n_edge : 1));
-    HIPCHK(ch.alloc(sizeof(int)));
+    // round state in one scratch region: best weight (8 B per node), then
+    // component and best edge (4 B each), the change flag, edge flags
+    auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t o_comp = up(sizeof(unsigned long long) * n_node);
+    const size_t o_be = o_comp + up(sizeof(uint32_t) * n_node);
+    const size_t o_ch = o_be + up(sizeof(unsigned int) * n_node);
+    const size_t o_it = o_ch + 256;
+    void* mem = nullptr;
+    HIPCHK(ctx_scratch(ctx, 3, o_it + (n_edge ? n_edge : 1), &mem));
+    uint8_t* const base = static_cast<uint8_t*>(mem);
     MstArgs a{};
     a.src = d_src;
     a.dst = d_dst;
     a.w = d_w;
     a.n_edge = n_edge;
     a.n_node = n_node;
-    a.comp = comp.as<uint32_t>();
-    a.best_w = bw.as<unsigned long long>();
-    a.best_e = be.as<unsigned int>();
-    a.in_tree = it.as<uint8_t>();
-    a.changed = ch.as<int>();
+    a.best_w = reinterpret_cast<unsigned long long*>(base);
+    a.comp = reinterpret_cast<uint32_t*>(base + o_comp);
+    a.best_e = reinterpret_cast<unsigned int*>(base + o_be);
+    a.changed = reinterpret_cast<int*>(base + o_ch);
+    a.in_tree = base + o_it;
     {
         // endpoint range check before any kernel indexes by an endpoint
         int bad = 0;

@@ -16,11 +16,12 @@ struct centre_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_pair_ms = 0.f;       // duration of the last pair-kernel launch
     // Grow-only scratch buffers kept across calls (0: vmKDE float32 columns,
-    // 1: vmKDE cos/sin pairs, 2: frame-split count dumps).  Calls on one
-    // context run on its stream in order and synchronise before returning,
-    // so the next call may reuse them; freed by centre_destroy.
-    void* scratch[3] = {nullptr, nullptr, nullptr};
-    size_t scratch_bytes[3] = {0, 0, 0};
+    // 1: vmKDE cos/sin pairs, 2: frame-split count dumps, 3: MST round
+    // state).  Calls on one context run on its stream in order and
+    // synchronise before returning, so the next call may reuse them; freed
+    // by centre_destroy.
+    void* scratch[4] = {nullptr, nullptr, nullptr, nullptr};
+    size_t scratch_bytes[4] = {0, 0, 0, 0};
 };
 
 struct centre_pair_plan {
