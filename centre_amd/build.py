@@ -5,7 +5,9 @@ Every ``csrc/*.hip`` and ``csrc/*.cpp`` file is compiled with hipcc for
 The C++ host (``host/*.cpp``, the drop-in for the centre_OMP executable) is
 compiled with g++ against that library's C ABI into ``centre_amd/centre_hip``
 (it loads libhdf5 at run time for NetCDF-4 input).
-Objects go to ``centre_amd/_build/``.  Always recompiles when called.
+Objects go to ``centre_amd/_build/``.  ``build()`` recompiles every source;
+``build(force=False)`` recompiles only sources newer than their object (or
+than any header under ``csrc/`` and ``include/``).
 """
 from __future__ import annotations
 
@@ -27,8 +29,18 @@ FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
          "-Wall", "-Wno-unused-result"]
 
 
-def _compile(src: str) -> str:
+def _headers_mtime() -> float:
+    hs = glob.glob(os.path.join(CSRC, "*.h")) + \
+        glob.glob(os.path.join(os.path.dirname(HERE), "include", "*.h"))
+    return max((os.path.getmtime(h) for h in hs), default=0.0)
+
+
+def _compile(src: str, force: bool = True) -> str:
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+    if not force and os.path.exists(obj):
+        t = os.path.getmtime(obj)
+        if t > os.path.getmtime(src) and t > _headers_mtime():
+            return obj
     cmd = [HIPCC] + FLAGS + ["-c", src, "-o", obj]
     if src.endswith(".cpp"):
         cmd = [HIPCC] + FLAGS + ["-x", "hip", "-c", src, "-o", obj]
@@ -38,12 +50,12 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(verbose: bool = False) -> str:
+def build(verbose: bool = False, force: bool = True) -> str:
     os.makedirs(BUILD, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")) +
                   glob.glob(os.path.join(CSRC, "*.cpp")))
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(lambda s: _compile(s, force), srcs))
     tmp = OUT + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -69,5 +81,5 @@ def build_host() -> str:
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, force="--changed" not in sys.argv)
     sys.exit(0)
