@@ -202,9 +202,6 @@ int run_dof1d(centre_ctx* ctx, const uint8_t* d_codes, uint64_t stride,
               const double* d_ext, int nbins, int nsteps, const int* ests, int nest,
               int jacobian, int kde, double* d_s1, double* d_L, double* d_w,
               uint64_t* d_counts) {
-    DevBuf de;
-    HIPCHK(de.alloc(sizeof(int) * nest));
-    HIPCHK(hipMemcpyAsync(de.p, ests, sizeof(int) * nest, hipMemcpyHostToDevice, ctx->stream));
     Dof1dArgs a{};
     a.codes = d_codes;
     a.stride = stride;
@@ -216,7 +213,7 @@ int run_dof1d(centre_ctx* ctx, const uint8_t* d_codes, uint64_t stride,
     a.ext = d_ext;
     a.jacobian = jacobian;
     a.kde = kde;
-    a.ests = de.as<int>();
+    for (int e = 0; e < nest; ++e) a.ests[e] = ests[e];   // nest <= 4 (check_ests)
     a.nest = nest;
     a.s1 = d_s1;
     a.L = d_L;
@@ -224,7 +221,6 @@ int run_dof1d(centre_ctx* ctx, const uint8_t* d_codes, uint64_t stride,
     a.counts = d_counts;
     a.flag = ctx->d_flag;
     HIPCHK(launch_dof1d(a, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));  // keeps `de` alive until done
     return 0;
 }
 }  // namespace
@@ -317,10 +313,12 @@ int centre_pair_plan_from_tiles(centre_ctx* ctx, uint32_t n_dof, int nbins, cons
     plan->nbins = (uint8_t)nbins;
     plan->dofs_per_tile = R;
     plan->all_pairs = true;
+    plan->tile_subset = true;
     plan->n_canon = (uint64_t)n_dof * (n_dof - 1) / 2;
     plan->n_rec = plan->n_canon;
     plan->n_full = full_tiles_first(tl, n_dof, R);
     plan->n_tiles = (int)tl.size();
+    plan->h_tiles = tl;
     hipError_t e = hipMalloc(&plan->d_tiles, sizeof(int2) * (tl.size() ? tl.size() : 1));
     if (e == hipSuccess && !tl.empty())
         e = hipMemcpy(plan->d_tiles, tl.data(), sizeof(int2) * tl.size(), hipMemcpyHostToDevice);
@@ -400,6 +398,8 @@ int centre_pair_plan_create(centre_ctx* ctx, uint32_t n_dof, int nbins,
                                   tpos.begin()];
             tmap[q * R * R + (i % R) * R + (j % R)] = (int)c;
         }
+        plan->h_rec_ij.assign(pairs, pairs + 2 * n_pair);
+        plan->h_rec2canon = r2c;
         hipError_t e;
         if ((e = hipMalloc(&plan->d_tilemap, sizeof(int) * (tmap.size() ? tmap.size() : 1))) != hipSuccess ||
             (!tmap.empty() && (e = hipMemcpy(plan->d_tilemap, tmap.data(), sizeof(int) * tmap.size(),
@@ -413,9 +413,11 @@ int centre_pair_plan_create(centre_ctx* ctx, uint32_t n_dof, int nbins,
             centre_pair_plan_destroy(plan);
             return fail(ctx, (int)e, std::string("plan upload: ") + hipGetErrorString(e));
         }
+        plan->h_tilemap = std::move(tmap);
     }
     if (!pairs) plan->n_full = full_tiles_first(tiles, n_dof, R);
     plan->n_tiles = (int)tiles.size();
+    plan->h_tiles = tiles;
     hipError_t e = hipMalloc(&plan->d_tiles, sizeof(int2) * (tiles.size() ? tiles.size() : 1));
     if (e == hipSuccess && !tiles.empty())
         e = hipMemcpy(plan->d_tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice);
@@ -433,6 +435,9 @@ void centre_pair_plan_destroy(centre_pair_plan* plan) {
     if (plan->d_tilemap) (void)hipFree(plan->d_tilemap);
     if (plan->d_rec2canon) (void)hipFree(plan->d_rec2canon);
     if (plan->d_rec_ij) (void)hipFree(plan->d_rec_ij);
+    if (plan->d_byrow_tiles) (void)hipFree(plan->d_byrow_tiles);
+    if (plan->d_byrow_tilemap) (void)hipFree(plan->d_byrow_tilemap);
+    if (plan->d_items) (void)hipFree(plan->d_items);
     delete plan;
 }
 
@@ -448,43 +453,43 @@ int centre_pair_plan_tiles(const centre_pair_plan* plan) {
 
 namespace {
 
-// Canonical (i<j) DOF list of a plan, for the count-epilogue kernel.
-std::vector<uint32_t> canon_pairs(const centre_pair_plan* plan) {
-    std::vector<uint32_t> ij(2 * plan->n_canon);
-    uint64_t p = 0;
-    for (uint32_t i = 0; i < plan->n_dof; ++i)
-        for (uint32_t j = i + 1; j < plan->n_dof; ++j) {
-            ij[2 * p] = i;
-            ij[2 * p + 1] = j;
-            ++p;
-        }
-    return ij;
-}
-
-}  // namespace
-
-namespace {
-
 // The pair kernel stages codes in 4-byte words, so every step of every code
 // row must start on a 4-byte boundary.  Codes laid out otherwise (a row
 // stride or a step length that is not a multiple of 4) are copied once to
 // rows of nsteps steps of round_up(ss, 4) frames.
-int step_aligned(centre_ctx* ctx, const uint8_t* codes, uint64_t stride, uint32_t D, uint64_t ss,
-                 int S, DevBuf& buf, const uint8_t** out, uint64_t* out_stride,
-                 uint64_t* out_sstride) {
-    if ((stride & 3) == 0 && (ss & 3) == 0 && (reinterpret_cast<uintptr_t>(codes) & 3) == 0) {
+bool codes_step_aligned(const uint8_t* codes, uint64_t stride, uint64_t ss) {
+    return (stride & 3) == 0 && (ss & 3) == 0 && (reinterpret_cast<uintptr_t>(codes) & 3) == 0;
+}
+
+size_t repack_bytes(uint32_t D, uint64_t ss, int S) {
+    return (size_t)D * S * ((ss + 3) & ~3ull);
+}
+
+// The codes as the pair kernel reads them: the caller's rows when they are
+// step-aligned, else a copy in buf (repack_bytes bytes).
+int stage_codes(centre_ctx* ctx, const uint8_t* codes, uint64_t stride, uint32_t D, uint64_t ss,
+                int S, uint8_t* buf, const uint8_t** out, uint64_t* out_stride,
+                uint64_t* out_sstride) {
+    if (codes_step_aligned(codes, stride, ss)) {
         *out = codes;
         *out_stride = stride;
         *out_sstride = ss;
         return 0;
     }
     const uint64_t ss4 = (ss + 3) & ~3ull;
-    HIPCHK(buf.alloc((size_t)D * S * ss4));
-    HIPCHK(launch_repack(codes, stride, D, ss, S, ss4, buf.as<uint8_t>(), ctx->stream));
-    *out = buf.as<uint8_t>();
+    HIPCHK(launch_repack(codes, stride, D, ss, S, ss4, buf, ctx->stream));
+    *out = buf;
     *out_stride = (uint64_t)S * ss4;
     *out_sstride = ss4;
     return 0;
+}
+
+int step_aligned(centre_ctx* ctx, const uint8_t* codes, uint64_t stride, uint32_t D, uint64_t ss,
+                 int S, DevBuf& buf, const uint8_t** out, uint64_t* out_stride,
+                 uint64_t* out_sstride) {
+    if (!codes_step_aligned(codes, stride, ss)) HIPCHK(buf.alloc(repack_bytes(D, ss, S)));
+    return stage_codes(ctx, codes, stride, D, ss, S, buf.as<uint8_t>(), out, out_stride,
+                       out_sstride);
 }
 
 // Tile list of a batch: entries (row block, column block, system).
@@ -753,74 +758,102 @@ extern "C" int centre_batch_pair_entropy(centre_ctx* ctx, const centre_batch_sys
     return check_flag(ctx);
 }
 
-extern "C" int centre_dev_pair_entropy(centre_ctx* ctx, const centre_pair_plan* plan,
-                                       const uint8_t* d_codes, uint64_t code_stride,
-                                       uint64_t n_frame, const uint8_t* d_types,
-                                       const double* d_extrema, int nsteps,
-                                       const int* ests, int nest, int jacobian, int kde,
-                                       double* d_s1, double* d_s2, double* d_mi,
-                                       uint32_t* d_counts) {
-    if (!ctx) return -3;
-    if (!plan) return fail(ctx, -3, "null plan");
-    if (int r = check_ests(ctx, ests, nest)) return r;
-    if (!d_codes || !d_types || !d_extrema || !d_s2) return fail(ctx, -3, "null buffer");
-    if (nsteps < 1 || nsteps > 255) return fail(ctx, -3, "nsteps must be in 1..255");
-    if (n_frame / nsteps == 0) return fail(ctx, -3, "fewer frames than steps");
-    if (code_stride < n_frame) return fail(ctx, -3, "code_stride < n_frame");
-    HIPCHK(hipSetDevice(ctx->device));
+namespace {
+
+// ---- pair stage ---------------------------------------------------------------
+// Device temporaries of one fused (ML / MM, no count output) pair-stage call.
+struct PairWork {
+    double* s1 = nullptr;      // [D][nest][S] marginals of the same frames
+    double* L = nullptr;       // [D][S] sum_b m_b log J_b
+    double* w = nullptr;       // [D] bin widths
+    double* s1e0 = nullptr;    // [D][S] S1 of estimator 0 (nest > 1 with MI)
+    uint8_t* packed = nullptr; // step-aligned copy of the codes, when needed
+    double* cs2 = nullptr;     // [P][nest][S] canonical-order S2 (explicit lists)
+    double* cmi = nullptr;     // [P][S] canonical-order MI (explicit lists)
+    uint32_t* split = nullptr; // frame-split count dumps
+};
+
+// Sizes (bytes) of the parts of PairWork a call needs (0 = not needed) and
+// the frame-split shape.
+struct PairLayout {
+    size_t s1 = 0, L = 0, w = 0, s1e0 = 0, packed = 0, cs2 = 0, cmi = 0, split = 0;
+    int nsplit = 1;
+    uint64_t split_len = 0;
+    int rec = 0;
+    // total of every part but the split dumps, each rounded up to 256 bytes
+    size_t base_total() const {
+        auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+        return up(s1) + up(L) + up(w) + up(s1e0) + up(packed) + up(cs2) + up(cmi);
+    }
+};
+
+PairLayout pair_layout(const centre_pair_plan* plan, const uint8_t* d_codes, uint64_t code_stride,
+                       uint64_t n_frame, int nsteps, int nest, bool want_mi) {
+    const uint64_t D = plan->n_dof, P = plan->n_canon, S = (uint64_t)nsteps;
+    const uint64_t ss = n_frame / nsteps;
+    PairLayout l;
+    l.s1 = sizeof(double) * D * nest * S;
+    l.L = sizeof(double) * D * S;
+    l.w = sizeof(double) * D;
+    if (nest > 1 && want_mi) l.s1e0 = sizeof(double) * D * S;
+    if (!codes_step_aligned(d_codes, code_stride, ss)) l.packed = repack_bytes(plan->n_dof, ss, nsteps);
+    if (!plan->all_pairs) {
+        l.cs2 = sizeof(double) * P * nest * S;
+        if (want_mi) l.cmi = sizeof(double) * P * S;
+    }
+    return l;
+}
+
+// Frame split for plans with few tiles at small nb: the four-wave kernel is
+// bound by one-hot staging there, so K workgroups per tile each count a K-th
+// of every step into their own count dump and a second kernel sums the
+// dumps and evaluates the entropies.  Chosen when the dumps fit max_bytes.
+int choose_split(centre_ctx* ctx, const centre_pair_plan* plan, uint64_t ss, int nsteps,
+                 size_t max_bytes, PairLayout* l) {
+    const int nb = plan->nbins;
+    l->rec = (nb * nb + 3) & ~3;
+    if (nb > 16 || plan->n_tiles <= 0) return 0;
+    int ncu = 0;
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    int k = std::min<int>(8, ncu / plan->n_tiles);
+    k = (int)std::min<uint64_t>((uint64_t)k, ss / 512);
+    if (k < 2) return 0;
+    const uint64_t len = ((ss + k - 1) / k + 63) / 64 * 64;   // whole chunks, 4-byte aligned
+    const int parts = (int)((ss + len - 1) / len);
+    const size_t bytes = sizeof(uint32_t) * plan->n_canon * nsteps * (size_t)l->rec * parts;
+    if (parts < 2 || bytes > max_bytes) return 0;
+    l->nsplit = parts;
+    l->split_len = len;
+    l->split = bytes;
+    return 0;
+}
+
+// Fused pair stage (ML / MM into the count pass, no count output) with the
+// temporaries in `ws` (sized by `l`).
+int run_pair_fused(centre_ctx* ctx, const centre_pair_plan* plan, const uint8_t* d_codes,
+                   uint64_t code_stride, uint64_t n_frame, const uint8_t* d_types,
+                   const double* d_extrema, int nsteps, const int* ests, int nest, int jacobian,
+                   int kde, double* d_s1, double* d_s2, double* d_mi, const PairLayout& l,
+                   const PairWork& ws) {
     const uint32_t D = plan->n_dof;
     const int nb = plan->nbins;
     const uint64_t ss = n_frame / nsteps;
-    // marginals of the same frames: S1 (all estimators), L, widths
-    DevBuf s1, L, w;
-    HIPCHK(s1.alloc(sizeof(double) * D * nest * nsteps));
-    HIPCHK(L.alloc(sizeof(double) * D * nsteps));
-    HIPCHK(w.alloc(sizeof(double) * D));
     HIPCHK(hipMemsetAsync(ctx->d_flag, 0, sizeof(int), ctx->stream));
-    if (int r = run_dof1d(ctx, d_codes, code_stride, D, n_frame, d_types, d_extrema, nb,
-                          nsteps, ests, nest, jacobian, kde, s1.as<double>(),
-                          L.as<double>(), w.as<double>(), nullptr))
+    if (int r = run_dof1d(ctx, d_codes, code_stride, D, n_frame, d_types, d_extrema, nb, nsteps,
+                          ests, nest, jacobian, kde, ws.s1, ws.L, ws.w, nullptr))
         return r;
     if (d_s1)
-        HIPCHK(hipMemcpyAsync(d_s1, s1.p, sizeof(double) * D * nest * nsteps,
+        HIPCHK(hipMemcpyAsync(d_s1, ws.s1, sizeof(double) * D * nest * nsteps,
                               hipMemcpyDeviceToDevice, ctx->stream));
-
-    bool fused_only = true;
     int slot_ml = -1, slot_mm = -1;
     for (int e = 0; e < nest; ++e) {
         if (ests[e] == 0 && slot_ml < 0) slot_ml = e;
         else if (ests[e] == 1 && slot_mm < 0) slot_mm = e;
-        else fused_only = false;
     }
-    const bool dump = !fused_only || d_counts != nullptr;
     const bool all = plan->all_pairs;
-    const uint64_t P = plan->n_canon;
-
-    // outputs in canonical order (the caller's record order for all-pair plans)
-    DevBuf cs2, cmi, ccnt;
-    double* s2c = d_s2;
-    double* mic = d_mi;
-    uint32_t* cnt = d_counts;
-    if (!all) {
-        HIPCHK(cs2.alloc(sizeof(double) * P * nest * nsteps));
-        s2c = cs2.as<double>();
-        if (d_mi) {
-            HIPCHK(cmi.alloc(sizeof(double) * P * nsteps));
-            mic = cmi.as<double>();
-        }
-    }
-    if (dump && (!all || !d_counts)) {
-        const size_t bytes = sizeof(uint32_t) * P * nsteps * (size_t)nb * nb;
-        if (bytes > ((size_t)16 << 30))
-            return fail(ctx, -3, "materialised counts exceed 16 GiB; use ML/MM or fewer pairs");
-        HIPCHK(ccnt.alloc(bytes));
-        cnt = ccnt.as<uint32_t>();
-    }
-
     PairKernelArgs a{};
-    DevBuf packed;
-    if (int r = step_aligned(ctx, d_codes, code_stride, D, ss, nsteps, packed, &a.codes, &a.stride,
-                             &a.sstride))
+    if (int r = stage_codes(ctx, d_codes, code_stride, D, ss, nsteps, ws.packed, &a.codes,
+                            &a.stride, &a.sstride))
         return r;
     a.n_dof = (int)D;
     a.nb = nb;
@@ -829,66 +862,39 @@ extern "C" int centre_dev_pair_entropy(centre_ctx* ctx, const centre_pair_plan* 
     a.ss = ss;
     a.tiles = plan->d_tiles;
     a.tilemap = plan->d_tilemap;
-    a.dof_w = w.as<double>();
-    a.dof_L = L.as<double>();
-    a.dof_S1 = s1.as<double>();
+    a.dof_w = ws.w;
+    a.dof_L = ws.L;
+    a.dof_S1 = ws.s1;
     a.slot_ml = slot_ml;
     a.slot_mm = slot_mm;
     a.nest = nest;
-    a.s2 = s2c;
-    a.mi = mic;
+    a.s2 = all ? d_s2 : ws.cs2;
+    a.mi = d_mi ? (all ? d_mi : ws.cmi) : nullptr;
     a.mi_from_mm = (ests[0] == 1);
-    a.counts = cnt;
     a.flag = ctx->d_flag;
-    // dof_S1 is [D][nest][S]; the kernel reads [D][S] of estimator 0.  With
-    // nest > 1 give it a compact copy.
-    DevBuf s1e0;
+    // dof_S1 is [D][nest][S]; the kernel reads [D][S] of estimator 0
     if (nest > 1 && d_mi) {
-        HIPCHK(s1e0.alloc(sizeof(double) * D * nsteps));
-        HIPCHK(launch_copy_2d(s1e0.p, sizeof(double) * nsteps, s1.p,
+        HIPCHK(launch_copy_2d(ws.s1e0, sizeof(double) * nsteps, ws.s1,
                               sizeof(double) * nsteps * nest, sizeof(double) * nsteps, D,
                               ctx->stream));
-        a.dof_S1 = s1e0.as<double>();
+        a.dof_S1 = ws.s1e0;
     }
-    if (!d_mi) a.mi = nullptr;
-    if (d_mi && ests[0] > 1) a.mi = nullptr;  // MI for CS/JS computed below
-    // Frame split for plans with few tiles at small nb (ML / MM only): the
-    // four-wave kernel is bound by one-hot staging there, so K workgroups per
-    // tile each count a K-th of every step into their own count dump and a
-    // second kernel sums the dumps and evaluates the entropies.
-    int nsplit = 1;
-    uint64_t split_len = 0;
-    if (!dump && nb <= 16 && plan->n_tiles > 0) {
-        int ncu = 0;
-        HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
-        int k = std::min<int>(8, ncu / plan->n_tiles);
-        k = (int)std::min<uint64_t>((uint64_t)k, ss / 512);
-        const uint64_t cbytes = sizeof(uint32_t) * P * nsteps * (size_t)((nb * nb + 3) & ~3);
-        if (k >= 2 && cbytes * k <= ((size_t)1 << 30)) {
-            split_len = ((ss + k - 1) / k + 63) / 64 * 64;   // whole chunks, 4-byte aligned
-            nsplit = (int)((ss + split_len - 1) / split_len);
-        }
-    }
-    void* split_counts = nullptr;   // allocated before the timed region
-    const int rec = (nb * nb + 3) & ~3;
-    if (nsplit > 1)
-        HIPCHK(ctx_scratch(ctx, 2, sizeof(uint32_t) * P * nsteps * (size_t)rec * nsplit, &split_counts));
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-    if (nsplit > 1) {
+    if (l.nsplit > 1) {
         PairKernelArgs sp = a;
-        sp.counts = static_cast<uint32_t*>(split_counts);
-        sp.nsplit = nsplit;
+        sp.counts = ws.split;
+        sp.nsplit = l.nsplit;
         sp.split_tiles = plan->n_tiles;
-        sp.split_len = split_len;
-        sp.split_rec = rec;
-        sp.split_cstride = P * nsteps * (uint64_t)rec;
+        sp.split_len = l.split_len;
+        sp.split_rec = l.rec;
+        sp.split_cstride = plan->n_canon * nsteps * (uint64_t)l.rec;
         PairKernelArgs dumpargs = sp;   // counts only
         dumpargs.slot_ml = dumpargs.slot_mm = -1;
         dumpargs.mi = nullptr;
         HIPCHK(launch_pair_tiles(dumpargs, plan->n_tiles, true, ctx->stream));
         HIPCHK(launch_split_entropy(sp, plan->n_tiles, ctx->stream));
-    } else if (!dump && pair_tile8_supported(a.R) && plan->n_full > 0 &&
-        ((a.stride | a.sstride | reinterpret_cast<uintptr_t>(a.codes)) & 3) == 0) {
+    } else if (pair_tile8_supported(a.R) && plan->n_full > 0 &&
+               ((a.stride | a.sstride | reinterpret_cast<uintptr_t>(a.codes)) & 3) == 0) {
         // full tiles on the eight-wave kernel, those of a partial block after
         HIPCHK(launch_pair_tiles8(a, plan->n_full, ctx->stream));
         PairKernelArgs r = a;
@@ -896,75 +902,14 @@ extern "C" int centre_dev_pair_entropy(centre_ctx* ctx, const centre_pair_plan* 
         if (r.tilemap) r.tilemap += (size_t)plan->n_full * a.R * a.R;
         HIPCHK(launch_pair_tiles(r, plan->n_tiles - plan->n_full, false, ctx->stream));
     } else {
-        HIPCHK(launch_pair_tiles(a, plan->n_tiles, dump, ctx->stream));
+        HIPCHK(launch_pair_tiles(a, plan->n_tiles, false, ctx->stream));
     }
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
-
-    if (!all && d_counts)
-        HIPCHK(launch_gather_counts(cnt, plan->d_rec2canon, plan->d_rec_ij, plan->n_rec, nsteps,
-                                    nb, d_counts, ctx->stream));
-
-    if (!fused_only) {
-        // every estimator from the materialised counts, per output record: a
-        // record keeps its own row/column order (the Chao-Shen Jacobian
-        // depends on it, Estimators.cpp:558,562)
-        const uint64_t NR = all ? P : plan->n_rec;
-        std::vector<uint32_t> ij;
-        DevBuf dij, de;
-        if (all) {
-            ij = canon_pairs(plan);
-            HIPCHK(dij.alloc(sizeof(uint32_t) * ij.size()));
-            HIPCHK(hipMemcpyAsync(dij.p, ij.data(), sizeof(uint32_t) * ij.size(),
-                                  hipMemcpyHostToDevice, ctx->stream));
-        } else if (d_mi && ests[0] > 1) {
-            ij.resize(2 * NR);
-            HIPCHK(hipMemcpyAsync(ij.data(), plan->d_rec_ij, sizeof(uint32_t) * ij.size(),
-                                  hipMemcpyDeviceToHost, ctx->stream));
-        }
-        HIPCHK(de.alloc(sizeof(int) * nest));
-        HIPCHK(hipMemcpyAsync(de.p, ests, sizeof(int) * nest, hipMemcpyHostToDevice, ctx->stream));
-        CountEntropyArgs c{};
-        c.counts = cnt;
-        c.n_pair = NR;
-        c.canon_ij = all ? dij.as<uint32_t>() : plan->d_rec_ij;
-        c.rec2canon = all ? nullptr : plan->d_rec2canon;
-        c.nb = nb;
-        c.nsteps = nsteps;
-        c.ss = ss;
-        c.types = d_types;
-        c.ext = d_extrema;
-        c.jacobian = jacobian;
-        c.kde = kde;
-        c.ests = de.as<int>();
-        c.nest = nest;
-        c.s2 = all ? s2c : d_s2;
-        HIPCHK(launch_count_entropy(c, ctx->stream));
-        if (d_mi && ests[0] > 1) {
-            // MI = S1_i + S1_j - S2 for estimator 0 on the host side of the ABI
-            std::vector<double> hs1((size_t)D * nest * nsteps), hs2((size_t)NR * nest * nsteps);
-            HIPCHK(hipMemcpyAsync(hs1.data(), s1.p, sizeof(double) * hs1.size(),
-                                  hipMemcpyDeviceToHost, ctx->stream));
-            HIPCHK(hipMemcpyAsync(hs2.data(), c.s2, sizeof(double) * hs2.size(),
-                                  hipMemcpyDeviceToHost, ctx->stream));
-            HIPCHK(hipStreamSynchronize(ctx->stream));
-            std::vector<double> hmi((size_t)NR * nsteps);
-            for (uint64_t p = 0; p < NR; ++p)
-                for (int s = 0; s < nsteps; ++s)
-                    hmi[p * nsteps + s] = hs1[((size_t)ij[2 * p] * nest) * nsteps + s] +
-                                          hs1[((size_t)ij[2 * p + 1] * nest) * nsteps + s] -
-                                          hs2[(p * nest) * nsteps + s];
-            HIPCHK(hipMemcpyAsync(all ? mic : d_mi, hmi.data(), sizeof(double) * hmi.size(),
-                                  hipMemcpyHostToDevice, ctx->stream));
-        }
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-    }
-
     if (!all) {
-        if (fused_only)
-            HIPCHK(launch_gather_rec(s2c, plan->d_rec2canon, plan->n_rec, nest * nsteps, d_s2,
-                                     ctx->stream));
-        if (d_mi && a.mi)
-            HIPCHK(launch_gather_rec(mic, plan->d_rec2canon, plan->n_rec, nsteps, d_mi,
+        HIPCHK(launch_gather_rec(ws.cs2, plan->d_rec2canon, plan->n_rec, nest * nsteps, d_s2,
+                                 ctx->stream));
+        if (d_mi)
+            HIPCHK(launch_gather_rec(ws.cmi, plan->d_rec2canon, plan->n_rec, nsteps, d_mi,
                                      ctx->stream));
     }
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -972,6 +917,283 @@ extern "C" int centre_dev_pair_entropy(centre_ctx* ctx, const centre_pair_plan* 
     HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->last_pair_ms = ms;
     return check_flag(ctx);
+}
+
+// Row-block grouping of a plan (see centre_pair_plan), built once.
+int ensure_grouping(centre_ctx* ctx, const centre_pair_plan* plan) {
+    if (plan->grouped) return 0;
+    const int R = plan->dofs_per_tile;
+    const uint64_t D = plan->n_dof;
+    const int nblk = (int)((D + R - 1) / R);
+    const size_t nt = plan->h_tiles.size();
+    // tiles by row block, launch order kept within a row block; a tile-subset
+    // list may name a tile twice
+    std::vector<size_t> ord(nt);
+    std::iota(ord.begin(), ord.end(), (size_t)0);
+    std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) {
+        const int2 a = plan->h_tiles[x], b = plan->h_tiles[y];
+        return a.x != b.x ? a.x < b.x : (plan->tile_subset && a.y < b.y);
+    });
+    std::vector<int2> bt;
+    std::vector<int> btm;
+    std::vector<int> tile_off(nblk + 1, 0);
+    for (size_t q = 0; q < nt; ++q) {
+        const int2 t = plan->h_tiles[ord[q]];
+        if (plan->tile_subset && !bt.empty() && bt.back().x == t.x && bt.back().y == t.y) continue;
+        bt.push_back(t);
+        tile_off[t.x + 1]++;
+        if (!plan->h_tilemap.empty())
+            btm.insert(btm.end(), plan->h_tilemap.begin() + ord[q] * R * R,
+                       plan->h_tilemap.begin() + (ord[q] + 1) * R * R);
+    }
+    for (int I = 0; I < nblk; ++I) tile_off[I + 1] += tile_off[I];
+    // canonical index ranges of the row blocks
+    std::vector<uint64_t> canon_off(nblk + 1, 0), item_off(nblk + 1, 0);
+    std::vector<uint64_t> items;
+    auto row_off = [&](uint64_t i) { return i * (2 * D - i - 1) / 2; };   // first pair of row i
+    if (plan->all_pairs) {
+        for (int I = 0; I <= nblk; ++I) canon_off[I] = row_off(std::min<uint64_t>((uint64_t)I * R, D));
+        if (plan->tile_subset) {
+            // canonical indices of the listed tiles' pairs, ascending per row block
+            for (int I = 0; I < nblk; ++I) {
+                const size_t first = items.size();
+                for (int q = tile_off[I]; q < tile_off[I + 1]; ++q) {
+                    const uint64_t J = (uint64_t)bt[q].y;
+                    for (uint64_t i = (uint64_t)I * R; i < std::min<uint64_t>((uint64_t)(I + 1) * R, D); ++i)
+                        for (uint64_t j = std::max(J * R, i + 1); j < std::min((J + 1) * R, D); ++j)
+                            items.push_back(row_off(i) + (j - i - 1));
+                }
+                std::sort(items.begin() + first, items.end());
+                items.erase(std::unique(items.begin() + first, items.end()), items.end());
+                item_off[I + 1] = items.size();
+            }
+        }
+    } else {
+        // canonical indices are ordered by (lower DOF, higher DOF): count the
+        // distinct ones per row block of the lower DOF
+        std::vector<uint8_t> seen(plan->n_canon, 0);
+        std::vector<uint64_t> per(nblk, 0), rper(nblk, 0);
+        const uint64_t NR = plan->n_rec;
+        for (uint64_t r = 0; r < NR; ++r) {
+            const uint32_t lo = std::min(plan->h_rec_ij[2 * r], plan->h_rec_ij[2 * r + 1]);
+            const uint64_t c = plan->h_rec2canon[r];
+            rper[lo / R]++;
+            if (!seen[c]) {
+                seen[c] = 1;
+                per[lo / R]++;
+            }
+        }
+        for (int I = 0; I < nblk; ++I) {
+            canon_off[I + 1] = canon_off[I] + per[I];
+            item_off[I + 1] = item_off[I] + rper[I];
+        }
+        // record ids sorted by canonical index (record order among equals)
+        items.resize(NR);
+        std::iota(items.begin(), items.end(), (uint64_t)0);
+        std::stable_sort(items.begin(), items.end(), [&](uint64_t x, uint64_t y) {
+            return plan->h_rec2canon[x] < plan->h_rec2canon[y];
+        });
+    }
+    hipError_t e = hipSuccess;
+    if (!bt.empty() && (e = hipMalloc(&plan->d_byrow_tiles, sizeof(int2) * bt.size())) == hipSuccess)
+        e = hipMemcpy(plan->d_byrow_tiles, bt.data(), sizeof(int2) * bt.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !btm.empty() &&
+        (e = hipMalloc(&plan->d_byrow_tilemap, sizeof(int) * btm.size())) == hipSuccess)
+        e = hipMemcpy(plan->d_byrow_tilemap, btm.data(), sizeof(int) * btm.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !items.empty() &&
+        (e = hipMalloc(&plan->d_items, sizeof(uint64_t) * items.size())) == hipSuccess)
+        e = hipMemcpy(plan->d_items, items.data(), sizeof(uint64_t) * items.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(ctx, (int)e, std::string("plan grouping: ") + hipGetErrorString(e));
+    plan->tile_off = std::move(tile_off);
+    plan->canon_off = std::move(canon_off);
+    plan->item_off = std::move(item_off);
+    plan->grouped = true;
+    return 0;
+}
+
+// Count budget of one group of the grouped pass (a single row block may
+// exceed it and then forms a group of its own).
+constexpr size_t kGroupCountBytes = (size_t)2 << 30;
+
+// Pair stage through materialised counts: any estimator (Chao-Shen and
+// James-Stein need whole count tables), and the count output.  The plan's
+// row blocks are taken in groups whose counts fit kGroupCountBytes: the
+// pair kernel dumps a group's counts, then count_entropy_kernel evaluates
+// every estimator, MI and the record-order count output of its records.
+int run_pair_grouped(centre_ctx* ctx, const centre_pair_plan* plan, const uint8_t* d_codes,
+                     uint64_t code_stride, uint64_t n_frame, const uint8_t* d_types,
+                     const double* d_extrema, int nsteps, const int* ests, int nest,
+                     int jacobian, int kde, double* d_s1, double* d_s2, double* d_mi,
+                     uint32_t* d_counts) {
+    const uint32_t D = plan->n_dof;
+    const int nb = plan->nbins;
+    const int R = plan->dofs_per_tile;
+    const uint64_t ss = n_frame / nsteps;
+    DevBuf s1, L, w;
+    HIPCHK(s1.alloc(sizeof(double) * D * nest * nsteps));
+    HIPCHK(L.alloc(sizeof(double) * D * nsteps));
+    HIPCHK(w.alloc(sizeof(double) * D));
+    HIPCHK(hipMemsetAsync(ctx->d_flag, 0, sizeof(int), ctx->stream));
+    if (int r = run_dof1d(ctx, d_codes, code_stride, D, n_frame, d_types, d_extrema, nb, nsteps,
+                          ests, nest, jacobian, kde, s1.as<double>(), L.as<double>(),
+                          w.as<double>(), nullptr))
+        return r;
+    if (d_s1)
+        HIPCHK(hipMemcpyAsync(d_s1, s1.p, sizeof(double) * D * nest * nsteps,
+                              hipMemcpyDeviceToDevice, ctx->stream));
+    if (int r = ensure_grouping(ctx, plan)) return r;
+    const int nblk = (int)((D + R - 1) / R);
+    const size_t per = sizeof(uint32_t) * (size_t)nsteps * nb * nb;   // counts of one pair
+    std::vector<std::pair<int, int>> groups;
+    size_t most = 0;
+    for (int I0 = 0; I0 < nblk;) {
+        int I1 = I0 + 1;
+        while (I1 < nblk && (plan->canon_off[I1 + 1] - plan->canon_off[I0]) * per <= kGroupCountBytes)
+            ++I1;
+        groups.emplace_back(I0, I1);
+        most = std::max(most, (size_t)(plan->canon_off[I1] - plan->canon_off[I0]) * per);
+        I0 = I1;
+    }
+    void* cnt = nullptr;
+    HIPCHK(ctx_scratch(ctx, 4, most, &cnt));
+    PairKernelArgs a{};
+    DevBuf packed;
+    if (int r = step_aligned(ctx, d_codes, code_stride, D, ss, nsteps, packed, &a.codes, &a.stride,
+                             &a.sstride))
+        return r;
+    a.n_dof = (int)D;
+    a.nb = nb;
+    a.R = R;
+    a.nsteps = nsteps;
+    a.ss = ss;
+    a.dof_w = w.as<double>();
+    a.dof_L = L.as<double>();
+    a.slot_ml = a.slot_mm = -1;   // counts only
+    a.nest = nest;
+    a.counts = static_cast<uint32_t*>(cnt);
+    a.flag = ctx->d_flag;
+    CountEntropyArgs c{};
+    c.counts = static_cast<const uint32_t*>(cnt);
+    c.mode = plan->all_pairs ? (plan->tile_subset ? 1 : 0) : 2;
+    c.rec2canon = plan->d_rec2canon;
+    c.rec_ij = plan->d_rec_ij;
+    c.n_dof = D;
+    c.nb = nb;
+    c.nsteps = nsteps;
+    c.ss = ss;
+    c.types = d_types;
+    c.ext = d_extrema;
+    c.jacobian = jacobian;
+    c.kde = kde;
+    for (int e = 0; e < nest; ++e) c.ests[e] = ests[e];
+    c.nest = nest;
+    c.s2 = d_s2;
+    c.s1 = s1.as<double>();
+    c.mi = d_mi;
+    c.counts_out = d_counts;
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    for (const auto& g : groups) {
+        const int nt = plan->tile_off[g.second] - plan->tile_off[g.first];
+        if (nt == 0) continue;
+        PairKernelArgs ag = a;
+        ag.tiles = plan->d_byrow_tiles + plan->tile_off[g.first];
+        ag.tilemap = plan->d_byrow_tilemap
+                         ? plan->d_byrow_tilemap + (size_t)plan->tile_off[g.first] * R * R
+                         : nullptr;
+        ag.count_base = plan->canon_off[g.first];
+        HIPCHK(launch_pair_tiles(ag, nt, true, ctx->stream));
+        CountEntropyArgs cg = c;
+        cg.count_base = plan->canon_off[g.first];
+        if (c.mode == 0) {
+            cg.item_first = plan->canon_off[g.first];
+            cg.n_item = plan->canon_off[g.second] - plan->canon_off[g.first];
+        } else {
+            cg.items = plan->d_items + plan->item_off[g.first];
+            cg.n_item = plan->item_off[g.second] - plan->item_off[g.first];
+        }
+        HIPCHK(launch_count_entropy(cg, ctx->stream));
+    }
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->last_pair_ms = ms;
+    return check_flag(ctx);
+}
+
+int check_pair_args(centre_ctx* ctx, const centre_pair_plan* plan, const uint8_t* d_codes,
+                    uint64_t code_stride, uint64_t n_frame, const uint8_t* d_types,
+                    const double* d_extrema, int nsteps, const int* ests, int nest,
+                    const double* d_s2) {
+    if (!plan) return fail(ctx, -3, "null plan");
+    if (int r = check_ests(ctx, ests, nest)) return r;
+    if (!d_codes || !d_types || !d_extrema || !d_s2) return fail(ctx, -3, "null buffer");
+    if (nsteps < 1 || nsteps > 255) return fail(ctx, -3, "nsteps must be in 1..255");
+    if (n_frame / nsteps == 0) return fail(ctx, -3, "fewer frames than steps");
+    if (code_stride < n_frame) return fail(ctx, -3, "code_stride < n_frame");
+    return 0;
+}
+
+bool fused_estimators(const int* ests, int nest) {
+    bool ml = false, mm = false;
+    for (int e = 0; e < nest; ++e) {
+        if (ests[e] == 0 && !ml) ml = true;
+        else if (ests[e] == 1 && !mm) mm = true;
+        else return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" int centre_dev_pair_entropy(centre_ctx* ctx, const centre_pair_plan* plan,
+                                       const uint8_t* d_codes, uint64_t code_stride,
+                                       uint64_t n_frame, const uint8_t* d_types,
+                                       const double* d_extrema, int nsteps,
+                                       const int* ests, int nest, int jacobian, int kde,
+                                       double* d_s1, double* d_s2, double* d_mi,
+                                       uint32_t* d_counts) {
+    if (!ctx) return -3;
+    if (int r = check_pair_args(ctx, plan, d_codes, code_stride, n_frame, d_types, d_extrema,
+                                nsteps, ests, nest, d_s2))
+        return r;
+    HIPCHK(hipSetDevice(ctx->device));
+    if (d_counts || !fused_estimators(ests, nest))
+        return run_pair_grouped(ctx, plan, d_codes, code_stride, n_frame, d_types, d_extrema,
+                                nsteps, ests, nest, jacobian, kde, d_s1, d_s2, d_mi, d_counts);
+    PairLayout l = pair_layout(plan, d_codes, code_stride, n_frame, nsteps, nest, d_mi != nullptr);
+    if (int r = choose_split(ctx, plan, n_frame / nsteps, nsteps, (size_t)1 << 30, &l)) return r;
+    DevBuf s1, L, w, s1e0, packed, cs2, cmi;
+    PairWork ws;
+    HIPCHK(s1.alloc(l.s1));
+    HIPCHK(L.alloc(l.L));
+    HIPCHK(w.alloc(l.w));
+    ws.s1 = s1.as<double>();
+    ws.L = L.as<double>();
+    ws.w = w.as<double>();
+    if (l.s1e0) {
+        HIPCHK(s1e0.alloc(l.s1e0));
+        ws.s1e0 = s1e0.as<double>();
+    }
+    if (l.packed) {
+        HIPCHK(packed.alloc(l.packed));
+        ws.packed = packed.as<uint8_t>();
+    }
+    if (l.cs2) {
+        HIPCHK(cs2.alloc(l.cs2));
+        ws.cs2 = cs2.as<double>();
+    }
+    if (l.cmi) {
+        HIPCHK(cmi.alloc(l.cmi));
+        ws.cmi = cmi.as<double>();
+    }
+    if (l.split) {   // allocated before the timed region, kept by the context
+        void* sp = nullptr;
+        HIPCHK(ctx_scratch(ctx, 2, l.split, &sp));
+        ws.split = static_cast<uint32_t*>(sp);
+    }
+    return run_pair_fused(ctx, plan, d_codes, code_stride, n_frame, d_types, d_extrema, nsteps,
+                          ests, nest, jacobian, kde, d_s1, d_s2, d_mi, l, ws);
 }
 
 extern "C" int centre_pair_entropy(centre_ctx* ctx, const uint8_t* codes, uint32_t n_dof,

@@ -20,8 +20,9 @@ struct centre_ctx {
     // state).  Calls on one context run on its stream in order and
     // synchronise before returning, so the next call may reuse them; freed
     // by centre_destroy.
-    void* scratch[4] = {nullptr, nullptr, nullptr, nullptr};
-    size_t scratch_bytes[4] = {0, 0, 0, 0};
+    // 4: count dumps of the grouped estimator pass (CS / JS, count output).
+    void* scratch[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    size_t scratch_bytes[5] = {0, 0, 0, 0, 0};
 };
 
 struct centre_pair_plan {
@@ -36,7 +37,26 @@ struct centre_pair_plan {
     uint64_t* d_rec2canon = nullptr;  // [n_rec] (explicit lists only)
     uint32_t* d_rec_ij = nullptr;     // [n_rec][2] as given (explicit lists only)
     bool all_pairs = true;
+    bool tile_subset = false;         // all-pairs layout, listed tiles only
     int n_full = 0;                   // leading tiles with R DOFs on both sides
+    // host copies for the grouped estimator pass (launch order)
+    std::vector<int2> h_tiles;
+    std::vector<int> h_tilemap;       // explicit lists only
+    std::vector<uint32_t> h_rec_ij;   // explicit lists only
+    std::vector<uint64_t> h_rec2canon;
+    // Row-block grouping, built on first use by the grouped pass.  The
+    // canonical (i < j, row-major) indices of the pairs whose row DOF lies in
+    // row block I form the range [canon_off[I], canon_off[I + 1]); the
+    // plan's tiles of row block I are byrow_tiles[tile_off[I] ..
+    // tile_off[I + 1]); the records (explicit lists: record ids sorted by
+    // canonical index; tile subsets: canonical indices of the listed pairs)
+    // are items[item_off[I] .. item_off[I + 1]).
+    mutable bool grouped = false;
+    mutable std::vector<uint64_t> canon_off, item_off;
+    mutable std::vector<int> tile_off;
+    mutable int2* d_byrow_tiles = nullptr;
+    mutable int* d_byrow_tilemap = nullptr;
+    mutable uint64_t* d_items = nullptr;
 };
 
 // Bins of one-hot rows per tile side (a multiple of 32, the MFMA block).
@@ -91,6 +111,7 @@ struct PairKernelArgs {
     double* mi;             // [P][S] or null
     int mi_from_mm;         // MI uses the MM S2 (when MM is the first estimator)
     uint32_t* counts;       // dump mode: [P][S][nb*nb] uint32 prefix counts
+    uint64_t count_base;    // dump mode: pair p is written at record p - count_base
     // frame split (dump mode only, nsplit > 1): workgroup w counts tile
     // w % split_tiles over frames [k*split_len, k*split_len + len_k) of every
     // step, k = w / split_tiles (the last part takes the rest of the step),
@@ -143,7 +164,7 @@ struct Dof1dArgs {
     const uint8_t* types;   // [D] 1 bond, 2 angle, 3 torsion
     const double* ext;      // [D][4] (min, max, ...)
     int jacobian, kde;
-    const int* ests;        // [nest] estimator ids
+    int ests[4];            // [nest] estimator ids
     int nest;
     double* s1;             // [D][nest][S]
     double* L;              // [D][S] sum_b m_b log J_b (for the pair epilogue)
@@ -153,21 +174,34 @@ struct Dof1dArgs {
 };
 hipError_t launch_dof1d(const Dof1dArgs& a, hipStream_t st);
 
-// estimator epilogue on materialised counts (CS / JS / any estimator)
+// Every estimator of entropy2D (Estimators.cpp:391-651) and MI from counts
+// materialised by the pair kernel's dump mode, for the records of one group
+// of row blocks.  Record t of the launch:
+//   mode 0: canonical index = record = item_first + t (all-pairs plans)
+//   mode 1: canonical index = record = items[t] (tile subsets)
+//   mode 2: record = items[t], canonical rec2canon[record], DOFs
+//           rec_ij[record] (explicit lists; i > j reads the counts transposed)
 struct CountEntropyArgs {
-    const uint32_t* counts;   // [canonical pair][S][nb*nb], row DOF < column DOF
-    uint64_t n_pair;          // output rows
-    const uint32_t* canon_ij; // [n_pair][2] row and column DOF of each output row
-    const uint64_t* rec2canon;// [n_pair] counts index of each row, NULL = identity;
-                              // a row with i > j reads its counts transposed
+    const uint32_t* counts;   // [canonical - count_base][S][nb*nb], row DOF < column DOF
+    uint64_t count_base;
+    uint64_t n_item;
+    int mode;
+    uint64_t item_first;
+    const uint64_t* items;
+    const uint64_t* rec2canon;
+    const uint32_t* rec_ij;
+    uint32_t n_dof;
     int nb, nsteps;
     uint64_t ss;
     const uint8_t* types;
     const double* ext;
     int jacobian, kde;
-    const int* ests;
+    int ests[4];
     int nest;
-    double* s2;               // [P][nest][S]
+    double* s2;               // [record][nest][S]
+    const double* s1;         // [D][nest][S] marginals, or null (no MI)
+    double* mi;               // [record][S] of estimator ests[0], or null
+    uint32_t* counts_out;     // [record][S][nb*nb] in the record's own order, or null
 };
 hipError_t launch_count_entropy(const CountEntropyArgs& a, hipStream_t st);
 
@@ -242,12 +276,6 @@ hipError_t launch_gather_rec(const double* src, const uint64_t* rec2canon,
                              hipStream_t st);
 hipError_t launch_copy_2d(void* dst, size_t dpitch, const void* src, size_t spitch, size_t width,
                           size_t height, hipStream_t st);
-// dst[r][s][x*nb + y] = counts of record r = (i, j) at step s, row bin x of i,
-// column bin y of j, from canonical counts (row DOF < column DOF).
-hipError_t launch_gather_counts(const uint32_t* src, const uint64_t* rec2canon,
-                                const uint32_t* rec_ij, uint64_t n_rec, int nsteps, int nb,
-                                uint32_t* dst, hipStream_t st);
-
 // ---- learned pair screen (screen.hip) ----------------------------------------
 constexpr int kScreenFeat = 8;     // features per pair
 constexpr int kScreenHidden = 64;  // width of both hidden layers

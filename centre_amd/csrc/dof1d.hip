@@ -9,10 +9,10 @@
 // Jacobian sum L = sum_b m_b log J_b and the bin width that the pair
 // epilogue needs.
 //
-// count_entropy_kernel evaluates entropy2D (Estimators.cpp:391-651) for any
-// estimator from prefix counts written by the pair kernel's dump mode: one
-// thread per (pair, step, estimator), in the reference's row/column order,
-// including the Chao-Shen second Jacobian evaluated at mid2[j]
+// count_entropy_kernel evaluates entropy2D (Estimators.cpp:391-651) for
+// every estimator, and MI, from prefix counts written by the pair kernel's
+// dump mode: one wave per (record, step), in the record's own row/column
+// order, including the Chao-Shen second Jacobian evaluated at mid2[j]
 // (Estimators.cpp:558,562) and the James-Stein target 1/nbins
 // (Estimators.cpp:510).
 #include "centre_internal.h"
@@ -169,18 +169,62 @@ __global__ __launch_bounds__(256) void dof1d_kernel(Dof1dArgs a) {
     if (tid == 0) a.w[d] = w;
 }
 
+// Sum over the 64 lanes of a wave in a fixed order; every lane returns lane
+// 0's total, so the value does not depend on the lane or the run.
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return __shfl(v, 0);
+}
+
+__device__ inline unsigned wave_sum_u(unsigned v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// (i, j), i < j, of canonical index p in the row-major upper triangle of D
+// DOFs (row i starts at i (2D - i - 1) / 2)
+__device__ inline void tri_pair(uint64_t p, uint32_t D, uint32_t* pi, uint32_t* pj) {
+    auto off = [D](uint64_t x) { return x * (2ull * D - x - 1) / 2; };
+    const double b = 2.0 * D - 1.0;
+    long long i = (long long)((b - sqrt(b * b - 8.0 * (double)p)) / 2.0);
+    if (i < 0) i = 0;
+    if (i > (long long)D - 2) i = (long long)D - 2;
+    while (i > 0 && off((uint64_t)i) > p) --i;
+    while (i + 1 <= (long long)D - 2 && off((uint64_t)i + 1) <= p) ++i;
+    *pi = (uint32_t)i;
+    *pj = (uint32_t)(p - off((uint64_t)i) + (uint64_t)i + 1);
+}
+
+// One wave per (record, step): the lanes take the record's nb*nb cells in
+// turn.  The statistics the estimators need (empty and singleton cells, the
+// James-Stein sums) come first, then each estimator's terms, as
+// entropy2D writes them (Estimators.cpp:455-648): the Chao-Shen second
+// Jacobian at the row bin (mid2[j], :558, :562), the James-Stein target
+// 1/nbins (:510).  (1 - q)^n is below 2^-54 once n q > 40, where
+// 1 - (1 - q)^n rounds to 1 exactly, so the Horvitz-Thompson divisor skips
+// pow there.
 __global__ __launch_bounds__(64) void count_entropy_kernel(CountEntropyArgs a) {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int S = a.nsteps, nest = a.nest, nb = a.nb;
-    if (t >= a.n_pair * S * nest) return;
-    const int e = (int)(t % nest);
-    const int s = (int)((t / nest) % S);
-    const uint64_t p = t / ((uint64_t)nest * S);
-    const int est = a.ests[e];
-    const uint32_t i = a.canon_ij[2 * p], j = a.canon_ij[2 * p + 1];
-    const uint64_t pc = a.rec2canon ? a.rec2canon[p] : p;
+    __shared__ double j1[256], j2[256];
+    const int S = a.nsteps, nb = a.nb, nest = a.nest;
+    const uint64_t item = blockIdx.x;
+    const int s = (int)blockIdx.y;
+    const int lane = (int)threadIdx.x;
+    uint64_t rec, canon;
+    uint32_t i, j;
+    if (a.mode == 2) {
+        rec = a.items[item];
+        canon = a.rec2canon[rec];
+        i = a.rec_ij[2 * rec];
+        j = a.rec_ij[2 * rec + 1];
+    } else {
+        canon = a.mode == 1 ? a.items[item] : a.item_first + item;
+        rec = canon;
+        tri_pair(canon, a.n_dof, &i, &j);
+    }
     // counts are stored with the lower DOF index as rows; the reference's
-    // row is the record's first DOF, which matters to the Chao-Shen Jacobian
+    // row is the record's first DOF
     const bool flip = i > j;
     const unsigned rs = flip ? 1u : (unsigned)nb, cs = flip ? (unsigned)nb : 1u;
     const int t1 = a.types[i], t2 = a.types[j];
@@ -188,59 +232,80 @@ __global__ __launch_bounds__(64) void count_entropy_kernel(CountEntropyArgs a) {
     const double mn2 = a.ext[4 * j], mx2 = a.ext[4 * j + 1];
     double w1 = (mx1 - mn1) / nb, w2 = (mx2 - mn2) / nb;
     const double hw1 = w1 / 2, hw2 = w2 / 2;
-    if (a.kde) { w1 = 1.0; w2 = 1.0; }
+    if (a.kde) {
+        w1 = 1.0;
+        w2 = 1.0;
+    }
     const bool hj1 = (t1 == 1 || t1 == 2), hj2 = (t2 == 1 || t2 == 2);
+    for (int b = lane; b < nb; b += 64) {
+        j1[b] = hj1 ? jac(t1, mn1 + (b * w1) + hw1) : 1.0;
+        j2[b] = hj2 ? jac(t2, mn2 + (b * w2) + hw2) : 1.0;
+    }
+    __syncthreads();
     const double area = w1 * w2;
     const double n = (s + 1.0) * a.ss;
-    const uint32_t* f = a.counts + ((size_t)pc * S + s) * nb * nb;
     const unsigned nb2 = (unsigned)nb * nb;
+    const uint32_t* f = a.counts + ((canon - a.count_base) * S + s) * nb2;
     unsigned blank = 0, single = 0;
-    double lam = 0.0, tk = 1.0 / (double)nb;
-    if (est == kMM) {
-        for (unsigned q = 0; q < nb2; ++q) if (f[q] == 0) ++blank;
-    } else if (est == kCS) {
-        for (unsigned q = 0; q < nb2; ++q) if (f[q] == 1) ++single;
-    } else if (est == kJS) {
-        double s2 = 0, tks = 0;
-        for (unsigned q = 0; q < nb2; ++q) {
-            const double fq = f[q] / n;
-            s2 += fq * fq;
-            tks += ((tk - fq) * (tk - fq));
-        }
-        lam = (1 - s2) / ((n - 1) * tks);
+    double sq = 0.0, tks = 0.0;
+    const double tk = 1.0 / (double)nb;
+    for (unsigned q = lane; q < nb2; q += 64) {
+        const uint32_t c = f[q];
+        blank += c == 0;
+        single += c == 1;
+        const double fq = c / n;
+        sq += fq * fq;
+        tks += (tk - fq) * (tk - fq);
     }
-    double sum = 0.0, jf1 = 1.0, jf2 = 1.0;
-    for (int r = 0; r < nb; ++r) {
-        if (hj1) jf1 = jac(t1, mn1 + (r * w1) + hw1);
-        double row = 0.0;
-        for (int c = 0; c < nb; ++c) {
+    blank = wave_sum_u(blank);
+    single = wave_sum_u(single);
+    sq = wave_sum(sq);
+    tks = wave_sum(tks);
+    const double lam = (1 - sq) / ((n - 1) * tks);
+    const double cov = 1 - (single / n);
+    for (int e = 0; e < nest; ++e) {
+        const int est = a.ests[e];
+        double part = 0.0;
+        for (unsigned q = lane; q < nb2; q += 64) {
+            const unsigned r = q / nb, c = q - r * nb;
             const uint32_t fc = f[r * rs + c * cs];
-            double q = fc / n;
-            if (est == kCS) q = (1 - (single / n)) * (fc / n);
-            else if (est == kJS) q = (lam * tk) + ((1 - lam) * q);
-            double pd = q / area;
+            double p = fc / n;
+            if (est == kCS) p = cov * (fc / n);
+            else if (est == kJS) p = (lam * tk) + ((1 - lam) * p);
+            double pd = p / area;
             if (pd > 0.0) {
-                double denom = 1.0;
+                double denom = 1.0, jf2;
                 if (est == kCS) {
-                    denom = 1 - pow((1 - q), n);
-                    if (hj2) jf2 = jac(t2, mn2 + (r * w2) + hw2);
-                } else if (hj2) {
-                    jf2 = jac(t2, mn2 + (c * w2) + hw2);
+                    if (n * p <= 40.0) denom = 1 - pow((1 - p), n);
+                    jf2 = j2[r];
+                } else {
+                    jf2 = j2[c];
                 }
                 if (a.jacobian) {
-                    const double jf = jf1 * jf2;
+                    const double jf = j1[r] * jf2;
                     pd = pd / jf;
-                    row += (est == kCS) ? (jf * pd * log(pd) / denom) : (jf * pd * log(pd));
+                    part += (est == kCS) ? (jf * pd * log(pd) / denom) : (jf * pd * log(pd));
                 } else {
-                    row += (est == kCS) ? (pd * log(pd) / denom) : (pd * log(pd));
+                    part += (est == kCS) ? (pd * log(pd) / denom) : (pd * log(pd));
                 }
             }
         }
-        row *= -area;
-        sum += row;
+        double sum = wave_sum(part) * -area;
+        if (est == kMM) sum += ((nb2 - blank) - 1) / (2.0 * n);
+        if (lane == 0) {
+            a.s2[(rec * nest + e) * S + s] = sum;
+            if (e == 0 && a.mi)
+                a.mi[rec * S + s] = (a.s1[((uint64_t)i * nest) * S + s] +
+                                     a.s1[((uint64_t)j * nest) * S + s]) - sum;
+        }
     }
-    if (est == kMM) sum += ((nb2 - blank) - 1) / (2.0 * n);
-    a.s2[((size_t)p * nest + e) * S + s] = sum;
+    if (a.counts_out) {
+        uint32_t* o = a.counts_out + (rec * S + s) * nb2;
+        for (unsigned q = lane; q < nb2; q += 64) {
+            const unsigned r = q / nb, c = q - r * nb;
+            o[q] = f[r * rs + c * cs];
+        }
+    }
 }
 
 __global__ void gather_rec_kernel(const double* src, const uint64_t* rec2canon,
@@ -250,22 +315,6 @@ __global__ void gather_rec_kernel(const double* src, const uint64_t* rec2canon,
     const uint64_t r = t / width;
     const int k = (int)(t - r * width);
     dst[t] = src[rec2canon[r] * width + k];
-}
-
-__global__ void gather_counts_kernel(const uint32_t* src, const uint64_t* rec2canon,
-                                     const uint32_t* rec_ij, uint64_t n_rec, int S, int nb,
-                                     uint32_t* dst) {
-    const uint64_t nb2 = (uint64_t)nb * nb;
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n_rec * S * nb2) return;
-    const uint64_t q = t % nb2;
-    const uint64_t rs = t / nb2;           // r * S + s
-    const uint64_t r = rs / S;
-    const uint64_t s = rs - r * S;
-    const bool flip = rec_ij[2 * r] > rec_ij[2 * r + 1];
-    const uint64_t x = q / nb, y = q - x * nb;
-    const uint64_t qs = flip ? y * nb + x : q;
-    dst[t] = src[(rec2canon[r] * S + s) * nb2 + qs];
 }
 
 // Pitched copy of height rows of width bytes, in 8-byte words when every
@@ -298,16 +347,6 @@ hipError_t launch_copy_2d(void* dst, size_t dpitch, const void* src, size_t spit
     return hipGetLastError();
 }
 
-hipError_t launch_gather_counts(const uint32_t* src, const uint64_t* rec2canon,
-                                const uint32_t* rec_ij, uint64_t n_rec, int nsteps, int nb,
-                                uint32_t* dst, hipStream_t st) {
-    const uint64_t n = n_rec * nsteps * (uint64_t)nb * nb;
-    if (!n) return hipSuccess;
-    hipLaunchKernelGGL(gather_counts_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
-                       src, rec2canon, rec_ij, n_rec, nsteps, nb, dst);
-    return hipGetLastError();
-}
-
 hipError_t launch_dof1d(const Dof1dArgs& a, hipStream_t st) {
     if (a.n_dof == 0) return hipSuccess;
     hipLaunchKernelGGL(dof1d_kernel, dim3(a.n_dof), dim3(256), 0, st, a);
@@ -315,9 +354,9 @@ hipError_t launch_dof1d(const Dof1dArgs& a, hipStream_t st) {
 }
 
 hipError_t launch_count_entropy(const CountEntropyArgs& a, hipStream_t st) {
-    const uint64_t n = a.n_pair * a.nsteps * a.nest;
-    if (!n) return hipSuccess;
-    hipLaunchKernelGGL(count_entropy_kernel, dim3((unsigned)((n + 63) / 64)),
+    if (!a.n_item) return hipSuccess;
+    if (a.n_item > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(count_entropy_kernel, dim3((unsigned)a.n_item, (unsigned)a.nsteps),
                        dim3(64), 0, st, a);
     return hipGetLastError();
 }

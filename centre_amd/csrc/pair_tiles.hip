@@ -484,7 +484,8 @@ __device__ __forceinline__ void tile_body(const PairKernelArgs& a_in, int tix) {
                                 if (p >= 0) {
                                     const int bi_ = grow - rd * nb, bj_ = gcol - cd * nb;
                                     const int rec = a.split_rec ? a.split_rec : nb * nb;
-                                    a.counts[((size_t)p * S + s) * rec + (size_t)bi_ * nb + bj_] =
+                                    a.counts[((size_t)(p - (long long)a.count_base) * S + s) * rec +
+                                             (size_t)bi_ * nb + bj_] =
                                         (uint32_t)cnt;
                                 }
                             }

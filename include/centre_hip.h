@@ -127,8 +127,11 @@ int centre_pair_plan_tiles(const centre_pair_plan *plan);
  * pairs of the plan (EntropyCalculator::run2d_xx / run2d_xy,
  * EntropyCalculator.cpp:281-1088), plus MI = S1_i + S1_j - S2_ij
  * (getMaxMI2D, Convergance.cpp:183-194).  Joint counts are exact int32 MFMA
- * contractions of one-hot codes; ML and MM are fused into the count pass,
- * CS and JS go through materialised counts.  The marginals of the same
+ * contractions of one-hot codes; ML and MM are fused into the count pass.
+ * With CS or JS among the estimators, or with d_counts, the counts of groups
+ * of row blocks (at most 2 GiB at a time, or one row block) are materialised
+ * and every estimator and MI is evaluated from them on the device, each
+ * record in its own row/column order.  The marginals of the same
  * frames are computed inside the call; d_s1 (optional) receives them as
  * [n_dof][nest][nsteps].  d_counts (optional, NULL = never materialised):
  * [record][nsteps][nbins*nbins] uint32 prefix counts, cell x*nbins + y for

@@ -391,17 +391,6 @@ def test_empty_pair_list(ctx):
     assert np.all(np.isfinite(s1))
 
 
-def test_count_materialisation_limit_is_an_error(ctx):
-    """Chao-Shen and James-Stein go through materialised counts; a plan whose
-    counts would exceed 16 GiB is refused (-3) instead of allocated."""
-    D, F, nb, S = 3000, 64, 50, 10
-    codes = np.zeros((D, F), np.uint8)
-    with pytest.raises(hip.CentreError) as e:
-        ctx.pair_entropy(codes, np.ones(D, np.uint8), np.tile([1.0, 2.0, 0, 0], (D, 1)), nb, S,
-                         ests=(2,), want_s1=False, want_mi=False)
-    assert e.value.code == -3
-
-
 @pytest.mark.parametrize("nb,D,F,S,ests,explicit", [
     (5, 40, 20000, 3, (0,), False),      # one tile, eight parts, short last part
     (16, 50, 9000, 2, (1, 0), True),     # ten tiles, MI from MM
