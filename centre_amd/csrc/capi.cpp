@@ -1,5 +1,6 @@
 // extern "C" layer: argument checking, device scratch, plans, launches.
 #include <algorithm>
+#include <mutex>
 #include <numeric>
 #include <cstdio>
 #include <cmath>
@@ -57,6 +58,33 @@ hipError_t ctx_scratch(centre_ctx* ctx, int slot, size_t bytes, void** out) {
     return hipSuccess;
 }
 
+// Scratch of the allocation-free pair stage (centre_dev_pair_entropy_noalloc):
+// one static array per GPU, which the loader places with the code object.
+// Host code never uses its name as an address: centre_init resolves the
+// device address once per device (hipGetSymbolAddress, with that device
+// current) and the entry point passes that address to the kernels.
+constexpr size_t kNoAllocBytes = CENTRE_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (which must be current).
+hipError_t noalloc_address(int device, void** out) {
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    if (!g_noalloc_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_noalloc_ptr[device] = p;
+    }
+    *out = g_noalloc_ptr[device];
+    return hipSuccess;
+}
+
 int check_flag(centre_ctx* ctx) {
     int f = 0;
     HIPCHK(hipMemcpyAsync(&f, ctx->d_flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -91,7 +119,8 @@ int centre_init(int device, centre_ctx** out) {
     if ((e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipMalloc(&ctx->d_flag, sizeof(int))) != hipSuccess ||
         (e = hipEventCreate(&ctx->ev0)) != hipSuccess ||
-        (e = hipEventCreate(&ctx->ev1)) != hipSuccess) {
+        (e = hipEventCreate(&ctx->ev1)) != hipSuccess ||
+        (e = noalloc_address(device, &ctx->noalloc)) != hipSuccess) {
         centre_destroy(ctx);
         return (int)e;
     }
@@ -1192,6 +1221,50 @@ extern "C" int centre_dev_pair_entropy(centre_ctx* ctx, const centre_pair_plan* 
         HIPCHK(ctx_scratch(ctx, 2, l.split, &sp));
         ws.split = static_cast<uint32_t*>(sp);
     }
+    return run_pair_fused(ctx, plan, d_codes, code_stride, n_frame, d_types, d_extrema, nsteps,
+                          ests, nest, jacobian, kde, d_s1, d_s2, d_mi, l, ws);
+}
+
+extern "C" int centre_dev_pair_entropy_noalloc(centre_ctx* ctx, const centre_pair_plan* plan,
+                                               const uint8_t* d_codes, uint64_t code_stride,
+                                               uint64_t n_frame, const uint8_t* d_types,
+                                               const double* d_extrema, int nsteps,
+                                               const int* ests, int nest, int jacobian,
+                                               int kde, double* d_s1, double* d_s2,
+                                               double* d_mi) {
+    if (!ctx) return -3;
+    if (int r = check_pair_args(ctx, plan, d_codes, code_stride, n_frame, d_types, d_extrema,
+                                nsteps, ests, nest, d_s2))
+        return r;
+    if (!fused_estimators(ests, nest))
+        return fail(ctx, -3, "the allocation-free pair stage takes the ML and MM estimators only");
+    if (!ctx->noalloc) return fail(ctx, -3, "no static scratch on this device");
+    HIPCHK(hipSetDevice(ctx->device));
+    PairLayout l = pair_layout(plan, d_codes, code_stride, n_frame, nsteps, nest, d_mi != nullptr);
+    const size_t base = l.base_total();
+    if (base > kNoAllocBytes)
+        return fail(ctx, -3, "the temporaries of this call (" + std::to_string(base) +
+                                 " bytes) exceed the static scratch (" +
+                                 std::to_string(kNoAllocBytes) + " bytes)");
+    // the frame split only where its count dumps fit what is left
+    if (int r = choose_split(ctx, plan, n_frame / nsteps, nsteps, kNoAllocBytes - base, &l))
+        return r;
+    uint8_t* next = static_cast<uint8_t*>(ctx->noalloc);
+    auto take = [&next](size_t bytes) -> void* {
+        if (!bytes) return nullptr;
+        void* p = next;
+        next += (bytes + 255) & ~(size_t)255;
+        return p;
+    };
+    PairWork ws;
+    ws.s1 = static_cast<double*>(take(l.s1));
+    ws.L = static_cast<double*>(take(l.L));
+    ws.w = static_cast<double*>(take(l.w));
+    ws.s1e0 = static_cast<double*>(take(l.s1e0));
+    ws.packed = static_cast<uint8_t*>(take(l.packed));
+    ws.cs2 = static_cast<double*>(take(l.cs2));
+    ws.cmi = static_cast<double*>(take(l.cmi));
+    ws.split = static_cast<uint32_t*>(take(l.split));
     return run_pair_fused(ctx, plan, d_codes, code_stride, n_frame, d_types, d_extrema, nsteps,
                           ests, nest, jacobian, kde, d_s1, d_s2, d_mi, l, ws);
 }

@@ -23,6 +23,9 @@ struct centre_ctx {
     // 4: count dumps of the grouped estimator pass (CS / JS, count output).
     void* scratch[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     size_t scratch_bytes[5] = {0, 0, 0, 0, 0};
+    // device address of the library's static scratch array on this device
+    // (the allocation-free pair-stage entry point carves it)
+    void* noalloc = nullptr;
 };
 
 struct centre_pair_plan {

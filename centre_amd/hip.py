@@ -50,6 +50,8 @@ _SIGS = {
     "centre_pair_plan_tiles": ([P], I32),
     "centre_dev_pair_entropy": ([P, P, P, U64, U64, P, P, I32, P, I32, I32, I32, P, P, P,
                                  P], I32),
+    "centre_dev_pair_entropy_noalloc": ([P, P, P, U64, U64, P, P, I32, P, I32, I32, I32, P,
+                                         P, P], I32),
     "centre_pair_entropy": ([P, P, U32, U64, P, P, I32, I32, P, U64, P, I32, I32, I32,
                              P, P, P], I32),
     "centre_dev_mst": ([P, P, P, P, U64, U32, P, P], I32),
@@ -322,6 +324,13 @@ class Context:
             self.h, plan.h, d_codes, stride, n_frame, d_types, d_ext, nsteps,
             _p(ests), len(ests), int(jacobian), int(kde), d_s1, d_s2, d_mi,
             d_counts))
+
+    def dev_pair_entropy_noalloc(self, plan: "PairPlan", d_codes, stride, n_frame, d_types,
+                                 d_ext, nsteps, ests, jacobian, kde, d_s1, d_s2, d_mi):
+        ests = _c(np.atleast_1d(ests), np.int32)
+        self.check(self._L.centre_dev_pair_entropy_noalloc(
+            self.h, plan.h, d_codes, stride, n_frame, d_types, d_ext, nsteps,
+            _p(ests), len(ests), int(jacobian), int(kde), d_s1, d_s2, d_mi))
 
     def dev_mst(self, d_src, d_dst, d_w, n_edge, n_node):
         out = np.zeros(max(n_node, 1), np.uint32)

@@ -145,6 +145,32 @@ int centre_dev_pair_entropy(centre_ctx *ctx, const centre_pair_plan *plan,
                             double *d_s1, double *d_s2, double *d_mi,
                             uint32_t *d_counts);
 
+/* Bytes of the static device array behind centre_dev_pair_entropy_noalloc
+ * (one per GPU the library runs on, held until the process ends). */
+#define CENTRE_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): centre_dev_pair_entropy for the ML and MM estimators
+ * without count output, calling no allocator.  Its temporaries (marginals,
+ * per-DOF widths and Jacobian sums, a step-aligned copy of misaligned codes,
+ * canonical-order outputs of explicit pair lists, frame-split count dumps)
+ * live in one static device array of CENTRE_NOALLOC_SCRATCH_BYTES per GPU;
+ * the plan is made beforehand (centre_pair_plan_create allocates).  Returns
+ * -3 when this call's temporaries exceed the array (the frame split is then
+ * left out first; it is used only where its dumps fit).  Every call on one
+ * GPU shares the array and leaves its contents behind: calls that overlap on
+ * one GPU (several contexts or host threads) race. */
+int centre_dev_pair_entropy_noalloc(centre_ctx *ctx,
+                                    const centre_pair_plan *plan,
+                                    const uint8_t *d_codes,
+                                    uint64_t code_stride, uint64_t n_frame,
+                                    const uint8_t *d_types,
+                                    const double *d_extrema, int nsteps,
+                                    const int *ests, int nest, int jacobian,
+                                    int kde, double *d_s1, double *d_s2,
+                                    double *d_mi);
+
 /* Host-buffer convenience: discretized codes in, S1/S2/MI out. */
 int centre_pair_entropy(centre_ctx *ctx, const uint8_t *codes,
                         uint32_t n_dof, uint64_t n_frame,
