@@ -1225,6 +1225,63 @@ extern "C" int centre_dev_pair_entropy(centre_ctx* ctx, const centre_pair_plan* 
                           ests, nest, jacobian, kde, d_s1, d_s2, d_mi, l, ws);
 }
 
+extern "C" int centre_dev_pair_entropy_schemes(centre_ctx* ctx,
+                                               const centre_pair_plan* const* plans,
+                                               int n_schemes, const uint8_t* d_codes,
+                                               uint64_t code_stride, uint64_t n_frame,
+                                               const uint8_t* d_types, const double* d_extrema,
+                                               int nsteps, const int* ests, int nest,
+                                               int jacobian, int kde, double* d_s1,
+                                               double* d_s2, double* d_mi) {
+    if (!ctx) return -3;
+    if (!plans || n_schemes < 1 || n_schemes > 255) return fail(ctx, -3, "need 1..255 schemes");
+    for (int sc = 0; sc < n_schemes; ++sc) {
+        if (!plans[sc]) return fail(ctx, -3, "null plan");
+        if (plans[sc]->n_dof != plans[0]->n_dof || plans[sc]->n_rec != plans[0]->n_rec ||
+            plans[sc]->all_pairs != plans[0]->all_pairs ||
+            plans[sc]->tile_subset != plans[0]->tile_subset ||
+            plans[sc]->h_rec_ij != plans[0]->h_rec_ij)
+            return fail(ctx, -3, "the schemes' plans must hold the same DOFs and pairs");
+    }
+    if (int r = check_pair_args(ctx, plans[0], d_codes, code_stride, n_frame, d_types, d_extrema,
+                                nsteps, ests, nest, d_s2))
+        return r;
+    if (n_schemes == 1)
+        return centre_dev_pair_entropy(ctx, plans[0], d_codes, code_stride, n_frame, d_types,
+                                       d_extrema, nsteps, ests, nest, jacobian, kde, d_s1, d_s2,
+                                       d_mi, nullptr);
+    HIPCHK(hipSetDevice(ctx->device));
+    const uint64_t D = plans[0]->n_dof, P = plans[0]->n_rec, S = (uint64_t)nsteps;
+    DevBuf s1, s2, mi;
+    HIPCHK(s1.alloc(sizeof(double) * D * nest * S));
+    HIPCHK(s2.alloc(sizeof(double) * P * nest * S));
+    if (d_mi) HIPCHK(mi.alloc(sizeof(double) * P * S));
+    const size_t row = sizeof(double) * S;   // one step series
+    float total_ms = 0.f;
+    for (int sc = 0; sc < n_schemes; ++sc) {
+        // scheme sc of DOF d is row d * n_schemes + sc of the code matrix
+        if (int r = centre_dev_pair_entropy(ctx, plans[sc], d_codes + (size_t)sc * code_stride,
+                                            code_stride * n_schemes, n_frame, d_types, d_extrema,
+                                            nsteps, ests, nest, jacobian, kde, s1.as<double>(),
+                                            s2.as<double>(), d_mi ? mi.as<double>() : nullptr,
+                                            nullptr))
+            return r;
+        total_ms += ctx->last_pair_ms;
+        // [x][S] series into the [x][n_schemes][S] layout
+        HIPCHK(launch_copy_2d(reinterpret_cast<uint8_t*>(d_s2) + sc * row, row * n_schemes,
+                              s2.p, row, row, P * nest, ctx->stream));
+        if (d_s1)
+            HIPCHK(launch_copy_2d(reinterpret_cast<uint8_t*>(d_s1) + sc * row, row * n_schemes,
+                                  s1.p, row, row, D * nest, ctx->stream));
+        if (d_mi)
+            HIPCHK(launch_copy_2d(reinterpret_cast<uint8_t*>(d_mi) + sc * row, row * n_schemes,
+                                  mi.p, row, row, P, ctx->stream));
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->last_pair_ms = total_ms;
+    return 0;
+}
+
 extern "C" int centre_dev_pair_entropy_noalloc(centre_ctx* ctx, const centre_pair_plan* plan,
                                                const uint8_t* d_codes, uint64_t code_stride,
                                                uint64_t n_frame, const uint8_t* d_types,

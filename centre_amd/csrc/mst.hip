@@ -16,7 +16,10 @@
 
 namespace {
 
+// -0.0 and +0.0 compare equal in the reference (operator <), so both map to
+// the key of +0.0 and the edge index breaks the tie.
 __device__ inline unsigned long long order_key(double w) {
+    if (w == 0.0) w = 0.0;
     const unsigned long long b = (unsigned long long)__double_as_longlong(w);
     return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
 }

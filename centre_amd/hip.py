@@ -50,6 +50,8 @@ _SIGS = {
     "centre_pair_plan_tiles": ([P], I32),
     "centre_dev_pair_entropy": ([P, P, P, U64, U64, P, P, I32, P, I32, I32, I32, P, P, P,
                                  P], I32),
+    "centre_dev_pair_entropy_schemes": ([P, P, I32, P, U64, U64, P, P, I32, P, I32, I32, I32,
+                                         P, P, P], I32),
     "centre_dev_pair_entropy_noalloc": ([P, P, P, U64, U64, P, P, I32, P, I32, I32, I32, P,
                                          P, P], I32),
     "centre_pair_entropy": ([P, P, U32, U64, P, P, I32, I32, P, U64, P, I32, I32, I32,
@@ -324,6 +326,15 @@ class Context:
             self.h, plan.h, d_codes, stride, n_frame, d_types, d_ext, nsteps,
             _p(ests), len(ests), int(jacobian), int(kde), d_s1, d_s2, d_mi,
             d_counts))
+
+    def dev_pair_entropy_schemes(self, plans, d_codes, stride, n_frame, d_types, d_ext,
+                                 nsteps, ests, jacobian, kde, d_s1, d_s2, d_mi):
+        """plans: one PairPlan per scheme; codes [D][nsch][stride]."""
+        ests = _c(np.atleast_1d(ests), np.int32)
+        arr = (C.c_void_p * len(plans))(*[pl.h.value for pl in plans])
+        self.check(self._L.centre_dev_pair_entropy_schemes(
+            self.h, C.cast(arr, C.c_void_p), len(plans), d_codes, stride, n_frame, d_types,
+            d_ext, nsteps, _p(ests), len(ests), int(jacobian), int(kde), d_s1, d_s2, d_mi))
 
     def dev_pair_entropy_noalloc(self, plan: "PairPlan", d_codes, stride, n_frame, d_types,
                                  d_ext, nsteps, ests, jacobian, kde, d_s1, d_s2, d_mi):

@@ -521,15 +521,13 @@ void Host::run_entropy() {
         if (hist_on(KINDS[k].set)) hist2[k].assign(pairs[k].size() / 2 * Se * bsq, 0);
     std::vector<double> kernel_ms;
     // Every pair of the concatenated subset DOFs (all six kinds, no neighbour
-    // list, no 2-D count files, fused estimators only): one all-pairs launch,
-    // whose triangle order then gives each kind's records.  Otherwise one
-    // explicit plan per kind.
+    // list, no 2-D count files): one all-pairs launch, whose triangle order
+    // then gives each kind's records.  Otherwise one explicit plan per kind.
     uint64_t n_req = 0;
     for (int k = 0; k < 6; ++k) n_req += pairs[k].size() / 2;
     bool allpairs = !(in.usesubset && in.useneighbor) &&
                     n_req == (uint64_t)Dall * (Dall - 1) / 2 && Dall >= 2;
     for (int k = 0; k < 6; ++k) allpairs = allpairs && hist2[k].empty();
-    for (int e : in.estimators) allpairs = allpairs && e <= 1;   // ML / MM: counts never stored
     for (int sc = 0; sc < nsch; ++sc) {
         std::vector<uint8_t> all((size_t)Dall * Fe);
         for (int ty = 0; ty < 3; ++ty)
@@ -839,7 +837,7 @@ void usage() {
                  "       centre_hip --inspect <file.nc> [var ...]\n"
                  "  print the format, global text attributes and, per variable,\n"
                  "  its shape and the sum of its values (no GPU is used)\n"
-                 "       centre_hip --print-config <input.cfg>\n"
+                 "       centre_hip --print-config <input.cfg> [frames in the trajectory]\n"
                  "  print the parsed input parameters (no GPU is used)\n";
 }
 
@@ -867,12 +865,15 @@ int inspect(int argc, char** argv, int first) {
     return 0;
 }
 
-int print_config(const char* path) {
+// frames_in_file > 0: also apply the values derived from a trajectory of
+// that many frames (frame range clamps, defaults) before printing.
+int print_config(const char* path, uint64_t frames_in_file) {
     std::ifstream f(path);
     if (!f) throw Fail(std::string("cannot read input file ") + path);
     std::stringstream ss;
     ss << f.rdbuf();
     Inputs in = inputs_from_symbols(parse_ini(ss.str()));
+    if (frames_in_file > 0 && in.ok()) derive(in, frames_in_file);
     for (auto& e : in.errors) std::cout << "error " << e << "\n";
     std::cout << "nsteps " << in.nsteps << "\ndiscretize " << in.discretize << "\ncalcentropy "
               << in.calcentropy << "\ngenconvgdata " << in.genconvgdata << "\ninfilepath "
@@ -885,7 +886,9 @@ int print_config(const char* path) {
               << "\njacobian " << in.jacobian << "\nestimators";
     for (auto e : in.estimators) std::cout << " " << EST_NAME[e];
     std::cout << "\nmethod " << (int)in.method << "\nkappa " << in.kappa << "\nnmaxconf "
-              << in.nmaxconf << "\ncacheentcdimspercpu " << in.cacheentcdimspercpu << "\n";
+              << in.nmaxconf << "\ncacheentcdimspercpu " << in.cacheentcdimspercpu
+              << "\nstartframe " << in.startframe << "\nstrideframe " << in.strideframe
+              << "\nnframe " << in.nframe << "\n";
     return in.ok() ? 0 : 1;
 }
 
@@ -896,7 +899,8 @@ int main(int argc, char** argv) {
     bool overwrite = false;
     try {
         if (argc >= 3 && !std::strcmp(argv[1], "--inspect")) return inspect(argc, argv, 2);
-        if (argc >= 3 && !std::strcmp(argv[1], "--print-config")) return print_config(argv[2]);
+        if (argc >= 3 && !std::strcmp(argv[1], "--print-config"))
+            return print_config(argv[2], argc >= 4 ? std::strtoull(argv[3], nullptr, 10) : 0);
     } catch (const std::exception& e) {
         std::cerr << "centre_hip: " << e.what() << "\n";
         return 2;

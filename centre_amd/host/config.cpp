@@ -156,11 +156,19 @@ Inputs inputs_from_symbols(const Symbols& sym) {
         in.errors.push_back("histogram.referencenbins has no default and must be given");
     if (in.nsteps < 1 || in.nsteps > 255) in.errors.push_back("control.nsteps must be in 1..255");
     if (in.fnames.empty()) in.errors.push_back("discretization.fname is required");
+    // a stride of 0 never advances (the reference divides by it,
+    // Inputs.cpp:639, Netcdf_TrjInt.cpp:122)
+    if (in.strideframe < 1) in.errors.push_back("entropy.strideframe must be at least 1");
     return in;
 }
 
 void derive(Inputs& in, uint64_t frames_in_file) {
     if (in.nframe == 0 || in.nframe > frames_in_file) in.nframe = frames_in_file;
+    // the frame reader's clamps (Netcdf_TrjInt.cpp:120-121): a start at or
+    // past the last frame reads from frame 0, a stride longer than the
+    // trajectory reads every frame
+    if (in.startframe >= in.nframe) in.startframe = 0;
+    if (in.strideframe > in.nframe) in.strideframe = 1;
     if (in.pdfmethod == PdfMethod::VonMisesKDE) {
         in.referencenbins = in.nmaxconf;
         in.nbins.assign(1, (uint8_t)in.nmaxconf);

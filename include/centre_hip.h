@@ -145,6 +145,25 @@ int centre_dev_pair_entropy(centre_ctx *ctx, const centre_pair_plan *plan,
                             double *d_s1, double *d_s2, double *d_mi,
                             uint32_t *d_counts);
 
+/* Several bin schemes in one call, as entropy2D loops its bin_schemes
+ * (Estimators.cpp:408-647) and binData2D bins each scheme's codes
+ * (Estimators.cpp:130-205): plans[s] is the plan of scheme s (the same DOFs
+ * and pair list for every scheme, nbins = that scheme's bin count); codes
+ * [n_dof][n_schemes][code_stride] (the coord layout of bin_*.nc,
+ * Netcdf_TrjInt.cpp:542-755).  Outputs in the reference's contri_S layout
+ * (Netcdf_EntContri.cpp:55-73): S1 [n_dof][nest][n_schemes][nsteps] (optional),
+ * S2 [record][nest][n_schemes][nsteps], MI [record][n_schemes][nsteps]
+ * (optional, estimator ests[0]). */
+int centre_dev_pair_entropy_schemes(centre_ctx *ctx,
+                                    const centre_pair_plan *const *plans,
+                                    int n_schemes, const uint8_t *d_codes,
+                                    uint64_t code_stride, uint64_t n_frame,
+                                    const uint8_t *d_types,
+                                    const double *d_extrema, int nsteps,
+                                    const int *ests, int nest, int jacobian,
+                                    int kde, double *d_s1, double *d_s2,
+                                    double *d_mi);
+
 /* Bytes of the static device array behind centre_dev_pair_entropy_noalloc
  * (one per GPU the library runs on, held until the process ends). */
 #define CENTRE_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)

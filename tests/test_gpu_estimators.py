@@ -162,3 +162,46 @@ def test_tile_subset_plan_non_fused_writes_only_listed_pairs(ests):
     assert rel_err(s2[inside], w2[:, :, 0, :]) < REL
     mi_w = w1[pairs[:, 0], 0, 0, :] + w1[pairs[:, 1], 0, 0, :] - w2[:, 0, 0, :]
     assert np.max(np.abs(mi[inside] - mi_w)) < 1e-10
+
+
+@pytest.mark.parametrize("explicit", [False, True])
+def test_several_bin_schemes_in_one_call(explicit):
+    """Bin schemes 30 and 50 of a 48-DOF trajectory discretised together
+    (codes [D][2][F]); S1, S2 and MI in the [record][estimator][scheme][step]
+    layout of contri_S, against the oracle's entropy2D over both schemes."""
+    F, S, schemes, ests = 8000, 4, [30, 50], (0, 2)
+    bat = hip.synth_bat(0xCE27E, 16, 16, 16, F)
+    types = np.array([1] * 16 + [2] * 16 + [3] * 16, np.uint8)
+    D = 48
+    with hip.Context(0) as ctx:
+        codes = np.zeros((D, 2, F), np.uint8)
+        ext = np.zeros((D, 4))
+        for t, sl in ((1, slice(0, 16)), (2, slice(16, 32)), (3, slice(32, 48))):
+            c, e = ctx.discretize_hist(bat[:, sl], t, schemes, 30, True)
+            codes[sl] = c
+            ext[sl] = e
+        iu, ju = np.triu_indices(D, 1)
+        pairs = np.stack([iu, ju], 1)
+        if explicit:
+            rng = np.random.default_rng(4)
+            pairs = pairs[rng.choice(len(pairs), 300, replace=False)]
+            pairs[::3] = pairs[::3, ::-1]
+        P = len(pairs)
+        plans = [hip.PairPlan(ctx, D, nb, pairs.astype(np.uint32) if explicit else None)
+                 for nb in schemes]
+        dc, dt, dx = ctx.to_device(codes), ctx.to_device(types), ctx.to_device(ext)
+        d_s1 = ctx.alloc((D, len(ests), 2, S), np.float64)
+        d_s2 = ctx.alloc((P, len(ests), 2, S), np.float64)
+        d_mi = ctx.alloc((P, 2, S), np.float64)
+        print(f"two schemes explicit={explicit} P={P}", flush=True)
+        ctx.dev_pair_entropy_schemes(plans, dc.ptr, F, F, dt.ptr, dx.ptr, S, ests, True, False,
+                                     d_s1.ptr, d_s2.ptr, d_mi.ptr)
+        s1, s2, mi = d_s1.numpy(), d_s2.numpy(), d_mi.numpy()
+        for pl in plans:
+            pl.close()
+    w1 = O.dof_entropies(codes, types, ext, S, schemes, ests)
+    w2 = O.pair_entropies(codes, types, ext, pairs, S, schemes, ests)
+    assert rel_err(s1, w1) < REL
+    assert rel_err(s2, w2) < REL
+    mi_w = w1[pairs[:, 0], 0] + w1[pairs[:, 1], 0] - w2[:, 0]
+    assert np.max(np.abs(mi - mi_w)) < 1e-10

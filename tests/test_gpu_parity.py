@@ -276,6 +276,24 @@ def test_mst_vs_oracle(ctx, seed):
     assert got.tolist() == kruskal_set(n, src.tolist(), dst.tolist(), w.tolist())
 
 
+def test_mst_signed_zero_weights_tie(ctx):
+    """-0.0 and +0.0 compare equal in boruvkaMST (operator <, Centre.cpp:43-53),
+    so among zero-weight edges the lowest index wins whatever the sign."""
+    rng = np.random.default_rng(7)
+    n, m = 60, 400
+    src = rng.integers(0, n, m)
+    dst = rng.integers(0, n, m)
+    keep = src != dst
+    src, dst = src[keep], dst[keep]
+    w = rng.choice([0.0, -0.0, 0.25], size=len(src))
+    w[0::3] = -0.0
+    assert np.signbit(w).any() and (~np.signbit(w) & (w == 0)).any()
+    got = ctx.mst(n, src, dst, w)
+    want = O.boruvka(n, src, dst, w)
+    assert sorted(got.tolist()) == sorted(want.tolist())
+    assert got.tolist() == kruskal_set(n, src.tolist(), dst.tolist(), w.tolist())
+
+
 def test_mst_neighbour_graph_scale(ctx):
     """The neighbour-list graph of the vmKDE workload's size (3,000 nodes,
     pairs within 30 of each other, ~90k edges) with weights from a small set
@@ -334,6 +352,10 @@ def test_vmkde_grid_and_states(ctx):
             assert nf[d] == wn
             assert np.array_equal(mins[d, :wn], wm)
             assert np.array_equal(codes[d], wc)
+            # end to end: the oracle's own grid gives the same states
+            ec, em, en = O.vmkde(x, nmax, kappa, 5, 1e-4, 1000, G)
+            assert nf[d] == en and np.array_equal(codes[d], ec)
+            assert np.max(np.abs(mins[d, :en] - em), initial=0.0) <= 1e-12
 
 
 def test_vmkde_eight_states():
@@ -361,6 +383,9 @@ def test_vmkde_eight_states():
                 assert wn >= 6 and nf[d] == wn
                 assert np.array_equal(mins[d, :wn], wm)
                 assert np.array_equal(codes[d], wc)
+                ec, em, en = O.vmkde(x, 9, kappa, 5, 1e-4, 1000, G)
+                assert nf[d] == en and np.array_equal(codes[d], ec)
+                assert np.max(np.abs(mins[d, :en] - em), initial=0.0) <= 1e-12
 
 
 # ---------------------------------------------------------------------------
@@ -524,6 +549,17 @@ def test_full_size_vmkde_properties():
         wc, wm, wn = O.vmkde_states(x, grid[d].reshape(-1), nmax, 5, 1e-4, 1000, G)
         assert nf[d] == wn and np.array_equal(mins[d, :wn], wm)
         assert np.array_equal(codes[d], wc)
+    # end to end on 40 strided torsions: the oracle's own grid at full size
+    # (h = 643 for 50k frames) gives the device's grid within 1e-11 of its
+    # largest value, the same state count and codes, and minima within 1e-12
+    for d in range(0, D, D // 40):
+        x = bat[:, d].astype(np.float32)
+        wg = O.vmkde_grid(x, kappa, G)
+        g = grid[d].reshape(-1)
+        assert np.max(np.abs(g[: G + 1] - wg[: G + 1])) <= 1e-11 * np.max(np.abs(wg[: G + 1]))
+        ec, em, en = O.vmkde_states(x, wg, nmax, 5, 1e-4, 1000, G)
+        assert nf[d] == en and np.array_equal(codes[d], ec), d
+        assert np.max(np.abs(mins[d, :en] - em), initial=0.0) <= 1e-12
     sample = np.arange(0, P, P // 500)[:500]
     w2 = O.pair_entropies(codes, types, np.zeros((D, 4)), pairs[sample], S, [nmax], (0,),
                           jacobian=True, kde=True)

@@ -1,0 +1,224 @@
+"""The batched and the learned-screen workloads at their full sizes, checked
+against the oracle where it finishes in seconds and through size-independent
+properties elsewhere.
+
+* Batch: 64 systems with D ~ U[200, 2000] DOFs and F ~ U[20k, 200k] frames
+  (the benchmark's seeded draw), 50 bins, 10 steps, one batched call.  Every
+  system: finite S2, ML MI >= 0 for every pair and step (a Kullback-Leibler
+  divergence), a strided sample of pairs against the oracle's entropy2D; every
+  eighth system: all S1 against the oracle's entropy1D.
+* Screen: 6,000 DOFs x 100,000 frames (17,997,000 pairs).  The plain pass
+  against the same properties and a strided oracle sample; then a screen
+  trained by the library's own kernels on seeded 1,000-DOF systems, its cutoff
+  chosen on held-back ones, against the plain pass: MIST total within 0.1%,
+  and the MST edge agreement reported.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from centre_amd import hip, screen
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-10
+
+
+def rel_err(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1.0)))
+
+
+def make_system(ctx, seed, D, F, nb):
+    nbd = nan = D // 3
+    ndh = D - nbd - nan
+    bat = hip.synth_bat(seed, nbd, nan, ndh, F)
+    d_bat = ctx.to_device(bat)
+    del bat
+    d_codes = ctx.alloc((D, F), np.uint8)
+    d_ext = ctx.alloc((D, 4), np.float64)
+    types = np.array([1] * nbd + [2] * nan + [3] * ndh, np.uint8)
+    col = 0
+    for t, n in ((1, nbd), (2, nan), (3, ndh)):
+        ctx.dev_discretize_hist(d_bat.ptr + 8 * col, D, n, F, t, [nb], nb, True,
+                                d_codes.ptr + col * F, d_ext.ptr + 32 * col)
+        col += n
+    ctx.sync()
+    d_bat.free()
+    return d_codes, d_ext, types, ctx.to_device(types)
+
+
+def code_rows(ctx, d_codes, rows, F):
+    """Rows of a device code matrix [D][F] (host copies of the few needed)."""
+    out = np.empty((len(rows), F), np.uint8)
+    for k, r in enumerate(rows):
+        ctx.check(ctx._L.centre_memcpy_d2h(ctx.h, out[k].ctypes.data, d_codes.ptr + int(r) * F, F))
+    return out
+
+
+def check_pair_sample(ctx, d_codes, types, ext, F, nb, S, i, j, s2, ests=(0,)):
+    rows = np.unique(np.concatenate([i, j]))
+    sub = code_rows(ctx, d_codes, rows, F)
+    loc = {int(r): k for k, r in enumerate(rows)}
+    pairs = np.array([[loc[int(a)], loc[int(b)]] for a, b in zip(i, j)], np.uint32)
+    w2 = O.pair_entropies(sub, types[rows], ext[rows], pairs, S, [nb], ests)
+    assert rel_err(s2, w2[:, :, 0, :]) < REL
+
+
+def test_batch_workload_64_uneven_systems():
+    nb, S, n_sys = 50, 10, 64
+    rng = np.random.default_rng(0xCE27E + 3)
+    dims = [(int(rng.integers(200, 2001)), int(rng.integers(20000, 200001))) for _ in range(n_sys)]
+    print(f"batch: {n_sys} systems, {sum(D * (D - 1) // 2 for D, _ in dims)} pairs", flush=True)
+    with hip.Context(0) as ctx:
+        systems, descs = [], []
+        for k, (D, F) in enumerate(dims):
+            d_codes, d_ext, types, d_types = make_system(ctx, 0xCE27E + 3000 + k, D, F, nb)
+            P = D * (D - 1) // 2
+            s = dict(D=D, F=F, P=P, codes=d_codes, ext=d_ext, types=types, d_types=d_types,
+                     s1=ctx.alloc((D, 1, S), np.float64), s2=ctx.alloc((P, 1, S), np.float64),
+                     mi=ctx.alloc((P, S), np.float64))
+            systems.append(s)
+            descs.append(hip.BatchSystem(d_codes.ptr, F, D, F, d_types.ptr, d_ext.ptr,
+                                         s["s1"].ptr, s["s2"].ptr, s["mi"].ptr))
+        print("batch: systems ready", flush=True)
+        sys_ms, tot, used = ctx.batch_pair_entropy(descs, nb, S, [0], order=0)
+        print(f"batch: {tot:.0f} ms, order {used}", flush=True)
+        assert np.all(sys_ms > 0) and np.all(sys_ms <= tot * 1.05 + 0.05)
+        for k, s in enumerate(systems):
+            D, F, P = s["D"], s["F"], s["P"]
+            ext = s["ext"].numpy()
+            s1, s2, mi = s["s1"].numpy(), s["s2"].numpy(), s["mi"].numpy()
+            assert np.all(np.isfinite(s2)) and np.all(np.isfinite(mi)), k
+            assert mi.min() > -1e-11, k
+            iu, ju = np.triu_indices(D, 1)
+            pick = np.linspace(0, P - 1, 6).astype(np.int64)
+            check_pair_sample(ctx, s["codes"], s["types"], ext, F, nb, S, iu[pick], ju[pick],
+                              s2[pick])
+            if k % 8 == 0:
+                codes = s["codes"].numpy()
+                w1 = O.dof_entropies(codes, s["types"], ext, S, [nb], (0,))
+                assert rel_err(s1, w1[:, :, 0, :]) < REL, k
+            for a in ("codes", "ext", "d_types", "s1", "s2", "mi"):
+                s[a].free()
+
+
+def _tile_pairs(tiles, D, R):
+    a, b = np.divmod(np.arange(R * R), R)
+    i = (tiles[:, 0:1] * R + a[None, :]).ravel()
+    j = (tiles[:, 1:2] * R + b[None, :]).ravel()
+    ok = (i < j) & (j < D)
+    return i[ok], j[ok]
+
+
+def _tri(i, j, D):
+    i = i.astype(np.int64)
+    return i * D - i * (i + 1) // 2 + (j.astype(np.int64) - i - 1)
+
+
+def _pass(ctx, d_codes, d_types, d_ext, D, F, nb, S, tiles=None):
+    P = D * (D - 1) // 2
+    d_s1 = ctx.alloc((D, 1, S), np.float64)
+    d_s2 = ctx.to_device(np.full((P, 1, S), np.nan))
+    d_mi = ctx.to_device(np.full((P, S), np.nan))
+    plan = hip.PairPlan(ctx, D, nb, tiles=tiles) if tiles is not None else hip.PairPlan(ctx, D, nb)
+    ctx.dev_pair_entropy(plan, d_codes.ptr, F, F, d_types.ptr, d_ext.ptr, S, [0], True, False,
+                         d_s1.ptr, d_s2.ptr, d_mi.ptr)
+    plan.close()
+    out = d_s1.numpy(), d_s2.numpy(), d_mi.numpy()
+    for a in (d_s1, d_s2, d_mi):
+        a.free()
+    return out
+
+
+def _mist(ctx, D, i, j, w, s1sum):
+    tree = ctx.mst(D, i.astype(np.uint32), j.astype(np.uint32), w)
+    return s1sum - float(w[tree].sum()), set(zip(i[tree].tolist(), j[tree].tolist()))
+
+
+def test_screen_workload_6000_dof():
+    nb, S, F, D = 50, 10, 100000, 6000
+    n_train, n_hold, n_sub, topk = 6, 2, 2000, 2
+    P = D * (D - 1) // 2
+    R = int(hip.lib().centre_pair_dofs_per_tile(nb))
+    with hip.Context(0) as ctx:
+        # training data: plain-path MI of seeded 1,000-DOF systems
+        feats, targets, held = [], [], []
+        for k in range(n_train + n_hold):
+            d_codes, d_ext, _, d_types = make_system(ctx, 0xCE27E + 100 + k, 1000, F, nb)
+            d_feat = screen.features(ctx, d_codes, F, d_types, d_ext, nb, n_sub)
+            s1, _, mi = _pass(ctx, d_codes, d_types, d_ext, 1000, F, nb, S)
+            if k < n_train:
+                feats.append(d_feat.numpy())
+                targets.append(screen.target(mi[:, S - 1]))
+            else:
+                held.append((d_feat.numpy(), mi[:, S - 1].copy(), s1[:, 0, S - 1].sum()))
+            for a in (d_codes, d_ext, d_types, d_feat):
+                a.free()
+        print("screen: training data ready", flush=True)
+        ftr, ttr = np.concatenate(feats), np.concatenate(targets)
+        net = screen.Screen(ctx, screen.init_params(ftr, 1))
+        d_f, d_t = ctx.to_device(ftr), ctx.to_device(ttr)
+        loss = net.train(d_f, d_t, len(ftr), epochs=4, batch=4096, lr=2e-3, seed=3)
+        assert loss[-1] < loss[0]
+        d_f.free()
+        d_t.free()
+        # cutoff: the largest whose MIST error on the held-back systems stays
+        # within 0.03% (margin for the 0.1% bar: training sums its gradients
+        # with FP32 atomics, so the network varies slightly from run to run)
+        iu1, ju1 = np.triu_indices(1000, 1)
+        chosen = 1e-3
+        for cut in (5e-3, 1e-2, 1.2e-2, 1.4e-2, 1.6e-2, 1.8e-2, 2e-2):
+            worst = 0.0
+            for fh, mih, s1h in held:
+                d_fh = ctx.to_device(fh)
+                d_ph = net.predict(d_fh, len(fh))
+                tl = ctx.screen_tiles(d_ph.ptr, 1000, nb, cut, topk)
+                d_fh.free()
+                d_ph.free()
+                i, j = _tile_pairs(tl, 1000, R)
+                full, _ = _mist(ctx, 1000, iu1, ju1, mih, s1h)
+                scr, _ = _mist(ctx, 1000, i, j, mih[_tri(i, j, 1000)], s1h)
+                worst = max(worst, abs(scr - full) / abs(full))
+            if worst <= 3e-4:
+                chosen = cut
+        print(f"screen: cutoff {chosen}", flush=True)
+        # the 6,000-DOF system: plain pass
+        d_codes, d_ext, types, d_types = make_system(ctx, 0xCE27E + 4, D, F, nb)
+        s1, s2, mi = _pass(ctx, d_codes, d_types, d_ext, D, F, nb, S)
+        print("screen: plain pass done", flush=True)
+        assert np.all(np.isfinite(s2)) and mi.min() > -1e-11
+        iu, ju = np.triu_indices(D, 1)
+        pick = np.linspace(0, P - 1, 40).astype(np.int64)
+        ext = d_ext.numpy()
+        check_pair_sample(ctx, d_codes, types, ext, F, nb, S, iu[pick], ju[pick], s2[pick])
+        w1 = O.dof_entropies(code_rows(ctx, d_codes, np.arange(0, D, 150), F), types[::150],
+                             ext[::150], S, [nb], (0,))
+        assert rel_err(s1[::150], w1[:, :, 0, :]) < REL
+        s1sum = s1[:, 0, S - 1].sum()
+        full, full_tree = _mist(ctx, D, iu, ju, mi[:, S - 1], s1sum)
+        del s2
+        # screened pass: features and predictions on the device, skipped
+        # tiles never launched
+        d_feat = screen.features(ctx, d_codes, F, d_types, d_ext, nb, n_sub)
+        d_pred = net.predict(d_feat, P)
+        tiles = ctx.screen_tiles(d_pred.ptr, D, nb, chosen, topk)
+        d_feat.free()
+        d_pred.free()
+        _, s2s, mis = _pass(ctx, d_codes, d_types, d_ext, D, F, nb, S, tiles=tiles)
+        i, j = _tile_pairs(tiles, D, R)
+        t = _tri(i, j, D)
+        inside = np.zeros(P, bool)
+        inside[t] = True
+        # computed pairs equal the plain pass (to the few ulp by which the
+        # order of the FP64 sums differs), skipped ones were never written
+        np.testing.assert_allclose(mis[t], mi[t], rtol=1e-12, atol=1e-14)
+        assert np.all(np.isnan(mis[~inside]))
+        scr, tree = _mist(ctx, D, i, j, mis[t, S - 1], s1sum)
+        err = abs(scr - full) / abs(full)
+        agree = len(tree & full_tree) / len(full_tree)
+        print(f"screen: cutoff {chosen}, pairs computed {len(t) / P:.3f}, "
+              f"MIST rel err {err:.2e}, MST edge agreement {agree:.3f}", flush=True)
+        assert err <= 1e-3
+        for a in (d_codes, d_ext, d_types):
+            a.free()

@@ -110,3 +110,35 @@ def test_refuses_non_netcdf(tmp_path):
     p.write_bytes(b"hello world, not netcdf")
     rc, _, err = run("--inspect", str(p))
     assert rc == 2 and "not a NetCDF file" in err
+
+
+@pytest.mark.parametrize("start,stride,nframe,F,want", [
+    (0, 1, 0, 1000, (0, 1, 1000)),          # defaults: every frame
+    (10, 3, 0, 1000, (10, 3, 1000)),
+    (1000, 2, 0, 1000, (0, 2, 1000)),       # start at the end reads from 0
+    (50, 5000, 0, 1000, (50, 1, 1000)),     # stride past the end reads every frame
+    (700, 2, 600, 1000, (0, 2, 600)),       # start past nframe
+    (0, 1, 5000, 1000, (0, 1, 1000)),       # nframe beyond the file
+])
+def test_frame_range_clamps(tmp_path, start, stride, nframe, F, want):
+    """The frame selection follows the reference's reader clamps
+    (Netcdf_TrjInt.cpp:120-121) and nframe default (Inputs.cpp:632-634)."""
+    cfg = tmp_path / "in.cfg"
+    cfg.write_text(f"[control]\n nsteps = 3\n[discretization]\n fname = a.nc\n"
+                   f"[histogram]\n referencenbins = 30\n nbins = 30\n"
+                   f"[entropy]\n startframe = {start}\n strideframe = {stride}\n"
+                   f" nframe = {nframe}\n")
+    rc, out, err = run("--print-config", str(cfg), str(F))
+    assert rc == 0, out + err
+    d = kv(out)
+    assert (int(d["startframe"]), int(d["strideframe"]), int(d["nframe"])) == want
+
+
+def test_zero_frame_stride_is_refused(tmp_path):
+    cfg = tmp_path / "in.cfg"
+    cfg.write_text("[control]\n nsteps = 3\n[discretization]\n fname = a.nc\n"
+                   "[histogram]\n referencenbins = 30\n nbins = 30\n"
+                   "[entropy]\n strideframe = 0\n")
+    rc, out, _ = run("--print-config", str(cfg), "1000")
+    assert rc == 1
+    assert "error entropy.strideframe must be at least 1" in out
