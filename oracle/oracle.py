@@ -19,17 +19,33 @@ ML, MM, CS, JS = 0, 1, 2, 3
 _lib = None
 
 
+# timing-only builds of the same source (CPU baseline of the benchmark)
+VARIANTS = {"O2": LIB_PATH,
+            "O3": os.path.join(HERE, "liboracle_O3.so"),
+            "Ofast": os.path.join(HERE, "liboracle_Ofast.so")}
+_variants = {}
+
+
 def build(force: bool = False) -> str:
-    """Compile the oracle shared library in place."""
+    """Compile the oracle shared libraries in place."""
     src = os.path.join(HERE, "centre_oracle.c")
-    if force or not os.path.exists(LIB_PATH) or \
-            os.path.getmtime(LIB_PATH) < os.path.getmtime(src):
-        subprocess.run(["make", "-s", "-C", HERE], check=True)
+    if force or any(not os.path.exists(p) or os.path.getmtime(p) < os.path.getmtime(src)
+                    for p in VARIANTS.values()):
+        subprocess.run(["make", "-s"] + (["-B"] if force else []) + ["-C", HERE], check=True)
     return LIB_PATH
 
 
-def lib():
+def lib(variant: str = "O2"):
+    """The oracle library; variant "O3" / "Ofast" selects a timing-only build."""
     global _lib
+    if variant != "O2":
+        if variant not in _variants:
+            if not os.path.exists(VARIANTS[variant]):
+                build()
+            L = C.CDLL(VARIANTS[variant])
+            _declare(L)
+            _variants[variant] = L
+        return _variants[variant]
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
@@ -221,8 +237,9 @@ def dof_entropies(codes, types, ext, nsteps, schemes=(30,), ests=(ML,),
 
 
 def pair_entropies(codes, types, ext, pairs, nsteps, schemes=(30,),
-                   ests=(ML,), jacobian=True, kde=False, nthreads=0):
-    """codes [D, nsch, F] -> S2 [npair, nest, nsch, nsteps]."""
+                   ests=(ML,), jacobian=True, kde=False, nthreads=0, variant="O2"):
+    """codes [D, nsch, F] -> S2 [npair, nest, nsch, nsteps].  variant picks
+    the build ("O3" / "Ofast" for timing only)."""
     _check_steps(nsteps)
     sch = _c(np.atleast_1d(schemes), np.uint8)
     codes = _c(codes, np.uint8)
@@ -233,7 +250,7 @@ def pair_entropies(codes, types, ext, pairs, nsteps, schemes=(30,),
     types = _c(types, np.uint8)
     ext = _c(ext, np.float64)
     out = np.zeros((len(pairs), len(ests), len(sch), nsteps))
-    r = lib().or_pair_entropies(_p(codes), D, F, _p(types), _p(ext),
+    r = lib(variant).or_pair_entropies(_p(codes), D, F, _p(types), _p(ext),
                                 _p(pairs), len(pairs), nsteps, _p(sch),
                                 len(sch), _p(ests), len(ests), int(jacobian),
                                 int(kde), _p(out), nthreads)
