@@ -1,0 +1,453 @@
+// Eight-wave pair kernel for full tiles (pair_tile8_kernel) and its batch form.
+//
+// What it replaces: binData2D + entropy2D (Estimators.cpp:130-205, 391-651)
+// for every pair of a tile, as pair_tiles.hip does (its header comment
+// describes the one-hot formulation and the step-end epilogue).
+#include "pair_common.h"
+
+namespace {
+
+// ---- eight-wave kernel for full tiles --------------------------------------------
+// Tiles with R DOFs on both sides (diagonal tiles included: both operands
+// are then the same rows, and only pairs i < j are kept) and 4-byte aligned
+// step starts, with 8 waves in a 2x4 layout: each wave owns 128 rows x 64
+// columns (4x2 MFMA blocks, 128 accumulator registers), so two waves share a
+// SIMD and one wave's fragment reads, staging and epilogue issue while the
+// other one's MFMAs run.  Only the first ceil(live slots/64) waves stage
+// codes (one slot per lane, lanes past the live slots repeat live ones of
+// their own wave); the others only multiply.
+__device__ __forceinline__ void epi_block8(const v16i& blk, int row0, int gcol, int lane,
+                                           const Smem& sm, int nb, int R, int nstr, int ntab) {
+    // the lane holds column gcol and rows row0 + 8*(r/4) + r%4, r = 0..15;
+    // with two waves per SIMD the accumulators are ordinary VGPRs and are
+    // read in place (the four-wave tile parks its AGPRs in LDS instead)
+    const int cd = sm.col2dof[gcol];
+    const int stripe = lane % nstr;
+    {   // nb >= 28: the lane's 16 rows touch at most two DOFs
+        const int dlo = sm.row2dof[row0];
+        const bool ok = (cd != 0xFF) & (dlo != 0xFF);
+        const int kk = (dlo + 1) * nb - row0;
+        double tot = 0.0, hi = 0.0;
+        int ntot = 0, nhi = 0;
+        // four rows at a time keeps few values live
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            int c[4];
+            double tv[4];
+            // counts 0 and 1 add nothing: only lanes with c > 1 read the
+            // table, which thins the bank traffic of the lookups
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                c[t] = ok ? blk[4 * q + t] : 0;
+                tv[t] = 0.0;
+                if (c[t] > 1) tv[t] = sm.ctab[min(c[t], ntab - 1)];
+            }
+            const int mx = max(max(c[0], c[1]), max(c[2], c[3]));
+            if (__builtin_amdgcn_ballot_w64(mx >= ntab) != 0) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    if (c[t] >= ntab)
+                        tv[t] = (double)c[t] * log_pos_lds((double)c[t], sm.logt, sm.invt, sm.invt + 256);
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const bool h = 8 * q + t >= kk;
+                const int nz = min(c[t], 1);
+                tot += tv[t];
+                hi += h ? tv[t] : 0.0;
+                ntot += nz;
+                nhi += h ? nz : 0;
+            }
+        }
+        if (!ok) return;
+        if (ntot > nhi) {
+            atomicAdd(&sm.slotT[(dlo * R + cd) * nstr + stripe], tot - hi);
+            atomicAdd(&sm.slotN[(dlo * R + cd) * nstr + stripe], ntot - nhi);
+        }
+        const int dhi = sm.row2dof[row0 + 27];
+        if (nhi && dhi != 0xFF && dhi != dlo) {
+            atomicAdd(&sm.slotT[(dhi * R + cd) * nstr + stripe], hi);
+            atomicAdd(&sm.slotN[(dhi * R + cd) * nstr + stripe], nhi);
+        }
+    }
+}
+
+// LDS of the eight-wave tile: kBuf8 one-hot operand buffers (chunk g is
+// multiplied from one, chunk g+1's fragments are read from the next while
+// that happens, chunk g+2 is staged into the third), the DOF maps and a
+// trash row, the per-pair slots, the log tables and as large a c log c table
+// as the rest leaves room for.
+constexpr int kBuf8 = 3;
+
+struct Lay8 {
+    size_t map, T, N, log, tab, total;
+    int nstripe, ntab;
+};
+
+__host__ __device__ inline Lay8 lay8(int R) {
+    Lay8 L;
+    size_t off = (size_t)kBuf8 * 2 * kOpBytes;
+    L.map = off;
+    off += 512 + kLdsRow;
+    off = (off + 15) & ~(size_t)15;
+    L.nstripe = 16;
+    L.T = off;
+    off += sizeof(double) * R * R * L.nstripe;
+    L.N = off;
+    off += sizeof(int) * R * R * L.nstripe;
+    off = (off + 15) & ~(size_t)15;
+    L.log = off;   // log(1 + j/256), 1/(1 + j/256), then 8 series coefficients
+    off += (2 * 256 + 8) * sizeof(double);
+    L.tab = off;
+    const size_t room = off < kLdsCap ? (kLdsCap - off) / sizeof(double) : 0;
+    L.ntab = (int)((room < (size_t)kTab ? room : (size_t)kTab) / 256 * 256);
+    off += sizeof(double) * L.ntab;
+    L.total = off;
+    return L;
+}
+
+template <bool kBatch>
+__device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) {
+    extern __shared__ __align__(16) uint8_t smem_raw[];
+    const PairKernelArgs& a = a_in;
+    // a batch entry takes codes, sizes and outputs from its system; the
+    // output pointers are read at each step end rather than kept live
+    const BatchSystem* const sdp = kBatch ? &a.systems[a.tile_sys[tix]] : nullptr;
+    const uint8_t* const codes = kBatch ? sdp->codes : a.codes;
+    const uint64_t cstride = kBatch ? sdp->stride : a.stride;
+    const uint64_t ss = kBatch ? sdp->ss : a.ss;
+    const uint64_t sstride = kBatch ? sdp->sstride : a.sstride;
+    const int R = a.R, nb = a.nb, S = a.nsteps;
+    const Lay8 lay = lay8(R);
+    const int nstr = lay.nstripe, ntab = lay.ntab;
+    Smem sm;
+    sm.oh = smem_raw;
+    sm.cl = nullptr;
+    sm.row2dof = smem_raw + lay.map;
+    sm.col2dof = smem_raw + lay.map + 256;
+    sm.slotT = reinterpret_cast<double*>(smem_raw + lay.T);
+    sm.slotN = reinterpret_cast<int*>(smem_raw + lay.N);
+    sm.logt = reinterpret_cast<double*>(smem_raw + lay.log);
+    sm.invt = sm.logt + 256;
+    sm.ctab = reinterpret_cast<double*>(smem_raw + lay.tab);
+    uint8_t* const trash = smem_raw + lay.map + 512;
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wr = wave >> 2, wc = wave & 3;
+    const int2 tile = a.tiles[tix];
+    const int r0 = tile.x * R, c0 = tile.y * R;
+    const bool diag = tile.x == tile.y;
+    const size_t boff = diag ? 0 : kOpBytes;   // column operand
+    const long long D = kBatch ? sdp->n_dof : a.n_dof;
+
+    for (int i = tid; i < kBuf8 * 2 * kOpBytes / 16; i += 512)
+        reinterpret_cast<uint4*>(sm.oh)[i] = make_uint4(0, 0, 0, 0);
+    if (tid < 256) {
+        const int d = tid / nb;
+        sm.row2dof[tid] = (uint8_t)(d < R ? d : 0xFF);
+        sm.col2dof[tid] = (uint8_t)(d < R ? d : 0xFF);
+        const double t = 1.0 + tid * (1.0 / 256.0);
+        sm.logt[tid] = log(t);
+        sm.invt[tid] = 1.0 / t;
+        if (tid < 8) {
+            const double cf[8] = {1.0, -0.5, 1.0 / 3.0, -0.25, 0.2, -1.0 / 6.0,
+                                  0.6931471805599453, 0.0};
+            sm.invt[256 + tid] = cf[tid];
+        }
+    }
+    for (int i = tid; i < R * R * nstr; i += 512) {
+        sm.slotT[i] = 0.0;
+        sm.slotN[i] = 0;
+    }
+    for (int c = tid; c < ntab; c += 512) sm.ctab[c] = c > 1 ? (double)c * log((double)c) : 0.0;
+    if (tid == 0) *reinterpret_cast<int*>(trash) = *reinterpret_cast<volatile int*>(a.flag);
+    __syncthreads();
+    // codes were range-checked by the marginal pass; with the flag set the
+    // results are discarded
+    if (*reinterpret_cast<const int*>(trash) != 0) return;
+
+    const int nch = (int)((ss + kChunk - 1) / kChunk);
+    const int G = S * nch;
+    const int nlive = (diag ? 1 : 2) * R * kGroups;
+    // this lane's slot: DOF ld, frames k4..k4+3 of each chunk.  The first
+    // waves stage one slot per lane (lanes past nlive repeat live slots of
+    // their own wave); spreading the slots over all eight waves instead
+    // (nlive/8 lanes each) ran 20% slower.
+    const bool stager = wave * 64 < nlive;   // wave-uniform
+    const int q = (wave * 64 + lane) % nlive;
+    const int ld = q / kGroups, k4 = (q - ld * kGroups) * 4;
+    const bool isA = ld < R;
+    const uint8_t* const src = codes + (uint64_t)(isA ? r0 + ld : c0 + (ld - R)) * cstride + k4;
+    const uint32_t dst = (uint32_t)((isA ? 0 : kOpBytes) + (isA ? ld : ld - R) * nb * kLdsRow + k4);
+    const uint32_t trash_off = (uint32_t)(lay.map + 512 + 16 + k4);
+    uint32_t last0 = 0u, last1 = 0u, last2 = 0u;
+    int ls = 0, lc = 0;   // next chunk to load
+    auto advance = [](int& s, int& c, int n) __attribute__((always_inline)) {
+        if (++c == n) { c = 0; ++s; }
+    };
+    auto nvalid_of = [&](int s, int c) __attribute__((always_inline)) -> int {
+        return s < S ? (int)min((uint64_t)kChunk, ss - (uint64_t)c * kChunk) : 0;
+    };
+    // this slot's code dword of the next chunk to load (past the last chunk
+    // a dword every row has, unused)
+    auto load1 = [&]() __attribute__((always_inline)) -> uint32_t {
+        const uint64_t f0 = (uint64_t)ls * sstride + (uint64_t)lc * kChunk;
+        const int nv = nvalid_of(ls, lc);
+        const uint32_t v = *reinterpret_cast<const uint32_t*>(k4 < nv ? src + f0 : codes);
+        advance(ls, lc, nch);
+        return v;
+    };
+    // chunk with nv valid frames (codes w) into buffer b: clear what this
+    // slot set when buffer b was last staged, three chunks ago, and set the
+    // new bytes; frames past nv go to the trash row.  Offsets stay integers
+    // so the stores compile to ds_write.
+    auto stage = [&](int b, int nv, uint32_t w) __attribute__((always_inline)) {
+        const uint32_t rowoff = (uint32_t)(b * 2 * kOpBytes) + dst;
+        const int nvk = nv - k4;
+        const uint32_t vmask = nvk >= 4 ? 0xFFFFFFFFu : (nvk <= 0 ? 0u : 0xFFFFFFFFu >> (8 * (4 - nvk)));
+        const uint32_t wv = w & vmask;
+        // masks rather than selects: a select of the three would be turned
+        // into a load through a selected address, which keeps them in memory
+        const uint32_t m0 = 0u - (uint32_t)(b == 0), m1 = 0u - (uint32_t)(b == 1),
+                       m2 = 0u - (uint32_t)(b == 2);
+        const uint32_t old = (last0 & m0) | (last1 & m1) | (last2 & m2);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) smem_raw[rowoff + ((old >> (8 * e)) & 0xFFu) * kLdsRow + e] = 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const uint32_t o = nvk > e ? rowoff : trash_off;
+            smem_raw[o + ((wv >> (8 * e)) & 0xFFu) * kLdsRow + e] = 1;
+        }
+        last0 = (last0 & ~m0) | (wv & m0);
+        last1 = (last1 & ~m1) | (wv & m1);
+        last2 = (last2 & ~m2) | (wv & m2);
+    };
+    v16i acc[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = v16i{};
+    // fragment i of the row operand / j of the column operand, 32 frames kk
+    const int lrow = (lane & 31) * kLdsRow + (lane >> 5) * 16;
+    // buffer offsets stay integers and every address is formed from the
+    // __shared__ array itself, so the reads compile to ds_read (a generic
+    // pointer carried through the loop compiles to flat loads)
+    auto rdA = [&](uint32_t bo, int kk, int i) __attribute__((always_inline)) -> v4i {
+        return *reinterpret_cast<const v4i*>(smem_raw + bo + (128 * wr + 32 * i) * kLdsRow + lrow +
+                                             kk * 32);
+    };
+    auto rdB = [&](uint32_t bo, int kk, int j) __attribute__((always_inline)) -> v4i {
+        return *reinterpret_cast<const v4i*>(smem_raw + bo + boff + (64 * wc + 32 * j) * kLdsRow +
+                                             lrow + kk * 32);
+    };
+    v4i fa0[4], fb0[2], fa1[4], fb1[2];
+    auto read_all = [&](uint32_t base) __attribute__((always_inline)) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) fa0[i] = rdA(base, 0, i);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) fb0[j] = rdB(base, 0, j);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) fa1[i] = rdA(base, 1, i);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) fb1[j] = rdB(base, 1, j);
+    };
+    // The MFMAs of 32 frames (kk); with kPref each fragment register is
+    // reloaded with chunk g+1's fragment (buffer nbase) as soon as its last
+    // MFMA of chunk g has issued, so the next chunk's LDS reads overlap this
+    // chunk's MFMAs and need no second register set.
+    auto mfma_kk = [&](auto pref, v4i (&af)[4], v4i (&bf)[2], uint32_t nbase,
+                       int kk) __attribute__((always_inline)) {
+        constexpr bool kPref = decltype(pref)::value;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            acc[i][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i], bf[0], acc[i][0], 0, 0, 0);
+            acc[i][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i], bf[1], acc[i][1], 0, 0, 0);
+            if constexpr (kPref) {
+                __builtin_amdgcn_sched_barrier(0);
+                af[i] = rdA(nbase, kk, i);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        if constexpr (kPref) {
+            bf[0] = rdB(nbase, kk, 0);
+            bf[1] = rdB(nbase, kk, 1);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    };
+    // Step-end epilogue, all waves: each block's c log c and nonzero bins
+    // into the per-pair LDS slots.
+    auto epilogue = [&]() __attribute__((always_inline)) {
+        __syncthreads();   // the pending finalize has read and cleared the slots
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                epi_block8(acc[i][j], 128 * wr + 32 * i + 4 * (lane >> 5),
+                           64 * wc + 32 * j + (lane & 31), lane, sm, nb, R, nstr, ntab);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+    };
+    // Slot sums of step fs -> S2 / MI, by wave 7 (never a stager) one
+    // iteration after the epilogue, so that it overlaps the next chunk; the
+    // iteration barrier orders it after the atomics, and the next epilogue's
+    // opening barrier after it.
+    auto finalize = [&](int fs) __attribute__((always_inline)) {
+        const double* const coef = sm.invt + 256;
+        const double n = (double)(fs + 1) * (double)ss;
+        const double logn = log_pos_lds(n, sm.logt, sm.invt, coef);
+        asm volatile("" ::: "memory");   // read the output pointers here
+        double* const s2 = kBatch ? sdp->s2 : a.s2;
+        double* const mi = kBatch ? sdp->mi : a.mi;
+        const double* const dof_w = kBatch ? sdp->dof_w : a.dof_w;
+        const double* const dof_L = kBatch ? sdp->dof_L : a.dof_L;
+        const double* const dof_S1 = kBatch ? sdp->dof_S1 : a.dof_S1;
+        for (int pq = lane; pq < R * R; pq += 64) {
+            const int rd = pq / R, cd = pq - rd * R;
+            const long long gi = r0 + rd, gj = c0 + cd;
+            const long long p = gi >= gj ? -1
+                              : (a.tilemap ? (long long)a.tilemap[(size_t)tix * R * R + pq] : tri_index(gi, gj, D));
+            double T = 0.0;
+            int nz = 0;
+            for (int k = 0; k < nstr; ++k) {
+                T += sm.slotT[pq * nstr + k];
+                nz += sm.slotN[pq * nstr + k];
+                sm.slotT[pq * nstr + k] = 0.0;
+                sm.slotN[pq * nstr + k] = 0;
+            }
+            if (p >= 0) {
+                const double L = dof_L[gi * S + fs] + dof_L[gj * S + fs];
+                const double area = dof_w[gi] * dof_w[gj];
+                const double ml = (logn + log_pos_lds(area, sm.logt, sm.invt, coef)) - T / n + L / n;
+                const double mm = ml + (double)(unsigned)(nz - 1) / (2.0 * n);
+                if (a.slot_ml >= 0) s2[((size_t)p * a.nest + a.slot_ml) * S + fs] = ml;
+                if (a.slot_mm >= 0) s2[((size_t)p * a.nest + a.slot_mm) * S + fs] = mm;
+                if (mi) {
+                    const double s1 = dof_S1[gi * S + fs] + dof_S1[gj * S + fs];
+                    mi[(size_t)p * S + fs] = s1 - (a.mi_from_mm ? mm : ml);
+                }
+            }
+        }
+    };
+
+    // One barrier per chunk.  Chunk g multiplies from buffer g % 3 while its
+    // fragment registers are reloaded with chunk g+1 (buffer (g+1) % 3,
+    // staged during chunk g-1) and the stager waves stage chunk g+2 into
+    // buffer (g+2) % 3, whose last reads (chunk g-1) all waves finished
+    // before the previous barrier.  At a step's last chunk the epilogue needs
+    // the registers, so that chunk reads nothing ahead and the next chunk's
+    // fragments are read after the barrier.
+    int ps = 0, pc = 0;   // chunk to stage next
+    int fin = -1;         // step whose finalize is pending
+    auto run = [&](auto stg) __attribute__((always_inline)) {
+        constexpr bool kStg = decltype(stg)::value;
+        // ring of code dwords: chunks g+2 and g+3 before chunk g's staging
+        uint32_t w0 = 0u, w1 = 0u;
+        if constexpr (kStg) {
+            w0 = load1();
+            w1 = load1();
+            stage(0, nvalid_of(ps, pc), w0);
+        }
+        advance(ps, pc, nch);
+        if constexpr (kStg) {
+            stage(1, nvalid_of(ps, pc), w1);
+            w0 = load1();
+            w1 = load1();
+        }
+        advance(ps, pc, nch);
+        __syncthreads();
+        read_all(0);
+        int b = 0;   // buffer of the chunk being multiplied
+        // one chunk: MFMAs (reading the next chunk's fragments ahead with
+        // kPref), staging of the chunk two ahead or the pending finalize
+        auto chunk = [&](auto pref) __attribute__((always_inline)) {
+            const int bn = b == kBuf8 - 1 ? 0 : b + 1;
+            const int bs = bn == kBuf8 - 1 ? 0 : bn + 1;
+            const uint32_t nbase = (uint32_t)(bn * 2 * kOpBytes);
+            uint32_t w2 = 0u;
+            if constexpr (kStg) w2 = load1();   // chunk g+4
+            mfma_kk(pref, fa0, fb0, nbase, 0);
+            mfma_kk(pref, fa1, fb1, nbase, 1);
+            if constexpr (kStg) {
+                // past the last chunk nvalid is 0: everything goes to the trash row
+                stage(bs, nvalid_of(ps, pc), w0);
+                w0 = w1;
+                w1 = w2;
+            } else {
+                if (fin >= 0 && wave == 7) finalize(fin);
+            }
+            fin = -1;
+            advance(ps, pc, nch);
+            b = bn;
+            return nbase;
+        };
+        for (int s = 0; s < S; ++s) {
+            for (int c = 0; c + 1 < nch; ++c) {
+                chunk(std::true_type{});
+                __syncthreads();
+            }
+            // the step's last chunk: the epilogue needs the registers, so the
+            // next chunk's fragments are read after the barrier
+            const uint32_t nbase = chunk(std::false_type{});
+            epilogue();
+            fin = s;
+            __syncthreads();
+            if (s + 1 < S) read_all(nbase);
+        }
+    };
+    if (stager) run(std::true_type{});
+    else run(std::false_type{});
+    if (wave == 7 && fin >= 0) finalize(fin);
+}
+
+__global__ __launch_bounds__(512) void pair_tile8_kernel(PairKernelArgs a) {
+    tile8_body<false>(a, blockIdx.x);
+}
+
+// Batch form of the eight-wave tile: one workgroup per entry of the ordered
+// batch tile list.  Workgroups are dispatched in list order as CUs free up,
+// which is the work queue of the persistent form; a loop around the tile
+// body would push it past the two-waves-per-SIMD register budget.  The
+// workgroup finishing a system's last tile stamps the system's completion.
+__global__ __launch_bounds__(512) void pair_batch8_kernel(PairKernelArgs a) {
+    const int t = blockIdx.x;
+    if (t == 0 && threadIdx.x == 0 && a.t0) *a.t0 = wall_clock64();
+    tile8_body<true>(a, t);
+    if (threadIdx.x == 0 && a.sys_count) {
+        const int sys = a.tile_sys[t];
+        __threadfence();
+        if (atomicAdd(&a.sys_count[sys], 1) == a.sys_tiles[sys] - 1)
+            a.sys_time[sys] = wall_clock64();
+    }
+}
+
+}  // namespace
+
+bool pair_tile8_supported(int R) {
+    // R <= 8 means nb >= 29: a lane's 16 rows of a block touch at most two DOFs.
+    const Lay8 L = lay8(R);
+    return R <= 8 && 2 * R * kGroups <= 512 && L.ntab >= 256 && L.total <= kLdsCap;
+}
+
+hipError_t launch_pair_tiles8(const PairKernelArgs& a, int n_tiles, hipStream_t st) {
+    if (n_tiles <= 0) return hipSuccess;
+    if (!pair_tile8_supported(a.R)) return hipErrorInvalidValue;
+    const size_t lds = lay8(a.R).total;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&pair_tile8_kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(pair_tile8_kernel, dim3(n_tiles), dim3(512), lds, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_pair_batch8(const PairKernelArgs& a, hipStream_t st) {
+    if (a.n_work <= 0) return hipSuccess;
+    if (!pair_tile8_supported(a.R)) return hipErrorInvalidValue;
+    const size_t lds = lay8(a.R).total;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&pair_batch8_kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(pair_batch8_kernel, dim3(a.n_work), dim3(512), lds, st, a);
+    return hipGetLastError();
+}
+

@@ -49,101 +49,9 @@
 // tile, each counting a K-th of every step into its own count dump
 // (pair_tile_kernel in dump mode with nsplit = K); split_entropy_kernel then
 // sums the K dumps and evaluates S2 and MI with the epilogue's formulas.
-#include "centre_internal.h"
-
-#include <type_traits>
-
-typedef int v4i __attribute__((ext_vector_type(4)));
-typedef int v16i __attribute__((ext_vector_type(16)));
+#include "pair_common.h"
 
 namespace {
-
-constexpr int kOpBytes = kTileRows * kLdsRow;   // one operand, one buffer
-constexpr int kGroups = kChunk / 4;             // dword groups per DOF row
-constexpr int kSuper = 4;                       // chunks per super-chunk
-
-struct Smem {
-    uint8_t* oh;        // [2 buf][2 operand][kTileRows * kLdsRow]
-    uint32_t* cl;       // [2 buf][2R DOFs][kGroups] codes staged per slot
-    uint8_t* row2dof;   // [256] local row DOF (0xFF = padding)
-    uint8_t* col2dof;   // [256]
-    double* slotT;      // [R*R][nstripe] sum c log c per local pair
-    int* slotN;         // [R*R][nstripe] nonzero bins per local pair
-    double* logt;       // [256] log(1 + j/256)
-    double* invt;       // [256] 1 / (1 + j/256)
-    double* ctab;       // [ntab] c log c for small counts
-};
-
-constexpr int kTab = 4096;                 // c log c table entries when LDS allows
-constexpr size_t kLdsCap = 160 * 1024;
-
-struct LdsLayout {
-    size_t cl, map, T, N, log, park, tab, total;
-    int nstripe;   // copies of each pair slot, picked by lane % nstripe, so
-                   // that the lanes of one wave rarely add to the same word
-    int ntab;      // 0 (no table) or kTab
-};
-
-__host__ __device__ inline LdsLayout lds_layout(int R, int nwave = 4) {
-    LdsLayout L;
-    size_t off = (size_t)2 * 2 * kOpBytes;
-    L.cl = off;
-    off += (size_t)2 * 2 * R * kGroups * 4;
-    L.map = off;
-    off += 512 + kLdsRow;   // row/col DOF maps, then a trash row
-    off = (off + 15) & ~(size_t)15;
-    L.nstripe = R <= 8 ? 16 : (R <= 16 ? 4 : 1);
-    L.T = off;
-    off += sizeof(double) * R * R * L.nstripe;
-    L.N = off;
-    off += sizeof(int) * R * R * L.nstripe;
-    off = (off + 15) & ~(size_t)15;
-    L.log = off;   // log(1 + j/256), 1/(1 + j/256), then 8 series coefficients
-    off += (2 * 256 + 8) * sizeof(double);
-    // epilogue parking area of the four-wave tile's fast path (R <= 32):
-    // 4 KB per wave; nwave = 0 for the eight-wave tile, which reads its
-    // accumulators in place
-    L.park = off;
-    if (R <= 32) off += (size_t)nwave * 64 * 16 * sizeof(int);
-    L.tab = off;
-    L.ntab = off + sizeof(double) * kTab <= kLdsCap ? kTab : 0;
-    off += sizeof(double) * L.ntab;
-    L.total = off;
-    return L;
-}
-
-// log(c) for an integer count c >= 1: c = 2^(e-1) * m, m in [1, 2),
-// m = t_j (1 + r) with t_j = 1 + j/256 from a table, log(1 + r) by its Taylor
-// series (|r| < 2^-8, truncation < 2e-18).  About 15 instructions instead of
-// the ~100 of the general FP64 log; accurate to a few ulp.
-__device__ inline double log_count(int c, const double* logt, const double* invt) {
-    int e;
-    const double m2 = 2.0 * frexp((double)c, &e);          // [1, 2), exact
-    const double f = m2 - 1.0;                              // exact
-    const int j = (int)(f * 256.0);
-    const double r = (f - j * (1.0 / 256.0)) * invt[j];
-    const double p = r * (1.0 + r * (-0.5 + r * (1.0 / 3.0 + r * (-0.25 + r * (0.2 + r * (-1.0 / 6.0))))));
-    return (double)(e - 1) * 0.6931471805599453 + logt[j] + p;
-}
-
-// The same for any positive finite x, with the series coefficients and log 2
-// read from LDS (coef = {1, -1/2, 1/3, -1/4, 1/5, -1/6, log 2}): as
-// literals they would be materialised in registers for the whole kernel.
-__device__ inline double log_pos_lds(double x, const double* logt, const double* invt,
-                                     const double* coef) {
-    int e;
-    const double m2 = 2.0 * frexp(x, &e);
-    const double f = m2 - 1.0;
-    const int j = (int)(f * 256.0);
-    const double r = (f - j * (1.0 / 256.0)) * invt[j];
-    const double p =
-        r * (coef[0] + r * (coef[1] + r * (coef[2] + r * (coef[3] + r * (coef[4] + r * coef[5])))));
-    return (double)(e - 1) * coef[6] + logt[j] + p;
-}
-
-__device__ inline long long tri_index(long long i, long long j, long long D) {
-    return i * D - i * (i + 1) / 2 + (j - i - 1);
-}
 
 // One tile: all DOF pairs of row block tile.x x column block tile.y of
 // tile list entry tix, over every frame and step.  With a batch (a_in.systems)
@@ -874,351 +782,6 @@ __global__ __launch_bounds__(256) void pair_persist_kernel(PairKernelArgs a) {
 }
 
 
-// ---- eight-wave kernel for full tiles --------------------------------------------
-// Tiles with R DOFs on both sides (diagonal tiles included: both operands
-// are then the same rows, and only pairs i < j are kept) and 4-byte aligned
-// step starts, with 8 waves in a 2x4 layout: each wave owns 128 rows x 64
-// columns (4x2 MFMA blocks, 128 accumulator registers), so two waves share a
-// SIMD and one wave's fragment reads, staging and epilogue issue while the
-// other one's MFMAs run.  Only the first ceil(live slots/64) waves stage
-// codes (one slot per lane, lanes past the live slots repeat live ones of
-// their own wave); the others only multiply.
-__device__ __forceinline__ void epi_block8(const v16i& blk, int row0, int gcol, int lane,
-                                           const Smem& sm, int nb, int R, int nstr, int ntab) {
-    // the lane holds column gcol and rows row0 + 8*(r/4) + r%4, r = 0..15;
-    // with two waves per SIMD the accumulators are ordinary VGPRs and are
-    // read in place (the four-wave tile parks its AGPRs in LDS instead)
-    const int cd = sm.col2dof[gcol];
-    const int stripe = lane % nstr;
-    {   // nb >= 28: the lane's 16 rows touch at most two DOFs
-        const int dlo = sm.row2dof[row0];
-        const bool ok = (cd != 0xFF) & (dlo != 0xFF);
-        const int kk = (dlo + 1) * nb - row0;
-        double tot = 0.0, hi = 0.0;
-        int ntot = 0, nhi = 0;
-        // four rows at a time keeps few values live
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            int c[4];
-            double tv[4];
-            // counts 0 and 1 add nothing: only lanes with c > 1 read the
-            // table, which thins the bank traffic of the lookups
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                c[t] = ok ? blk[4 * q + t] : 0;
-                tv[t] = 0.0;
-                if (c[t] > 1) tv[t] = sm.ctab[min(c[t], ntab - 1)];
-            }
-            const int mx = max(max(c[0], c[1]), max(c[2], c[3]));
-            if (__builtin_amdgcn_ballot_w64(mx >= ntab) != 0) {
-#pragma unroll
-                for (int t = 0; t < 4; ++t)
-                    if (c[t] >= ntab)
-                        tv[t] = (double)c[t] * log_pos_lds((double)c[t], sm.logt, sm.invt, sm.invt + 256);
-            }
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const bool h = 8 * q + t >= kk;
-                const int nz = min(c[t], 1);
-                tot += tv[t];
-                hi += h ? tv[t] : 0.0;
-                ntot += nz;
-                nhi += h ? nz : 0;
-            }
-        }
-        if (!ok) return;
-        if (ntot > nhi) {
-            atomicAdd(&sm.slotT[(dlo * R + cd) * nstr + stripe], tot - hi);
-            atomicAdd(&sm.slotN[(dlo * R + cd) * nstr + stripe], ntot - nhi);
-        }
-        const int dhi = sm.row2dof[row0 + 27];
-        if (nhi && dhi != 0xFF && dhi != dlo) {
-            atomicAdd(&sm.slotT[(dhi * R + cd) * nstr + stripe], hi);
-            atomicAdd(&sm.slotN[(dhi * R + cd) * nstr + stripe], nhi);
-        }
-    }
-}
-
-template <bool kBatch>
-__device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) {
-    extern __shared__ __align__(16) uint8_t smem_raw[];
-    const PairKernelArgs& a = a_in;
-    // a batch entry takes codes, sizes and outputs from its system; the
-    // output pointers are read at each step end rather than kept live
-    const BatchSystem* const sdp = kBatch ? &a.systems[a.tile_sys[tix]] : nullptr;
-    const uint8_t* const codes = kBatch ? sdp->codes : a.codes;
-    const uint64_t cstride = kBatch ? sdp->stride : a.stride;
-    const uint64_t ss = kBatch ? sdp->ss : a.ss;
-    const uint64_t sstride = kBatch ? sdp->sstride : a.sstride;
-    const int R = a.R, nb = a.nb, S = a.nsteps;
-    const LdsLayout lay = lds_layout(R, 0);
-    const int nstr = lay.nstripe, ntab = lay.ntab;
-    Smem sm;
-    sm.oh = smem_raw;
-    sm.cl = reinterpret_cast<uint32_t*>(smem_raw + lay.cl);
-    sm.row2dof = smem_raw + lay.map;
-    sm.col2dof = smem_raw + lay.map + 256;
-    sm.slotT = reinterpret_cast<double*>(smem_raw + lay.T);
-    sm.slotN = reinterpret_cast<int*>(smem_raw + lay.N);
-    sm.logt = reinterpret_cast<double*>(smem_raw + lay.log);
-    sm.invt = sm.logt + 256;
-    sm.ctab = reinterpret_cast<double*>(smem_raw + lay.tab);
-    uint8_t* const trash = smem_raw + lay.map + 512;
-
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int wr = wave >> 2, wc = wave & 3;
-    const int2 tile = a.tiles[tix];
-    const int r0 = tile.x * R, c0 = tile.y * R;
-    const bool diag = tile.x == tile.y;
-    const size_t boff = diag ? 0 : kOpBytes;   // column operand
-    const long long D = kBatch ? sdp->n_dof : a.n_dof;
-
-    for (int i = tid; i < 2 * 2 * kOpBytes / 16; i += 512)
-        reinterpret_cast<uint4*>(sm.oh)[i] = make_uint4(0, 0, 0, 0);
-    if (tid < 256) {
-        const int d = tid / nb;
-        sm.row2dof[tid] = (uint8_t)(d < R ? d : 0xFF);
-        sm.col2dof[tid] = (uint8_t)(d < R ? d : 0xFF);
-        const double t = 1.0 + tid * (1.0 / 256.0);
-        sm.logt[tid] = log(t);
-        sm.invt[tid] = 1.0 / t;
-        if (tid < 8) {
-            const double cf[8] = {1.0, -0.5, 1.0 / 3.0, -0.25, 0.2, -1.0 / 6.0,
-                                  0.6931471805599453, 0.0};
-            sm.invt[256 + tid] = cf[tid];
-        }
-    }
-    for (int i = tid; i < R * R * nstr; i += 512) {
-        sm.slotT[i] = 0.0;
-        sm.slotN[i] = 0;
-    }
-    for (int c = tid; c < ntab; c += 512) sm.ctab[c] = c > 1 ? (double)c * log((double)c) : 0.0;
-    if (tid == 0) *reinterpret_cast<int*>(trash) = *reinterpret_cast<volatile int*>(a.flag);
-    __syncthreads();
-    // codes were range-checked by the marginal pass; with the flag set the
-    // results are discarded
-    if (*reinterpret_cast<const int*>(trash) != 0) return;
-
-    const int nch = (int)((ss + kChunk - 1) / kChunk);
-    const int G = S * nch;
-    const int nlive = (diag ? 1 : 2) * R * kGroups;
-    // this lane's slot: DOF ld, frames k4..k4+3 of each chunk.  The first
-    // waves stage one slot per lane (lanes past nlive repeat live slots of
-    // their own wave); spreading the slots over all eight waves instead
-    // (nlive/8 lanes each) ran 20% slower.
-    const bool stager = wave * 64 < nlive;   // wave-uniform
-    const int q = (wave * 64 + lane) % nlive;
-    const int ld = q / kGroups, k4 = (q - ld * kGroups) * 4;
-    const bool isA = ld < R;
-    const uint8_t* const src = codes + (uint64_t)(isA ? r0 + ld : c0 + (ld - R)) * cstride + k4;
-    const uint32_t dst = (uint32_t)((isA ? 0 : kOpBytes) + (isA ? ld : ld - R) * nb * kLdsRow + k4);
-    uint32_t cur[kSuper], nxt[kSuper], last0 = 0u, last1 = 0u;
-    int ls = 0, lc = 0;
-    auto advance = [](int& s, int& c, int n) __attribute__((always_inline)) {
-        if (++c == n) { c = 0; ++s; }
-    };
-    auto nvalid_of = [&](int s, int c) __attribute__((always_inline)) -> int {
-        return s < S ? (int)min((uint64_t)kChunk, ss - (uint64_t)c * kChunk) : 0;
-    };
-    auto load4 = [&](uint32_t (&d)[kSuper]) __attribute__((always_inline)) {
-#pragma unroll
-        for (int c = 0; c < kSuper; ++c) {
-            const uint64_t f0 = (uint64_t)ls * sstride + (uint64_t)lc * kChunk;
-            const int nv = nvalid_of(ls, lc);
-            d[c] = *reinterpret_cast<const uint32_t*>(k4 < nv ? src + f0 : codes);
-            advance(ls, lc, nch);
-        }
-    };
-    // chunk with nv valid frames from dword cis of cw into buffer b (clear
-    // what this slot set two chunks ago, set the new bytes; frames past nv go
-    // to the trash row)
-    auto stage = [&](int b, int cis, int nv, const uint32_t (&cw)[kSuper]) __attribute__((always_inline)) {
-        uint8_t* const row = sm.oh + (size_t)b * 2 * kOpBytes + dst;
-        const int nvk = nv - k4;
-        const uint32_t vmask = nvk >= 4 ? 0xFFFFFFFFu : (nvk <= 0 ? 0u : 0xFFFFFFFFu >> (8 * (4 - nvk)));
-        uint32_t wv = 0;
-#pragma unroll
-        for (int c = 0; c < kSuper; ++c) wv |= cw[c] & (0u - (uint32_t)(cis == c));
-        wv &= vmask;
-        const uint32_t old = b ? last1 : last0;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) row[((old >> (8 * e)) & 0xFFu) * kLdsRow + e] = 0;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            uint8_t* const srow = nvk > e ? row : trash + 16 + k4;
-            srow[((wv >> (8 * e)) & 0xFFu) * kLdsRow + e] = 1;
-        }
-        last1 = b ? wv : last1;
-        last0 = b ? last0 : wv;
-    };
-    v16i acc[4][2];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = v16i{};
-    auto frags = [&](const uint8_t* base, int kk, v4i (&af)[4], v4i (&bf)[2]) __attribute__((always_inline)) {
-        const int koff = kk * 32 + (lane >> 5) * 16;
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-            af[i] = *reinterpret_cast<const v4i*>(base + (128 * wr + 32 * i + (lane & 31)) * kLdsRow + koff);
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-            bf[j] = *reinterpret_cast<const v4i*>(base + boff + (64 * wc + 32 * j + (lane & 31)) * kLdsRow + koff);
-    };
-    auto mfma8 = [&](const v4i (&af)[4], const v4i (&bf)[2]) __attribute__((always_inline)) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
-    };
-    // Step-end epilogue, all waves: each block's c log c and nonzero bins
-    // into the per-pair LDS slots.
-    auto epilogue = [&]() __attribute__((always_inline)) {
-        __syncthreads();   // the pending finalize has read and cleared the slots
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                epi_block8(acc[i][j], 128 * wr + 32 * i + 4 * (lane >> 5),
-                           64 * wc + 32 * j + (lane & 31), lane, sm, nb, R, nstr, ntab);
-                __builtin_amdgcn_sched_barrier(0);
-            }
-    };
-    // Slot sums of step fs -> S2 / MI, by wave 7 (never a stager) one
-    // iteration after the epilogue, so that it overlaps the next chunk; the
-    // iteration barrier orders it after the atomics, and the next epilogue's
-    // opening barrier after it.
-    auto finalize = [&](int fs) __attribute__((always_inline)) {
-        const double* const coef = sm.invt + 256;
-        const double n = (double)(fs + 1) * (double)ss;
-        const double logn = log_pos_lds(n, sm.logt, sm.invt, coef);
-        asm volatile("" ::: "memory");   // read the output pointers here
-        double* const s2 = kBatch ? sdp->s2 : a.s2;
-        double* const mi = kBatch ? sdp->mi : a.mi;
-        const double* const dof_w = kBatch ? sdp->dof_w : a.dof_w;
-        const double* const dof_L = kBatch ? sdp->dof_L : a.dof_L;
-        const double* const dof_S1 = kBatch ? sdp->dof_S1 : a.dof_S1;
-        for (int pq = lane; pq < R * R; pq += 64) {
-            const int rd = pq / R, cd = pq - rd * R;
-            const long long gi = r0 + rd, gj = c0 + cd;
-            const long long p = gi >= gj ? -1
-                              : (a.tilemap ? (long long)a.tilemap[(size_t)tix * R * R + pq] : tri_index(gi, gj, D));
-            double T = 0.0;
-            int nz = 0;
-            for (int k = 0; k < nstr; ++k) {
-                T += sm.slotT[pq * nstr + k];
-                nz += sm.slotN[pq * nstr + k];
-                sm.slotT[pq * nstr + k] = 0.0;
-                sm.slotN[pq * nstr + k] = 0;
-            }
-            if (p >= 0) {
-                const double L = dof_L[gi * S + fs] + dof_L[gj * S + fs];
-                const double area = dof_w[gi] * dof_w[gj];
-                const double ml = (logn + log_pos_lds(area, sm.logt, sm.invt, coef)) - T / n + L / n;
-                const double mm = ml + (double)(unsigned)(nz - 1) / (2.0 * n);
-                if (a.slot_ml >= 0) s2[((size_t)p * a.nest + a.slot_ml) * S + fs] = ml;
-                if (a.slot_mm >= 0) s2[((size_t)p * a.nest + a.slot_mm) * S + fs] = mm;
-                if (mi) {
-                    const double s1 = dof_S1[gi * S + fs] + dof_S1[gj * S + fs];
-                    mi[(size_t)p * S + fs] = s1 - (a.mi_from_mm ? mm : ml);
-                }
-            }
-        }
-    };
-
-    // One barrier per chunk: chunk g multiplies from buffer g&1 while the
-    // stager waves write chunk g+1 into the other buffer, their staging
-    // instructions spread between their MFMAs (stager waves and the others
-    // run separate copies of the loop; both pass the same barriers).
-    // Reading chunk g+1's fragments during chunk g's MFMAs (half a chunk
-    // ahead) needs 24 more registers than the two-waves-per-SIMD budget
-    // leaves next to the 128 accumulators: that variant spills and ran 2.5x
-    // slower.
-    int ps = 0, pc = 0;   // chunk to stage next
-    int ms = 0, mc = 0;   // chunk being multiplied
-    int fin = -1;         // step whose finalize is pending
-    auto run = [&](auto stg) __attribute__((always_inline)) {
-        constexpr bool kStg = decltype(stg)::value;
-        if constexpr (kStg) {
-            load4(cur);
-            load4(nxt);
-            stage(0, 0, nvalid_of(0, 0), cur);
-        }
-        advance(ps, pc, nch);
-        __syncthreads();
-        for (int g = 0; g < G; ++g) {
-            const int b = g & 1;
-            const uint8_t* base = sm.oh + (size_t)b * 2 * kOpBytes;
-            const int cis = (ps * nch + pc) & (kSuper - 1);   // chunk g+1 in its super-chunk
-            if constexpr (kStg) {
-                if (cis == 0) {
-#pragma unroll
-                    for (int c = 0; c < kSuper; ++c) cur[c] = nxt[c];
-                    load4(nxt);
-                }
-            }
-            v4i fa0[4], fb0[2], fa1[4], fb1[2];
-            frags(base, 0, fa0, fb0);
-            frags(base, 1, fa1, fb1);
-            mfma8(fa0, fb0);
-            mfma8(fa1, fb1);
-            if constexpr (kStg) {
-                // past the last chunk nvalid is 0: everything goes to the trash row
-                stage(b ^ 1, cis, nvalid_of(ps, pc), cur);
-                __builtin_amdgcn_sched_group_barrier(0x100, 6, 0);
-#pragma unroll
-                for (int i = 0; i < 6; ++i) {
-                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-                    __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-                }
-#pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-                    __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);
-                    __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
-                }
-                __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
-            } else {
-                if (fin >= 0 && wave == 7) finalize(fin);
-            }
-            advance(ps, pc, nch);
-            fin = -1;
-            if (mc == nch - 1) {
-                epilogue();
-                fin = ms;
-            }
-            advance(ms, mc, nch);
-            __syncthreads();
-        }
-    };
-    if (stager) run(std::true_type{});
-    else run(std::false_type{});
-    if (wave == 7 && fin >= 0) finalize(fin);
-}
-
-__global__ __launch_bounds__(512) void pair_tile8_kernel(PairKernelArgs a) {
-    tile8_body<false>(a, blockIdx.x);
-}
-
-// Batch form of the eight-wave tile: one workgroup per entry of the ordered
-// batch tile list.  Workgroups are dispatched in list order as CUs free up,
-// which is the work queue of the persistent form; a loop around the tile
-// body would push it past the two-waves-per-SIMD register budget.  The
-// workgroup finishing a system's last tile stamps the system's completion.
-__global__ __launch_bounds__(512) void pair_batch8_kernel(PairKernelArgs a) {
-    const int t = blockIdx.x;
-    if (t == 0 && threadIdx.x == 0 && a.t0) *a.t0 = wall_clock64();
-    tile8_body<true>(a, t);
-    if (threadIdx.x == 0 && a.sys_count) {
-        const int sys = a.tile_sys[t];
-        __threadfence();
-        if (atomicAdd(&a.sys_count[sys], 1) == a.sys_tiles[sys] - 1)
-            a.sys_time[sys] = wall_clock64();
-    }
-}
-
 // Copies the ss frames of every step of every DOF to rows where each step
 // starts on a 4-byte boundary: out[d][s * ss4 + f] = codes[d][s * ss + f]
 // (padding bytes 0, never counted).
@@ -1290,35 +853,6 @@ hipError_t launch_pair_persistent(const PairKernelArgs& a, int grid, hipStream_t
     if (need <= 2) return launch_p<2>(a, grid, lds, st);
     if (need <= 4) return launch_p<4>(a, grid, lds, st);
     return launch_p<8>(a, grid, lds, st);
-}
-
-bool pair_tile8_supported(int R) {
-    // R <= 8 means nb >= 29: a lane's 16 rows of a block touch at most two DOFs.
-    // The eight-wave tile needs no park area (its accumulators are VGPRs).
-    const LdsLayout L = lds_layout(R, 0);
-    return R <= 8 && 2 * R * kGroups <= 512 && L.ntab > 0 && L.total <= kLdsCap;
-}
-
-hipError_t launch_pair_tiles8(const PairKernelArgs& a, int n_tiles, hipStream_t st) {
-    if (n_tiles <= 0) return hipSuccess;
-    if (!pair_tile8_supported(a.R)) return hipErrorInvalidValue;
-    const size_t lds = lds_layout(a.R, 0).total;
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&pair_tile8_kernel),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(pair_tile8_kernel, dim3(n_tiles), dim3(512), lds, st, a);
-    return hipGetLastError();
-}
-
-hipError_t launch_pair_batch8(const PairKernelArgs& a, hipStream_t st) {
-    if (a.n_work <= 0) return hipSuccess;
-    if (!pair_tile8_supported(a.R)) return hipErrorInvalidValue;
-    const size_t lds = lds_layout(a.R, 0).total;
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&pair_batch8_kernel),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(pair_batch8_kernel, dim3(a.n_work), dim3(512), lds, st, a);
-    return hipGetLastError();
 }
 
 hipError_t launch_pair_tiles(const PairKernelArgs& a, int n_tiles, bool dump,
