@@ -16,59 +16,68 @@ namespace {
 // other one's MFMAs run.  Only the first ceil(live slots/64) waves stage
 // codes (one slot per lane, lanes past the live slots repeat live ones of
 // their own wave); the others only multiply.
-__device__ __forceinline__ void epi_block8(const v16i& blk, int row0, int gcol, int lane,
+__device__ __forceinline__ void epi_block8(const v16i& blk, int brow, int gcol, int lane,
                                            const Smem& sm, int nb, int R, int nstr, int ntab) {
-    // the lane holds column gcol and rows row0 + 8*(r/4) + r%4, r = 0..15;
-    // with two waves per SIMD the accumulators are ordinary VGPRs and are
-    // read in place (the four-wave tile parks its AGPRs in LDS instead)
+    // the lane holds column gcol and rows row0 + 8*(r/4) + r%4, r = 0..15,
+    // row0 = brow + 4*(lane/32) in the 32-row block starting at brow; with
+    // two waves per SIMD the accumulators are ordinary VGPRs and are read in
+    // place (the four-wave tile parks its AGPRs in LDS instead).  The table
+    // holds 0 for counts 0 and 1, so every count is one lookup; counts past
+    // the table are corrected afterwards (one ballot per block).
     const int cd = sm.col2dof[gcol];
     const int stripe = lane % nstr;
-    {   // nb >= 28: the lane's 16 rows touch at most two DOFs
-        const int dlo = sm.row2dof[row0];
-        const bool ok = (cd != 0xFF) & (dlo != 0xFF);
-        const int kk = (dlo + 1) * nb - row0;
-        double tot = 0.0, hi = 0.0;
-        int ntot = 0, nhi = 0;
-        // four rows at a time keeps few values live
+    const int row0 = brow + 4 * (lane >> 5);
+    const int dlo = sm.row2dof[row0];
+    const bool ok = (cd != 0xFF) & (dlo != 0xFF);
+    const int kk = (dlo + 1) * nb - row0;   // first local row of the next DOF
+    // nb >= 28: the lane's 16 rows touch at most two DOFs, and a block whose
+    // first and last rows hold one DOF (wave-uniform) needs no split
+    const bool one = sm.row2dof[brow] == sm.row2dof[brow + 31];
+    double tot = 0.0, hi = 0.0;
+    int ntot = 0, nhi = 0, mx = 0;
+    if (one) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            int c[4];
-            double tv[4];
-            // counts 0 and 1 add nothing: only lanes with c > 1 read the
-            // table, which thins the bank traffic of the lookups
+        for (int r = 0; r < 16; ++r) {
+            const int c = ok ? blk[r] : 0;
+            mx = max(mx, c);
+            tot += sm.ctab[min(c, ntab - 1)];
+            ntot += min(c, 1);
+        }
+    } else {
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                c[t] = ok ? blk[4 * q + t] : 0;
-                tv[t] = 0.0;
-                if (c[t] > 1) tv[t] = sm.ctab[min(c[t], ntab - 1)];
-            }
-            const int mx = max(max(c[0], c[1]), max(c[2], c[3]));
-            if (__builtin_amdgcn_ballot_w64(mx >= ntab) != 0) {
-#pragma unroll
-                for (int t = 0; t < 4; ++t)
-                    if (c[t] >= ntab)
-                        tv[t] = (double)c[t] * log_pos_lds((double)c[t], sm.logt, sm.invt, sm.invt + 256);
-            }
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const bool h = 8 * q + t >= kk;
-                const int nz = min(c[t], 1);
-                tot += tv[t];
-                hi += h ? tv[t] : 0.0;
-                ntot += nz;
-                nhi += h ? nz : 0;
+        for (int r = 0; r < 16; ++r) {
+            const int c = ok ? blk[r] : 0;
+            const bool h = 8 * (r >> 2) + (r & 3) >= kk;
+            mx = max(mx, c);
+            const double v = sm.ctab[min(c, ntab - 1)];
+            const int nz = min(c, 1);
+            tot += v;
+            hi += h ? v : 0.0;
+            ntot += nz;
+            nhi += h ? nz : 0;
+        }
+    }
+    if (__builtin_amdgcn_ballot_w64(mx >= ntab) != 0) {
+        const double clamp = sm.ctab[ntab - 1];
+#pragma unroll 1
+        for (int r = 0; r < 16; ++r) {
+            const int c = ok ? blk[r] : 0;
+            if (c >= ntab) {
+                const double d = (double)c * log_pos_lds((double)c, sm.logt, sm.invt, sm.invt + 256) - clamp;
+                tot += d;
+                if (!one && 8 * (r >> 2) + (r & 3) >= kk) hi += d;
             }
         }
-        if (!ok) return;
-        if (ntot > nhi) {
-            atomicAdd(&sm.slotT[(dlo * R + cd) * nstr + stripe], tot - hi);
-            atomicAdd(&sm.slotN[(dlo * R + cd) * nstr + stripe], ntot - nhi);
-        }
-        const int dhi = sm.row2dof[row0 + 27];
-        if (nhi && dhi != 0xFF && dhi != dlo) {
-            atomicAdd(&sm.slotT[(dhi * R + cd) * nstr + stripe], hi);
-            atomicAdd(&sm.slotN[(dhi * R + cd) * nstr + stripe], nhi);
-        }
+    }
+    if (!ok) return;
+    if (ntot > nhi) {
+        atomicAdd(&sm.slotT[(dlo * R + cd) * nstr + stripe], tot - hi);
+        atomicAdd(&sm.slotN[(dlo * R + cd) * nstr + stripe], ntot - nhi);
+    }
+    const int dhi = sm.row2dof[row0 + 27];
+    if (nhi && dhi != 0xFF && dhi != dlo) {
+        atomicAdd(&sm.slotT[(dhi * R + cd) * nstr + stripe], hi);
+        atomicAdd(&sm.slotN[(dhi * R + cd) * nstr + stripe], nhi);
     }
 }
 
@@ -283,8 +292,8 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
-                epi_block8(acc[i][j], 128 * wr + 32 * i + 4 * (lane >> 5),
-                           64 * wc + 32 * j + (lane & 31), lane, sm, nb, R, nstr, ntab);
+                epi_block8(acc[i][j], 128 * wr + 32 * i, 64 * wc + 32 * j + (lane & 31), lane, sm,
+                           nb, R, nstr, ntab);
                 __builtin_amdgcn_sched_barrier(0);
             }
     };
