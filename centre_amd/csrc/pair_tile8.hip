@@ -190,7 +190,9 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     const uint8_t* const src = codes + (uint64_t)(isA ? r0 + ld : c0 + (ld - R)) * cstride + k4;
     const uint32_t dst = (uint32_t)((isA ? 0 : kOpBytes) + (isA ? ld : ld - R) * nb * kLdsRow + k4);
     const uint32_t trash_off = (uint32_t)(lay.map + 512 + 16 + k4);
-    uint32_t last0 = 0u, last1 = 0u, last2 = 0u;
+    // what this slot set in the buffers staged one, two and three chunks
+    // before the next (the buffer staged next is the one staged three ago)
+    uint32_t set3 = 0u, set2 = 0u, set1 = 0u;
     int ls = 0, lc = 0;   // next chunk to load
     auto advance = [](int& s, int& c, int n) __attribute__((always_inline)) {
         if (++c == n) { c = 0; ++s; }
@@ -208,29 +210,29 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
         return v;
     };
     // chunk with nv valid frames (codes w) into buffer b: clear what this
-    // slot set when buffer b was last staged, three chunks ago, and set the
-    // new bytes; frames past nv go to the trash row.  Offsets stay integers
-    // so the stores compile to ds_write.
-    auto stage = [&](int b, int nv, uint32_t w) __attribute__((always_inline)) {
+    // slot set when buffer b was last staged, three chunks ago (old), and set
+    // the new bytes; returns what it set.  Every chunk but a step's last is
+    // full and takes the short path; in a partial chunk the frames past nv
+    // go to the trash row.  Offsets stay integers so the stores compile to
+    // ds_write.
+    auto stage = [&](int b, int nv, uint32_t w, uint32_t old) __attribute__((always_inline)) -> uint32_t {
         const uint32_t rowoff = (uint32_t)(b * 2 * kOpBytes) + dst;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) smem_raw[rowoff + ((old >> (8 * e)) & 0xFFu) * kLdsRow + e] = 0;
+        if (nv >= kChunk) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) smem_raw[rowoff + ((w >> (8 * e)) & 0xFFu) * kLdsRow + e] = 1;
+            return w;
+        }
         const int nvk = nv - k4;
         const uint32_t vmask = nvk >= 4 ? 0xFFFFFFFFu : (nvk <= 0 ? 0u : 0xFFFFFFFFu >> (8 * (4 - nvk)));
         const uint32_t wv = w & vmask;
-        // masks rather than selects: a select of the three would be turned
-        // into a load through a selected address, which keeps them in memory
-        const uint32_t m0 = 0u - (uint32_t)(b == 0), m1 = 0u - (uint32_t)(b == 1),
-                       m2 = 0u - (uint32_t)(b == 2);
-        const uint32_t old = (last0 & m0) | (last1 & m1) | (last2 & m2);
-#pragma unroll
-        for (int e = 0; e < 4; ++e) smem_raw[rowoff + ((old >> (8 * e)) & 0xFFu) * kLdsRow + e] = 0;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const uint32_t o = nvk > e ? rowoff : trash_off;
             smem_raw[o + ((wv >> (8 * e)) & 0xFFu) * kLdsRow + e] = 1;
         }
-        last0 = (last0 & ~m0) | (wv & m0);
-        last1 = (last1 & ~m1) | (wv & m1);
-        last2 = (last2 & ~m2) | (wv & m2);
+        return wv;
     };
     v16i acc[4][2];
 #pragma unroll
@@ -355,11 +357,11 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
         if constexpr (kStg) {
             w0 = load1();
             w1 = load1();
-            stage(0, nvalid_of(ps, pc), w0);
+            set2 = stage(0, nvalid_of(ps, pc), w0, 0u);
         }
         advance(ps, pc, nch);
         if constexpr (kStg) {
-            stage(1, nvalid_of(ps, pc), w1);
+            set1 = stage(1, nvalid_of(ps, pc), w1, 0u);
             w0 = load1();
             w1 = load1();
         }
@@ -382,7 +384,10 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
             // headline tiles)
             if constexpr (kStg) {
                 // past the last chunk nvalid is 0: everything goes to the trash row
-                stage(bs, nvalid_of(ps, pc), w0);
+                const uint32_t now = stage(bs, nvalid_of(ps, pc), w0, set3);
+                set3 = set2;
+                set2 = set1;
+                set1 = now;
                 w0 = w1;
             }
             mfma_kk(pref, fa0, fb0, nbase, 0);
