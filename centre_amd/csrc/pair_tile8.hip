@@ -16,68 +16,106 @@ namespace {
 // other one's MFMAs run.  Only the first ceil(live slots/64) waves stage
 // codes (one slot per lane, lanes past the live slots repeat live ones of
 // their own wave); the others only multiply.
+template <bool kNz>
 __device__ __forceinline__ void epi_block8(const v16i& blk, int brow, int gcol, int lane,
                                            const Smem& sm, int nb, int R, int nstr, int ntab) {
     // the lane holds column gcol and rows row0 + 8*(r/4) + r%4, r = 0..15,
     // row0 = brow + 4*(lane/32) in the 32-row block starting at brow; with
     // two waves per SIMD the accumulators are ordinary VGPRs and are read in
     // place (the four-wave tile parks its AGPRs in LDS instead).  The table
-    // holds 0 for counts 0 and 1, so every count is one lookup; counts past
-    // the table are corrected afterwards (one ballot per block).
+    // holds 0 for counts 0 and 1, so every count is one lookup.  Rows and
+    // columns past the tile's last DOF are never staged, so their counts are
+    // 0 and add nothing.  Nonzero bins are counted only with kNz (the
+    // Miller-Madow correction needs them).
+    //
+    // nb >= 28: a block's 32 rows touch at most two DOFs.  A block whose
+    // first and last rows hold one DOF, or one DOF and padding rows, needs no
+    // split (wave-uniform); otherwise the next DOF starts at local row bl
+    // (wave-uniform) and the lane's sum is split into rows before and from
+    // bl (tot - hi and hi).  With kNz the nonzero counts are split the same
+    // way, at the lane's own next DOF (kk).
     const int cd = sm.col2dof[gcol];
     const int stripe = lane % nstr;
     const int row0 = brow + 4 * (lane >> 5);
+    const int dfirst = sm.row2dof[brow], dlast = sm.row2dof[brow + 31];
+    const bool one = dfirst == dlast || dlast == 0xFF;
     const int dlo = sm.row2dof[row0];
-    const bool ok = (cd != 0xFF) & (dlo != 0xFF);
-    const int kk = (dlo + 1) * nb - row0;   // first local row of the next DOF
-    // nb >= 28: the lane's 16 rows touch at most two DOFs, and a block whose
-    // first and last rows hold one DOF (wave-uniform) needs no split
-    const bool one = sm.row2dof[brow] == sm.row2dof[brow + 31];
+    const int kk = (dlo + 1) * nb - row0;                          // kNz split, lane-relative
+    const int bl = one ? 32 : __builtin_amdgcn_readfirstlane((dfirst + 1) * nb - brow);
+    const int tl = bl - 4 * (lane >> 5);                           // bl, lane-relative
+    const int split = kNz ? kk : tl;
+    int mx = 0;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) mx = max(mx, blk[r]);
     double tot = 0.0, hi = 0.0;
-    int ntot = 0, nhi = 0, mx = 0;
-    if (one) {
+    int ntot = 0, nhi = 0;
+    if (__builtin_amdgcn_ballot_w64(mx >= ntab) == 0) {
+        // every count of the block is in the table (wave-uniform)
+        if (one) {
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int c = ok ? blk[r] : 0;
-            mx = max(mx, c);
-            tot += sm.ctab[min(c, ntab - 1)];
-            ntot += min(c, 1);
-        }
-    } else {
+            for (int r = 0; r < 16; ++r) {
+                tot += sm.ctab[blk[r]];
+                if constexpr (kNz) ntot += min(blk[r], 1);
+            }
+        } else if constexpr (kNz) {
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int c = ok ? blk[r] : 0;
-            const bool h = 8 * (r >> 2) + (r & 3) >= kk;
-            mx = max(mx, c);
-            const double v = sm.ctab[min(c, ntab - 1)];
-            const int nz = min(c, 1);
-            tot += v;
-            hi += h ? v : 0.0;
-            ntot += nz;
-            nhi += h ? nz : 0;
-        }
-    }
-    if (__builtin_amdgcn_ballot_w64(mx >= ntab) != 0) {
-        const double clamp = sm.ctab[ntab - 1];
-#pragma unroll 1
-        for (int r = 0; r < 16; ++r) {
-            const int c = ok ? blk[r] : 0;
-            if (c >= ntab) {
-                const double d = (double)c * log_pos_lds((double)c, sm.logt, sm.invt, sm.invt + 256) - clamp;
-                tot += d;
-                if (!one && 8 * (r >> 2) + (r & 3) >= kk) hi += d;
+            for (int r = 0; r < 16; ++r) {
+                const bool h = 8 * (r >> 2) + (r & 3) >= kk;
+                const double v = sm.ctab[blk[r]];
+                tot += v;
+                hi += h ? v : 0.0;
+                ntot += min(blk[r], 1);
+                nhi += h ? min(blk[r], 1) : 0;
+            }
+        } else {
+            // the lane's rows come in four groups of four (q): local rows
+            // 8q..8q+3 in the first half-wave, 8q+4..8q+7 in the second, so
+            // only the group holding bl is split, and per lane only there:
+            // it adds the suffix of its four values from the first row >= bl
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const double s3 = sm.ctab[blk[4 * q + 3]];
+                const double s2 = s3 + sm.ctab[blk[4 * q + 2]];
+                const double s1 = s2 + sm.ctab[blk[4 * q + 1]];
+                const double g = s1 + sm.ctab[blk[4 * q]];
+                tot += g;
+                if (bl <= 8 * q) {
+                    hi += g;
+                } else if (bl < 8 * q + 8) {
+                    const int t = tl - 8 * q;   // rows of the group before bl
+                    hi += t <= 0 ? g : t == 1 ? s1 : t == 2 ? s2 : t == 3 ? s3 : 0.0;
+                }
             }
         }
+    } else {
+        // a count past the table: c log c from the log tables
+#pragma unroll 1
+        for (int r = 0; r < 16; ++r) {
+            const int c = blk[r];
+            const bool h = 8 * (r >> 2) + (r & 3) >= split;
+            const double v = c < ntab ? sm.ctab[c]
+                                      : (double)c * log_pos_lds((double)c, sm.logt, sm.invt, sm.invt + 256);
+            tot += v;
+            hi += h ? v : 0.0;
+            ntot += min(c, 1);
+            nhi += h ? min(c, 1) : 0;
+        }
     }
-    if (!ok) return;
-    if (ntot > nhi) {
-        atomicAdd(&sm.slotT[(dlo * R + cd) * nstr + stripe], tot - hi);
-        atomicAdd(&sm.slotN[(dlo * R + cd) * nstr + stripe], ntot - nhi);
-    }
-    const int dhi = sm.row2dof[row0 + 27];
-    if (nhi && dhi != 0xFF && dhi != dlo) {
-        atomicAdd(&sm.slotT[(dhi * R + cd) * nstr + stripe], hi);
-        atomicAdd(&sm.slotN[(dhi * R + cd) * nstr + stripe], nhi);
+    if constexpr (kNz) {
+        if (cd == 0xFF || dlo == 0xFF) return;
+        if (ntot > nhi) {
+            atomicAdd(&sm.slotT[(dlo * R + cd) * nstr + stripe], tot - hi);
+            atomicAdd(&sm.slotN[(dlo * R + cd) * nstr + stripe], ntot - nhi);
+        }
+        const int dhi = sm.row2dof[row0 + 27];
+        if (nhi && dhi != 0xFF && dhi != dlo) {
+            atomicAdd(&sm.slotT[(dhi * R + cd) * nstr + stripe], hi);
+            atomicAdd(&sm.slotN[(dhi * R + cd) * nstr + stripe], nhi);
+        }
+    } else {
+        if (cd == 0xFF || dfirst == 0xFF) return;
+        atomicAdd(&sm.slotT[(dfirst * R + cd) * nstr + stripe], tot - hi);
+        if (!one) atomicAdd(&sm.slotT[((dfirst + 1) * R + cd) * nstr + stripe], hi);
     }
 }
 
@@ -115,7 +153,7 @@ __host__ __device__ inline Lay8 lay8(int R) {
     return L;
 }
 
-template <bool kBatch>
+template <bool kBatch, bool kNz>
 __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) {
     extern __shared__ __align__(16) uint8_t smem_raw[];
     const PairKernelArgs& a = a_in;
@@ -197,15 +235,20 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     auto advance = [](int& s, int& c, int n) __attribute__((always_inline)) {
         if (++c == n) { c = 0; ++s; }
     };
+    const int nv_last = (int)(ss - (uint64_t)(nch - 1) * kChunk);   // a step's last chunk
     auto nvalid_of = [&](int s, int c) __attribute__((always_inline)) -> int {
-        return s < S ? (int)min((uint64_t)kChunk, ss - (uint64_t)c * kChunk) : 0;
+        return s < S ? (c + 1 < nch ? kChunk : nv_last) : 0;
     };
     // this slot's code dword of the next chunk to load (past the last chunk
     // a dword every row has, unused)
     auto load1 = [&]() __attribute__((always_inline)) -> uint32_t {
         const uint64_t f0 = (uint64_t)ls * sstride + (uint64_t)lc * kChunk;
         const int nv = nvalid_of(ls, lc);
-        const uint32_t v = *reinterpret_cast<const uint32_t*>(k4 < nv ? src + f0 : codes);
+        uint32_t v;
+        if (nv >= kChunk)   // wave-uniform: a full chunk, every lane's dword is in the step
+            v = *reinterpret_cast<const uint32_t*>(src + f0);
+        else
+            v = *reinterpret_cast<const uint32_t*>(k4 < nv ? src + f0 : codes);
         advance(ls, lc, nch);
         return v;
     };
@@ -294,7 +337,7 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
-                epi_block8(acc[i][j], 128 * wr + 32 * i, 64 * wc + 32 * j + (lane & 31), lane, sm,
+                epi_block8<kNz>(acc[i][j], 128 * wr + 32 * i, 64 * wc + 32 * j + (lane & 31), lane, sm,
                            nb, R, nstr, ntab);
                 __builtin_amdgcn_sched_barrier(0);
             }
@@ -421,8 +464,9 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     if (wave == 7 && fin >= 0) finalize(fin);
 }
 
+template <bool kNz>
 __global__ __launch_bounds__(512) void pair_tile8_kernel(PairKernelArgs a) {
-    tile8_body<false>(a, blockIdx.x);
+    tile8_body<false, kNz>(a, blockIdx.x);
 }
 
 // Batch form of the eight-wave tile: one workgroup per entry of the ordered
@@ -430,10 +474,11 @@ __global__ __launch_bounds__(512) void pair_tile8_kernel(PairKernelArgs a) {
 // which is the work queue of the persistent form; a loop around the tile
 // body would push it past the two-waves-per-SIMD register budget.  The
 // workgroup finishing a system's last tile stamps the system's completion.
+template <bool kNz>
 __global__ __launch_bounds__(512) void pair_batch8_kernel(PairKernelArgs a) {
     const int t = blockIdx.x;
     if (t == 0 && threadIdx.x == 0 && a.t0) *a.t0 = wall_clock64();
-    tile8_body<true>(a, t);
+    tile8_body<true, kNz>(a, t);
     if (threadIdx.x == 0 && a.sys_count) {
         const int sys = a.tile_sys[t];
         __threadfence();
@@ -454,10 +499,14 @@ hipError_t launch_pair_tiles8(const PairKernelArgs& a, int n_tiles, hipStream_t 
     if (n_tiles <= 0) return hipSuccess;
     if (!pair_tile8_supported(a.R)) return hipErrorInvalidValue;
     const size_t lds = lay8(a.R).total;
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&pair_tile8_kernel),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    // nonzero bins are counted only for the Miller-Madow estimator
+    const bool nz = a.slot_mm >= 0 || a.mi_from_mm;
+    const void* const fn = nz ? reinterpret_cast<const void*>(&pair_tile8_kernel<true>)
+                              : reinterpret_cast<const void*>(&pair_tile8_kernel<false>);
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(pair_tile8_kernel, dim3(n_tiles), dim3(512), lds, st, a);
+    if (nz) hipLaunchKernelGGL(pair_tile8_kernel<true>, dim3(n_tiles), dim3(512), lds, st, a);
+    else hipLaunchKernelGGL(pair_tile8_kernel<false>, dim3(n_tiles), dim3(512), lds, st, a);
     return hipGetLastError();
 }
 
@@ -465,10 +514,13 @@ hipError_t launch_pair_batch8(const PairKernelArgs& a, hipStream_t st) {
     if (a.n_work <= 0) return hipSuccess;
     if (!pair_tile8_supported(a.R)) return hipErrorInvalidValue;
     const size_t lds = lay8(a.R).total;
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&pair_batch8_kernel),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const bool nz = a.slot_mm >= 0 || a.mi_from_mm;
+    const void* const fn = nz ? reinterpret_cast<const void*>(&pair_batch8_kernel<true>)
+                              : reinterpret_cast<const void*>(&pair_batch8_kernel<false>);
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(pair_batch8_kernel, dim3(a.n_work), dim3(512), lds, st, a);
+    if (nz) hipLaunchKernelGGL(pair_batch8_kernel<true>, dim3(a.n_work), dim3(512), lds, st, a);
+    else hipLaunchKernelGGL(pair_batch8_kernel<false>, dim3(a.n_work), dim3(512), lds, st, a);
     return hipGetLastError();
 }
 

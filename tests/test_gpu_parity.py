@@ -157,13 +157,19 @@ def test_ragged_shapes_vs_oracle(ctx, nb, D, F, S):
     assert rel_err(s2, w2[:, :, 0, :]) < REL
 
 
+@pytest.mark.parametrize("ests", [(0, 1), (0,)])
 @pytest.mark.parametrize("nb,D,F,S", [(50, 20, 4000, 4), (42, 18, 2500, 5), (36, 14, 3333, 3),
-                                      (50, 23, 6002, 2)])
-def test_full_tiles_vs_oracle(ctx, nb, D, F, S):
+                                      (50, 23, 6002, 2), (29, 17, 1500, 2), (31, 16, 2000, 3),
+                                      (33, 15, 1000, 2), (40, 13, 1200, 2), (45, 11, 1100, 2),
+                                      (130, 4, 700, 2)])
+def test_full_tiles_vs_oracle(ctx, nb, D, F, S, ests):
     """Tiles with R DOFs on both sides (diagonal ones included) run on the
-    eight-wave kernel for 33 <= nb <= 51; shapes with whole blocks only, a
+    eight-wave kernel for nb >= 29; shapes with whole blocks only, a
     partial last block, and step lengths that are and are not a multiple of
-    the 64-frame chunk."""
+    the 64-frame chunk.  With ML alone the kernel leaves out the nonzero-bin
+    counts and splits each 32-row block at its DOF boundary group-wise, with
+    MM it counts them and splits row by row: both forms, at bin counts that
+    put the boundaries at every row offset mod 4."""
     rng = np.random.default_rng(nb * 7 + D)
     codes = rng.integers(0, nb, size=(D, F), dtype=np.uint8)
     codes[1::3] = (codes[0::3][: len(codes[1::3])] + rng.integers(0, 3, size=F)) % nb
@@ -173,14 +179,18 @@ def test_full_tiles_vs_oracle(ctx, nb, D, F, S):
     ext[:, 0] = rng.uniform(0.5, 1.0, D)
     ext[:, 1] = ext[:, 0] + rng.uniform(0.2, 1.5, D)
     iu, ju = np.triu_indices(D, 1)
-    print(f"full tiles nb={nb} D={D} F={F} S={S}", flush=True)
-    s1, s2, mi = ctx.pair_entropy(codes, types, ext, nb, S, ests=(0, 1))
-    w2 = O.pair_entropies(codes, types, ext, np.stack([iu, ju], 1), S, [nb], (0, 1))
+    print(f"full tiles nb={nb} D={D} F={F} S={S} ests={ests}", flush=True)
+    s1, s2, mi = ctx.pair_entropy(codes, types, ext, nb, S, ests=ests)
+    w2 = O.pair_entropies(codes, types, ext, np.stack([iu, ju], 1), S, [nb], ests)
     assert rel_err(s2, w2[:, :, 0, :]) < REL
+    w1 = O.dof_entropies(codes, types, ext, S, [nb], ests)
+    wmi = w1[iu, 0, 0, :] + w1[ju, 0, 0, :] - w2[:, 0, 0, :]
+    assert rel_err(mi, wmi) < 1e-9
     pairs = np.stack([iu, ju], 1)[rng.permutation(len(iu))[:60]]
     pairs[::2] = pairs[::2, ::-1]
-    s1, s2, mi = ctx.pair_entropy(codes, types, ext, nb, S, pairs=pairs, ests=(1, 0))
-    w2 = O.pair_entropies(codes, types, ext, pairs, S, [nb], (1, 0))
+    rev = ests[::-1]
+    s1, s2, mi = ctx.pair_entropy(codes, types, ext, nb, S, pairs=pairs, ests=rev)
+    w2 = O.pair_entropies(codes, types, ext, pairs, S, [nb], rev)
     assert rel_err(s2, w2[:, :, 0, :]) < REL
 
 
