@@ -50,12 +50,21 @@ __device__ __forceinline__ void epi_block8(const v16i& blk, int brow, int gcol, 
     double tot = 0.0, hi = 0.0;
     int ntot = 0, nhi = 0;
     if (__builtin_amdgcn_ballot_w64(mx >= ntab) == 0) {
-        // every count of the block is in the table (wave-uniform)
+        // every count of the block is in the table (wave-uniform); the 16
+        // lookups are issued together and summed as a tree, so that their
+        // latencies overlap
         if (one) {
+            double v[16];
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                tot += sm.ctab[blk[r]];
-                if constexpr (kNz) ntot += min(blk[r], 1);
+            for (int r = 0; r < 16; ++r) v[r] = sm.ctab[blk[r]];
+#pragma unroll
+            for (int r = 0; r < 8; ++r) v[r] = v[2 * r] + v[2 * r + 1];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) v[r] = v[2 * r] + v[2 * r + 1];
+            tot = (v[0] + v[1]) + (v[2] + v[3]);
+            if constexpr (kNz) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r) ntot += min(blk[r], 1);
             }
         } else if constexpr (kNz) {
 #pragma unroll
@@ -72,20 +81,29 @@ __device__ __forceinline__ void epi_block8(const v16i& blk, int brow, int gcol, 
             // 8q..8q+3 in the first half-wave, 8q+4..8q+7 in the second, so
             // only the group holding bl is split, and per lane only there:
             // it adds the suffix of its four values from the first row >= bl
+            double v[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) v[r] = sm.ctab[blk[r]];
+            double g[4];
+            hi = -0.0;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const double s3 = sm.ctab[blk[4 * q + 3]];
-                const double s2 = s3 + sm.ctab[blk[4 * q + 2]];
-                const double s1 = s2 + sm.ctab[blk[4 * q + 1]];
-                const double g = s1 + sm.ctab[blk[4 * q]];
-                tot += g;
+                const double s3 = v[4 * q + 3];
+                const double s2 = s3 + v[4 * q + 2];
+                const double s1 = s2 + v[4 * q + 1];
+                g[q] = s1 + v[4 * q];
                 if (bl <= 8 * q) {
-                    hi += g;
+                    hi += g[q];
                 } else if (bl < 8 * q + 8) {
                     const int t = tl - 8 * q;   // rows of the group before bl
-                    hi += t <= 0 ? g : t == 1 ? s1 : t == 2 ? s2 : t == 3 ? s3 : 0.0;
+                    double part = t == 3 ? s3 : 0.0;
+                    part = t == 2 ? s2 : part;
+                    part = t == 1 ? s1 : part;
+                    part = t <= 0 ? g[q] : part;
+                    hi += part;
                 }
             }
+            tot = (g[0] + g[1]) + (g[2] + g[3]);
         }
     } else {
         // a count past the table: c log c from the log tables
