@@ -611,26 +611,30 @@ struct BatchRun {
 
 // Launches over a prepared tile list: tiles of a system's last, partial
 // block first (persistent four-wave tile), then the full tiles (eight-wave
-// tile, one workgroup per tile in list order), each list in the given order.  Every system's completion stamp is
-// taken by whichever launch finishes its last tile.
+// tile, one workgroup per tile in list order), each list in the given order;
+// at nb <= 8 every tile runs on the eight-wave small-bin tile.  Every
+// system's completion stamp is taken by whichever launch finishes its last
+// tile.
 int batch_launch(centre_ctx* ctx, const PairKernelArgs& base, const std::vector<BatchSystem>& bs,
                  const std::vector<BatchTile>& tl, int grid, BatchRun* run) {
     const int n_sys = (int)bs.size();
-    const bool w8 = pair_tile8_supported(base.R);
+    const bool w8 = pair_tile8_supported(base.R, base.nb);
+    const bool w8all = w8 && pair_tile8_small(base.nb);   // partial tiles too
+    auto eight = [&](const BatchTile& q) {
+        const int nfull = bs[q.sys].n_dof / base.R;
+        return w8all || (w8 && q.I < nfull && q.J < nfull);
+    };
     std::vector<int2> tiles;
     std::vector<int> tsys, per(n_sys, 0);
     for (int pass = 0; pass < 2; ++pass)
         for (const BatchTile& q : tl) {
-            const int nfull = bs[q.sys].n_dof / base.R;
-            const bool full = w8 && q.I < nfull && q.J < nfull;
-            if (full != (pass == 1)) continue;
+            if (eight(q) != (pass == 1)) continue;
             tiles.push_back(make_int2(q.I, q.J));
             tsys.push_back(q.sys);
             per[q.sys]++;
         }
     int n_part = 0;
-    for (const BatchTile& q : tl)
-        n_part += !(w8 && q.I < bs[q.sys].n_dof / base.R && q.J < bs[q.sys].n_dof / base.R);
+    for (const BatchTile& q : tl) n_part += !eight(q);
     DevBuf dt, ds, dsys, dper, dcnt, dtime, dwork;
     HIPCHK(dt.alloc(sizeof(int2) * tiles.size()));
     HIPCHK(ds.alloc(sizeof(int) * tsys.size()));
@@ -841,7 +845,8 @@ int choose_split(centre_ctx* ctx, const centre_pair_plan* plan, uint64_t ss, int
                  size_t max_bytes, PairLayout* l) {
     const int nb = plan->nbins;
     l->rec = (nb * nb + 3) & ~3;
-    if (nb > 16 || plan->n_tiles <= 0) return 0;
+    // nb <= 8 runs on the small-bin eight-wave tile, which needs no split
+    if (nb > 16 || plan->n_tiles <= 0 || pair_tile8_small(nb)) return 0;
     int ncu = 0;
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
     int k = std::min<int>(8, ncu / plan->n_tiles);
@@ -922,7 +927,11 @@ int run_pair_fused(centre_ctx* ctx, const centre_pair_plan* plan, const uint8_t*
         dumpargs.mi = nullptr;
         HIPCHK(launch_pair_tiles(dumpargs, plan->n_tiles, true, ctx->stream));
         HIPCHK(launch_split_entropy(sp, plan->n_tiles, ctx->stream));
-    } else if (pair_tile8_supported(a.R) && plan->n_full > 0 &&
+    } else if (pair_tile8_supported(a.R, nb) && pair_tile8_small(nb) &&
+               ((a.stride | a.sstride | reinterpret_cast<uintptr_t>(a.codes)) & 3) == 0) {
+        // every tile, partial ones included, on the small-bin eight-wave kernel
+        HIPCHK(launch_pair_tiles8(a, plan->n_tiles, ctx->stream));
+    } else if (pair_tile8_supported(a.R, nb) && plan->n_full > 0 &&
                ((a.stride | a.sstride | reinterpret_cast<uintptr_t>(a.codes)) & 3) == 0) {
         // full tiles on the eight-wave kernel, those of a partial block after
         HIPCHK(launch_pair_tiles8(a, plan->n_full, ctx->stream));

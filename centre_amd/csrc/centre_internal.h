@@ -70,9 +70,11 @@ constexpr int kChunk = 64;
 // the 16-byte fragment reads of one row block over all banks.
 constexpr int kLdsRow = kChunk + 16;
 
-// DOFs per tile side; capped at 60 so that the per-pair LDS slots of very
-// small bin counts still fit next to the one-hot buffers (160 KB LDS).
+// DOFs per tile side: 32 for nb <= 8, whose tiles give each DOF 8 one-hot
+// rows; otherwise as many as fit 256 rows, capped at 60 so that the per-pair
+// LDS slots still fit next to the one-hot buffers (160 KB LDS).
 inline int dofs_per_tile(int nb) {
+    if (nb <= 8) return 32;
     int r = kTileRows / nb;
     return r > 60 ? 60 : r;
 }
@@ -141,7 +143,8 @@ struct PairKernelArgs {
 // Eight-wave kernel for tiles with R DOFs on both sides and step-aligned codes
 // (no count dump); supported for R <= 8 (nb >= 29) when its LDS layout keeps
 // the c log c table.
-bool pair_tile8_supported(int R);
+bool pair_tile8_small(int nb);
+bool pair_tile8_supported(int R, int nb);
 hipError_t launch_pair_tiles8(const PairKernelArgs& a, int n_tiles, hipStream_t st);
 // Batch form (full tiles only, step-aligned codes): one workgroup per entry
 // of a.tiles[0 .. a.n_work), dispatched in list order.

@@ -137,6 +137,76 @@ __device__ __forceinline__ void epi_block8(const v16i& blk, int brow, int gcol, 
     }
 }
 
+// Epilogue block of the small-bin tile (nb <= 8): every DOF has 8 one-hot
+// rows (bins past nb stay zero), so a 32-row block holds four row DOFs and
+// the lane's four row groups q (rows 8q..8q+3 in the first half-wave,
+// 8q+4..8q+7 in the second) each lie in one of them; the lane's column is
+// bin gcol % 8 of column DOF gcol / 8.  The eight lanes of a column DOF sum
+// their group totals by DPP row shifts, and their last lane adds the sum to
+// the pair's slot (one stripe per half-wave).
+template <int kCtl>
+__device__ __forceinline__ double dpp_d(double v) {
+    // a DPP move of both halves of a double; lanes whose source lies outside
+    // their 16-lane row read 0
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, kCtl, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), kCtl, 0xF, 0xF, true);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// lane l ends with the sum over lanes l-7..l of its 16-lane row (row_shr 1,
+// 2, 4), so lane 8k+7 holds the sum of lanes 8k..8k+7
+__device__ __forceinline__ double dpp_sum8(double v) {
+    v += dpp_d<0x111>(v);
+    v += dpp_d<0x112>(v);
+    v += dpp_d<0x114>(v);
+    return v;
+}
+
+__device__ __forceinline__ int dpp_sum8_i(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, true);
+    return v;
+}
+
+template <bool kNz>
+__device__ __forceinline__ void epi_block_small(const v16i& blk, int brow, int gcol, int lane,
+                                                const Smem& sm, int R, int nstr, int ntab) {
+    const double* const coef = sm.invt + 256;
+    int mx = 0;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) mx = max(mx, blk[r]);
+    const bool big = __builtin_amdgcn_ballot_w64(mx >= ntab) != 0;   // wave-uniform
+    const int cd = gcol >> 3;
+    const int stripe = lane >> 5;
+    const bool last = (lane & 7) == 7;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        double v[4];
+        if (!big) {
+#pragma unroll
+            for (int m = 0; m < 4; ++m) v[m] = sm.ctab[blk[4 * q + m]];
+        } else {
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const int c = blk[4 * q + m];
+                v[m] = c < ntab ? sm.ctab[c] : (double)c * log_pos_lds((double)c, sm.logt, sm.invt, coef);
+            }
+        }
+        const double g = dpp_sum8((v[0] + v[1]) + (v[2] + v[3]));
+        const int rd = (brow >> 3) + q;
+        if (last) atomicAdd(&sm.slotT[(rd * R + cd) * nstr + stripe], g);
+        if constexpr (kNz) {
+            int nz = 0;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) nz += min(blk[4 * q + m], 1);
+            nz = dpp_sum8_i(nz);
+            if (last) atomicAdd(&sm.slotN[(rd * R + cd) * nstr + stripe], nz);
+        }
+    }
+}
+
 // LDS of the eight-wave tile: kBuf8 one-hot operand buffers (chunk g is
 // multiplied from one, chunk g+1's fragments are read from the next while
 // that happens, chunk g+2 is staged into the third), the DOF maps and a
@@ -149,17 +219,20 @@ struct Lay8 {
     int nstripe, ntab;
 };
 
-__host__ __device__ inline Lay8 lay8(int R) {
+// nz: nonzero-bin slots (Miller-Madow) are kept; without them the c log c
+// table takes their room.  The small-bin tile's map area also holds the
+// 64-bit mask of the 32x32 blocks that contain a pair to evaluate.
+__host__ __device__ inline Lay8 lay8(int R, bool small, bool nz) {
     Lay8 L;
     size_t off = (size_t)kBuf8 * 2 * kOpBytes;
     L.map = off;
-    off += 512 + kLdsRow;
+    off += 512 + kLdsRow + 16;
     off = (off + 15) & ~(size_t)15;
-    L.nstripe = 16;
+    L.nstripe = small ? 2 : 16;
     L.T = off;
     off += sizeof(double) * R * R * L.nstripe;
     L.N = off;
-    off += sizeof(int) * R * R * L.nstripe;
+    if (nz) off += sizeof(int) * R * R * L.nstripe;
     off = (off + 15) & ~(size_t)15;
     L.log = off;   // log(1 + j/256), 1/(1 + j/256), then 8 series coefficients
     off += (2 * 256 + 8) * sizeof(double);
@@ -171,7 +244,7 @@ __host__ __device__ inline Lay8 lay8(int R) {
     return L;
 }
 
-template <bool kBatch, bool kNz>
+template <bool kBatch, bool kNz, bool kSmall>
 __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) {
     extern __shared__ __align__(16) uint8_t smem_raw[];
     const PairKernelArgs& a = a_in;
@@ -183,7 +256,7 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     const uint64_t ss = kBatch ? sdp->ss : a.ss;
     const uint64_t sstride = kBatch ? sdp->sstride : a.sstride;
     const int R = a.R, nb = a.nb, S = a.nsteps;
-    const Lay8 lay = lay8(R);
+    const Lay8 lay = lay8(R, kSmall, kNz);
     const int nstr = lay.nstripe, ntab = lay.ntab;
     Smem sm;
     sm.oh = smem_raw;
@@ -209,7 +282,7 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     for (int i = tid; i < kBuf8 * 2 * kOpBytes / 16; i += 512)
         reinterpret_cast<uint4*>(sm.oh)[i] = make_uint4(0, 0, 0, 0);
     if (tid < 256) {
-        const int d = tid / nb;
+        const int d = tid / (kSmall ? 8 : nb);
         sm.row2dof[tid] = (uint8_t)(d < R ? d : 0xFF);
         sm.col2dof[tid] = (uint8_t)(d < R ? d : 0xFF);
         const double t = 1.0 + tid * (1.0 / 256.0);
@@ -223,14 +296,29 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     }
     for (int i = tid; i < R * R * nstr; i += 512) {
         sm.slotT[i] = 0.0;
-        sm.slotN[i] = 0;
+        if constexpr (kNz) sm.slotN[i] = 0;
     }
     for (int c = tid; c < ntab; c += 512) sm.ctab[c] = c > 1 ? (double)c * log((double)c) : 0.0;
     if (tid == 0) *reinterpret_cast<int*>(trash) = *reinterpret_cast<volatile int*>(a.flag);
+    // small-bin tile: bit 8*rb + cb marks a 32x32 MFMA block (row DOFs
+    // 4rb..4rb+3, column DOFs 4cb..4cb+3) holding a pair to evaluate; the
+    // epilogue skips the others (a diagonal tile's lower half, the pairs
+    // an explicit list leaves out)
+    unsigned long long* const blkmask = reinterpret_cast<unsigned long long*>(trash + kLdsRow);
+    if (kSmall && tid == 0) *blkmask = 0ull;
     __syncthreads();
     // codes were range-checked by the marginal pass; with the flag set the
     // results are discarded
     if (*reinterpret_cast<const int*>(trash) != 0) return;
+    if constexpr (kSmall) {
+        for (int pq = tid; pq < R * R; pq += 512) {
+            const int rd = pq / R, cd = pq - rd * R;
+            const long long gi = (long long)tile.x * R + rd, gj = (long long)tile.y * R + cd;
+            const bool want = gi < gj && gj < D &&
+                              (a.tilemap == nullptr || a.tilemap[(size_t)tix * R * R + pq] >= 0);
+            if (want) atomicOr(blkmask, 1ull << (8 * (rd >> 2) + (cd >> 2)));
+        }
+    }
 
     const int nch = (int)((ss + kChunk - 1) / kChunk);
     const int G = S * nch;
@@ -239,7 +327,7 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     // waves stage one slot per lane (lanes past nlive repeat live slots of
     // their own wave); spreading the slots over all eight waves instead
     // (nlive/8 lanes each) ran 20% slower.
-    const bool stager = wave * 64 < nlive;   // wave-uniform
+    const bool stager = kSmall ? wave * 64 < (diag ? 1 : 2) * R * 4 : wave * 64 < nlive;
     const int q = (wave * 64 + lane) % nlive;
     const int ld = q / kGroups, k4 = (q - ld * kGroups) * 4;
     const bool isA = ld < R;
@@ -249,6 +337,19 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     // what this slot set in the buffers staged one, two and three chunks
     // before the next (the buffer staged next is the one staged three ago)
     uint32_t set3 = 0u, set2 = 0u, set1 = 0u;
+    // Small-bin tile: this lane's item is DOF sd of the two operands and
+    // frames g16..g16+15 of each chunk, one 16-byte row segment per bin;
+    // the first waves stage (256 items, 128 on a diagonal tile).  Rows past
+    // the system's last DOF read row 0 and stage no frames.
+    const int nlive_s = (diag ? 1 : 2) * R * 4;
+    const int qs = (wave * 64 + lane) % nlive_s;
+    const int sd = qs >> 2, g16 = (qs & 3) * 16;
+    const bool sA = sd < R;
+    const long long srow = sA ? r0 + sd : c0 + (sd - R);
+    const bool srow_ok = srow < D;
+    const uint8_t* const src_s = codes + (uint64_t)(srow_ok ? srow : 0) * cstride + g16;
+    const uint32_t dst_s = (uint32_t)((sA ? 0 : kOpBytes) + (sA ? sd : sd - R) * 8 * kLdsRow + g16);
+    const bool stager_s = wave * 64 < nlive_s;   // wave-uniform
     int ls = 0, lc = 0;   // next chunk to load
     auto advance = [](int& s, int& c, int n) __attribute__((always_inline)) {
         if (++c == n) { c = 0; ++s; }
@@ -269,6 +370,53 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
             v = *reinterpret_cast<const uint32_t*>(k4 < nv ? src + f0 : codes);
         advance(ls, lc, nch);
         return v;
+    };
+    // small-bin tile: the 16 code bytes of the next chunk to load (past the
+    // last chunk or the step's valid frames, bytes of the first row, unused)
+    auto load4 = [&]() __attribute__((always_inline)) -> uint4 {
+        const uint64_t f0 = (uint64_t)ls * sstride + (uint64_t)lc * kChunk;
+        const int nv = nvalid_of(ls, lc);
+        uint32_t v[4];
+        if (nv >= kChunk) {   // wave-uniform: every dword is in the step
+#pragma unroll
+            for (int d = 0; d < 4; ++d) v[d] = *reinterpret_cast<const uint32_t*>(src_s + f0 + 4 * d);
+        } else {
+#pragma unroll
+            for (int d = 0; d < 4; ++d)
+                v[d] = *reinterpret_cast<const uint32_t*>(g16 + 4 * d < nv ? src_s + f0 + 4 * d : codes);
+        }
+        advance(ls, lc, nch);
+        return make_uint4(v[0], v[1], v[2], v[3]);
+    };
+    // small-bin tile: chunk with nv valid frames (codes w) into buffer b.
+    // Each bin's 16 one-hot bytes are one byte permute per code dword: the
+    // permute selects byte `code` of an 8-byte table holding 1 at the bin,
+    // and selector 12 (an invalid frame) reads 0.  Rows of bins past nb stay
+    // zero, every other row is overwritten whole, so nothing is cleared.
+    auto stage4 = [&](int b, int nv, uint4 w) __attribute__((always_inline)) {
+        const int nvk = (srow_ok ? nv : 0) - g16;
+        uint32_t x[4] = {w.x, w.y, w.z, w.w};
+        if (nvk < 16) {
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+                const int n = nvk - 4 * d;
+                const uint32_t keep = n >= 4 ? 0xFFFFFFFFu : (n <= 0 ? 0u : 0xFFFFFFFFu >> (8 * (4 - n)));
+                x[d] = (x[d] & keep) | (0x0C0C0C0Cu & ~keep);
+            }
+        }
+        const uint32_t base = (uint32_t)(b * 2 * kOpBytes) + dst_s;
+#pragma unroll
+        for (int bin = 0; bin < 8; ++bin) {
+            if (bin >= nb) break;   // uniform
+            const uint32_t lo = bin < 4 ? 1u << (8 * bin) : 0u;
+            const uint32_t hi = bin < 4 ? 0u : 1u << (8 * (bin - 4));
+            uint4 o;
+            o.x = __builtin_amdgcn_perm(hi, lo, x[0]);
+            o.y = __builtin_amdgcn_perm(hi, lo, x[1]);
+            o.z = __builtin_amdgcn_perm(hi, lo, x[2]);
+            o.w = __builtin_amdgcn_perm(hi, lo, x[3]);
+            *reinterpret_cast<uint4*>(smem_raw + base + bin * kLdsRow) = o;
+        }
     };
     // chunk with nv valid frames (codes w) into buffer b: clear what this
     // slot set when buffer b was last staged, three chunks ago (old), and set
@@ -295,22 +443,46 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
         }
         return wv;
     };
+    using CodeT = std::conditional_t<kSmall, uint4, uint32_t>;
+    auto loadc = [&]() __attribute__((always_inline)) -> CodeT {
+        if constexpr (kSmall) return load4();
+        else return load1();
+    };
+    // stages a chunk; returns what the byte scatter set (0 for the small-bin tile)
+    auto stagec = [&](int b, int nv, CodeT w, uint32_t old) __attribute__((always_inline)) -> uint32_t {
+        if constexpr (kSmall) {
+            stage4(b, nv, w);
+            return 0u;
+        } else {
+            return stage(b, nv, w, old);
+        }
+    };
     v16i acc[4][2];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = v16i{};
-    // fragment i of the row operand / j of the column operand, 32 frames kk
+    // fragment i of the row operand / j of the column operand, 32 frames kk.
+    // The wave's 4 row blocks and 2 column blocks of 32: contiguous (rows
+    // 128wr.., columns 64wc..), or on the small-bin tile interleaved (rows
+    // 32(wr + 2i), columns 32(wc + 4j)), so that the blocks whose epilogue
+    // is skipped (a triangle of the tile) are spread over the waves.
     const int lrow = (lane & 31) * kLdsRow + (lane >> 5) * 16;
+    auto rowblk = [&](int i) __attribute__((always_inline)) {
+        return kSmall ? 32 * (wr + 2 * i) : 128 * wr + 32 * i;
+    };
+    auto colblk = [&](int j) __attribute__((always_inline)) {
+        return kSmall ? 32 * (wc + 4 * j) : 64 * wc + 32 * j;
+    };
     // buffer offsets stay integers and every address is formed from the
     // __shared__ array itself, so the reads compile to ds_read (a generic
     // pointer carried through the loop compiles to flat loads)
     auto rdA = [&](uint32_t bo, int kk, int i) __attribute__((always_inline)) -> v4i {
-        return *reinterpret_cast<const v4i*>(smem_raw + bo + (128 * wr + 32 * i) * kLdsRow + lrow +
+        return *reinterpret_cast<const v4i*>(smem_raw + bo + rowblk(i) * kLdsRow + lrow +
                                              kk * 32);
     };
     auto rdB = [&](uint32_t bo, int kk, int j) __attribute__((always_inline)) -> v4i {
-        return *reinterpret_cast<const v4i*>(smem_raw + bo + boff + (64 * wc + 32 * j) * kLdsRow +
+        return *reinterpret_cast<const v4i*>(smem_raw + bo + boff + colblk(j) * kLdsRow +
                                              lrow + kk * 32);
     };
     v4i fa0[4], fb0[2], fa1[4], fb1[2];
@@ -351,19 +523,31 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     // into the per-pair LDS slots.
     auto epilogue = [&]() __attribute__((always_inline)) {
         __syncthreads();   // the pending finalize has read and cleared the slots
+        unsigned long long need = ~0ull;
+        if constexpr (kSmall) {
+            const unsigned long long m = *blkmask;
+            need = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(m >> 32)) << 32) |
+                   (unsigned)__builtin_amdgcn_readfirstlane((unsigned)m);
+        }
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
-                epi_block8<kNz>(acc[i][j], 128 * wr + 32 * i, 64 * wc + 32 * j + (lane & 31), lane, sm,
-                           nb, R, nstr, ntab);
+                if constexpr (kSmall) {
+                    if ((need >> (8 * (rowblk(i) >> 5) + (colblk(j) >> 5))) & 1)
+                        epi_block_small<kNz>(acc[i][j], rowblk(i), colblk(j) + (lane & 31), lane, sm,
+                                             R, nstr, ntab);
+                }
+                else
+                    epi_block8<kNz>(acc[i][j], rowblk(i), colblk(j) + (lane & 31), lane,
+                                    sm, nb, R, nstr, ntab);
                 __builtin_amdgcn_sched_barrier(0);
             }
     };
-    // Slot sums of step fs -> S2 / MI, by wave 7 (never a stager) one
-    // iteration after the epilogue, so that it overlaps the next chunk; the
-    // iteration barrier orders it after the atomics, and the next epilogue's
-    // opening barrier after it.
+    // Slot sums of step fs -> S2 / MI, by wave 7 (never a stager; on the
+    // small-bin tile by every wave) one iteration after the epilogue, so
+    // that it overlaps the next chunk; the iteration barrier orders it after
+    // the atomics, and the next epilogue's opening barrier after it.
     auto finalize = [&](int fs) __attribute__((always_inline)) {
         const double* const coef = sm.invt + 256;
         const double n = (double)(fs + 1) * (double)ss;
@@ -374,18 +558,22 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
         const double* const dof_w = kBatch ? sdp->dof_w : a.dof_w;
         const double* const dof_L = kBatch ? sdp->dof_L : a.dof_L;
         const double* const dof_S1 = kBatch ? sdp->dof_S1 : a.dof_S1;
-        for (int pq = lane; pq < R * R; pq += 64) {
+        for (int pq = kSmall ? tid : lane; pq < R * R; pq += kSmall ? 512 : 64) {
             const int rd = pq / R, cd = pq - rd * R;
             const long long gi = r0 + rd, gj = c0 + cd;
-            const long long p = gi >= gj ? -1
-                              : (a.tilemap ? (long long)a.tilemap[(size_t)tix * R * R + pq] : tri_index(gi, gj, D));
+            const long long p = gi >= gj || gj >= D
+                                    ? -1
+                                    : (a.tilemap ? (long long)a.tilemap[(size_t)tix * R * R + pq]
+                                                 : tri_index(gi, gj, D));
             double T = 0.0;
             int nz = 0;
             for (int k = 0; k < nstr; ++k) {
                 T += sm.slotT[pq * nstr + k];
-                nz += sm.slotN[pq * nstr + k];
                 sm.slotT[pq * nstr + k] = 0.0;
-                sm.slotN[pq * nstr + k] = 0;
+                if constexpr (kNz) {
+                    nz += sm.slotN[pq * nstr + k];
+                    sm.slotN[pq * nstr + k] = 0;
+                }
             }
             if (p >= 0) {
                 const double L = dof_L[gi * S + fs] + dof_L[gj * S + fs];
@@ -414,17 +602,17 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     auto run = [&](auto stg) __attribute__((always_inline)) {
         constexpr bool kStg = decltype(stg)::value;
         // ring of code dwords: chunks g+2 and g+3 before chunk g's staging
-        uint32_t w0 = 0u, w1 = 0u;
+        CodeT w0{}, w1{};
         if constexpr (kStg) {
-            w0 = load1();
-            w1 = load1();
-            set2 = stage(0, nvalid_of(ps, pc), w0, 0u);
+            w0 = loadc();
+            w1 = loadc();
+            set2 = stagec(0, nvalid_of(ps, pc), w0, 0u);
         }
         advance(ps, pc, nch);
         if constexpr (kStg) {
-            set1 = stage(1, nvalid_of(ps, pc), w1, 0u);
-            w0 = load1();
-            w1 = load1();
+            set1 = stagec(1, nvalid_of(ps, pc), w1, 0u);
+            w0 = loadc();
+            w1 = loadc();
         }
         advance(ps, pc, nch);
         __syncthreads();
@@ -437,15 +625,15 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
             const int bn = b == kBuf8 - 1 ? 0 : b + 1;
             const int bs = bn == kBuf8 - 1 ? 0 : bn + 1;
             const uint32_t nbase = (uint32_t)(bn * 2 * kOpBytes);
-            uint32_t w2 = 0u;
-            if constexpr (kStg) w2 = load1();   // chunk g+4
+            CodeT w2{};
+            if constexpr (kStg) w2 = loadc();   // chunk g+4
             // the staging goes first: its scattered byte writes (bank
             // conflicts depend on the codes) then drain behind the MFMAs
             // instead of holding up the barrier (29.8 against 31.4 ms on the
             // headline tiles)
             if constexpr (kStg) {
                 // past the last chunk nvalid is 0: everything goes to the trash row
-                const uint32_t now = stage(bs, nvalid_of(ps, pc), w0, set3);
+                const uint32_t now = stagec(bs, nvalid_of(ps, pc), w0, set3);
                 set3 = set2;
                 set2 = set1;
                 set1 = now;
@@ -453,9 +641,11 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
             }
             mfma_kk(pref, fa0, fb0, nbase, 0);
             mfma_kk(pref, fa1, fb1, nbase, 1);
-            if constexpr (kStg) {
-                w1 = w2;
-            } else {
+            if constexpr (kStg) w1 = w2;
+            // the small-bin tile's 1,024 pairs: every wave, two pairs a lane
+            if constexpr (kSmall) {
+                if (fin >= 0) finalize(fin);
+            } else if constexpr (!kStg) {
                 if (fin >= 0 && wave == 7) finalize(fin);
             }
             fin = -1;
@@ -479,12 +669,12 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     };
     if (stager) run(std::true_type{});
     else run(std::false_type{});
-    if (wave == 7 && fin >= 0) finalize(fin);
+    if (fin >= 0 && (kSmall || wave == 7)) finalize(fin);
 }
 
-template <bool kNz>
+template <bool kNz, bool kSmall>
 __global__ __launch_bounds__(512) void pair_tile8_kernel(PairKernelArgs a) {
-    tile8_body<false, kNz>(a, blockIdx.x);
+    tile8_body<false, kNz, kSmall>(a, blockIdx.x);
 }
 
 // Batch form of the eight-wave tile: one workgroup per entry of the ordered
@@ -492,11 +682,11 @@ __global__ __launch_bounds__(512) void pair_tile8_kernel(PairKernelArgs a) {
 // which is the work queue of the persistent form; a loop around the tile
 // body would push it past the two-waves-per-SIMD register budget.  The
 // workgroup finishing a system's last tile stamps the system's completion.
-template <bool kNz>
+template <bool kNz, bool kSmall>
 __global__ __launch_bounds__(512) void pair_batch8_kernel(PairKernelArgs a) {
     const int t = blockIdx.x;
     if (t == 0 && threadIdx.x == 0 && a.t0) *a.t0 = wall_clock64();
-    tile8_body<true, kNz>(a, t);
+    tile8_body<true, kNz, kSmall>(a, t);
     if (threadIdx.x == 0 && a.sys_count) {
         const int sys = a.tile_sys[t];
         __threadfence();
@@ -507,38 +697,61 @@ __global__ __launch_bounds__(512) void pair_batch8_kernel(PairKernelArgs a) {
 
 }  // namespace
 
-bool pair_tile8_supported(int R) {
-    // R <= 8 means nb >= 29: a lane's 16 rows of a block touch at most two DOFs.
-    const Lay8 L = lay8(R);
-    return R <= 8 && 2 * R * kGroups <= 512 && L.ntab >= 256 && L.total <= kLdsCap;
+// Small-bin tiles (nb <= 8) have 32 DOFs a side (8 rows each) and take the
+// tiles of a partial last block too; otherwise R <= 8, i.e. nb >= 29, so that
+// a lane's 16 rows of a block touch at most two DOFs.
+bool pair_tile8_small(int nb) { return nb >= 1 && nb <= 8; }
+
+bool pair_tile8_supported(int R, int nb) {
+    const bool small = pair_tile8_small(nb);
+    const Lay8 L = lay8(R, small, true);
+    if (L.ntab < 256 || L.total > kLdsCap) return false;
+    return small ? R == 32 : (R <= 8 && 2 * R * kGroups <= 512);
 }
 
 hipError_t launch_pair_tiles8(const PairKernelArgs& a, int n_tiles, hipStream_t st) {
     if (n_tiles <= 0) return hipSuccess;
-    if (!pair_tile8_supported(a.R)) return hipErrorInvalidValue;
-    const size_t lds = lay8(a.R).total;
+    if (!pair_tile8_supported(a.R, a.nb)) return hipErrorInvalidValue;
+    const bool small = pair_tile8_small(a.nb);
     // nonzero bins are counted only for the Miller-Madow estimator
     const bool nz = a.slot_mm >= 0 || a.mi_from_mm;
-    const void* const fn = nz ? reinterpret_cast<const void*>(&pair_tile8_kernel<true>)
-                              : reinterpret_cast<const void*>(&pair_tile8_kernel<false>);
+    const size_t lds = lay8(a.R, small, nz).total;
+    const void* fn;
+    if (small) fn = nz ? reinterpret_cast<const void*>(&pair_tile8_kernel<true, true>)
+                       : reinterpret_cast<const void*>(&pair_tile8_kernel<false, true>);
+    else fn = nz ? reinterpret_cast<const void*>(&pair_tile8_kernel<true, false>)
+                 : reinterpret_cast<const void*>(&pair_tile8_kernel<false, false>);
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    if (nz) hipLaunchKernelGGL(pair_tile8_kernel<true>, dim3(n_tiles), dim3(512), lds, st, a);
-    else hipLaunchKernelGGL(pair_tile8_kernel<false>, dim3(n_tiles), dim3(512), lds, st, a);
+    if (small) {
+        if (nz) hipLaunchKernelGGL((pair_tile8_kernel<true, true>), dim3(n_tiles), dim3(512), lds, st, a);
+        else hipLaunchKernelGGL((pair_tile8_kernel<false, true>), dim3(n_tiles), dim3(512), lds, st, a);
+    } else {
+        if (nz) hipLaunchKernelGGL((pair_tile8_kernel<true, false>), dim3(n_tiles), dim3(512), lds, st, a);
+        else hipLaunchKernelGGL((pair_tile8_kernel<false, false>), dim3(n_tiles), dim3(512), lds, st, a);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_pair_batch8(const PairKernelArgs& a, hipStream_t st) {
     if (a.n_work <= 0) return hipSuccess;
-    if (!pair_tile8_supported(a.R)) return hipErrorInvalidValue;
-    const size_t lds = lay8(a.R).total;
+    if (!pair_tile8_supported(a.R, a.nb)) return hipErrorInvalidValue;
+    const bool small = pair_tile8_small(a.nb);
     const bool nz = a.slot_mm >= 0 || a.mi_from_mm;
-    const void* const fn = nz ? reinterpret_cast<const void*>(&pair_batch8_kernel<true>)
-                              : reinterpret_cast<const void*>(&pair_batch8_kernel<false>);
+    const size_t lds = lay8(a.R, small, nz).total;
+    const void* fn;
+    if (small) fn = nz ? reinterpret_cast<const void*>(&pair_batch8_kernel<true, true>)
+                       : reinterpret_cast<const void*>(&pair_batch8_kernel<false, true>);
+    else fn = nz ? reinterpret_cast<const void*>(&pair_batch8_kernel<true, false>)
+                 : reinterpret_cast<const void*>(&pair_batch8_kernel<false, false>);
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    if (nz) hipLaunchKernelGGL(pair_batch8_kernel<true>, dim3(a.n_work), dim3(512), lds, st, a);
-    else hipLaunchKernelGGL(pair_batch8_kernel<false>, dim3(a.n_work), dim3(512), lds, st, a);
+    if (small) {
+        if (nz) hipLaunchKernelGGL((pair_batch8_kernel<true, true>), dim3(a.n_work), dim3(512), lds, st, a);
+        else hipLaunchKernelGGL((pair_batch8_kernel<false, true>), dim3(a.n_work), dim3(512), lds, st, a);
+    } else {
+        if (nz) hipLaunchKernelGGL((pair_batch8_kernel<true, false>), dim3(a.n_work), dim3(512), lds, st, a);
+        else hipLaunchKernelGGL((pair_batch8_kernel<false, false>), dim3(a.n_work), dim3(512), lds, st, a);
+    }
     return hipGetLastError();
 }
-

@@ -158,6 +158,36 @@ def test_ragged_shapes_vs_oracle(ctx, nb, D, F, S):
 
 
 @pytest.mark.parametrize("ests", [(0, 1), (0,)])
+@pytest.mark.parametrize("nb,D,F,S", [(5, 70, 3001, 3), (8, 64, 2048, 2), (2, 33, 999, 1),
+                                      (3, 100, 5000, 5), (7, 40, 130, 2), (1, 35, 700, 2)])
+def test_small_bin_tiles_vs_oracle(ctx, nb, D, F, S, ests):
+    """nb <= 8 runs every tile, those of a partial last block included, on
+    the eight-wave small-bin kernel (32 DOFs a side, 8 one-hot rows per DOF
+    built by byte permutes); D past and below a block multiple, steps of
+    partial chunks, ML alone and with MM, all pairs and explicit records."""
+    rng = np.random.default_rng(nb * 11 + D)
+    codes = rng.integers(0, nb, size=(D, F), dtype=np.uint8)
+    codes[1::3] = (codes[0::3][: len(codes[1::3])] + rng.integers(0, 2, size=F)) % nb
+    codes[2::7] = 0                                                   # one bin
+    types = rng.integers(1, 4, size=D).astype(np.uint8)
+    ext = np.zeros((D, 4))
+    ext[:, 0] = rng.uniform(0.5, 1.0, D)
+    ext[:, 1] = ext[:, 0] + rng.uniform(0.2, 1.5, D)
+    iu, ju = np.triu_indices(D, 1)
+    print(f"small-bin tiles nb={nb} D={D} F={F} S={S} ests={ests}", flush=True)
+    s1, s2, mi = ctx.pair_entropy(codes, types, ext, nb, S, ests=ests)
+    w2 = O.pair_entropies(codes, types, ext, np.stack([iu, ju], 1), S, [nb], ests)
+    assert rel_err(s2, w2[:, :, 0, :]) < REL
+    w1 = O.dof_entropies(codes, types, ext, S, [nb], ests)
+    assert rel_err(mi, w1[iu, 0, 0, :] + w1[ju, 0, 0, :] - w2[:, 0, 0, :]) < 1e-9
+    pairs = np.stack([iu, ju], 1)[rng.permutation(len(iu))[:80]]
+    pairs[::2] = pairs[::2, ::-1]
+    s1, s2, mi = ctx.pair_entropy(codes, types, ext, nb, S, pairs=pairs, ests=ests[::-1])
+    w2 = O.pair_entropies(codes, types, ext, pairs, S, [nb], ests[::-1])
+    assert rel_err(s2, w2[:, :, 0, :]) < REL
+
+
+@pytest.mark.parametrize("ests", [(0, 1), (0,)])
 @pytest.mark.parametrize("nb,D,F,S", [(50, 20, 4000, 4), (42, 18, 2500, 5), (36, 14, 3333, 3),
                                       (50, 23, 6002, 2), (29, 17, 1500, 2), (31, 16, 2000, 3),
                                       (33, 15, 1000, 2), (40, 13, 1200, 2), (45, 11, 1100, 2),
