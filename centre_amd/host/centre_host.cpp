@@ -109,21 +109,38 @@ std::vector<std::vector<std::string>> records_of(const std::string& path) {
 
 using NeighMap = std::map<uint32_t, std::vector<uint32_t>>;   // sorted keys
 
-// record order of one pair kind (EntropyCalculator.cpp:406-473)
+// Record order of one pair kind.  Rows: blocks of `block` neighbour keys.
+// Columns: blocks of block * procs subset members; each is split among the
+// MPI ranks of the build being emulated, ranks 1.. first (ceil(bc / procs)
+// columns each when the block is partial), rank 0 (the master) last, and a
+// rank's records are its rows in key order, each row's neighbours in list
+// order, kept when inside its column range.  procs = 1 is centre_OMP
+// (EntropyCalculator.cpp:406-473, 827-890); procs > 1 is centre_OMPMPI
+// (EntropyCalculator_mpi.cpp:815-1110, 1952-2100), whose order the stored
+// example runs carry (two ranks).
 std::vector<std::pair<uint32_t, uint32_t>> record_order(const NeighMap& nm,
                                                         const std::vector<uint32_t>& cols,
-                                                        unsigned block) {
+                                                        unsigned block, unsigned procs) {
     std::vector<uint32_t> keys;
     for (auto& kv : nm) keys.push_back(kv.first);
     std::vector<std::pair<uint32_t, uint32_t>> out;
+    const size_t cached = (size_t)block * procs;
     for (size_t bi = 0; bi < keys.size(); bi += block) {
         const size_t br = std::min<size_t>(block, keys.size() - bi);
-        for (size_t bj = 0; bj < cols.size(); bj += block) {
-            const size_t bc = std::min<size_t>(block, cols.size() - bj);
-            const uint32_t lo = cols[bj], hi = cols[bj + bc - 1];
-            for (size_t r = bi; r < bi + br; ++r)
-                for (uint32_t cj : nm.at(keys[r]))
-                    if (cj >= lo && cj <= hi) out.emplace_back(keys[r], cj);
+        for (size_t bj = 0; bj < cols.size(); bj += cached) {
+            const size_t bc = std::min<size_t>(cached, cols.size() - bj);
+            size_t per = block, ranks = procs;
+            if (bc < cached) {
+                per = (bc + procs - 1) / procs;
+                ranks = (bc + per - 1) / per;
+            }
+            for (size_t p = 1; p <= ranks; ++p) {
+                const size_t c0 = bj + (p - 1) * per, c1 = p < ranks ? bj + p * per : bj + bc;
+                const uint32_t lo = cols[c0], hi = cols[c1 - 1];
+                for (size_t r = bi; r < bi + br; ++r)
+                    for (uint32_t cj : nm.at(keys[r]))
+                        if (cj >= lo && cj <= hi) out.emplace_back(keys[r], cj);
+            }
         }
     }
     return out;
@@ -164,10 +181,15 @@ void write_bin_file(const std::string& path, int type, const std::vector<uint8_t
     w.write(path);
 }
 
+// Entropy-contribution file (Netcdf_EntContri.cpp:577-760).  The reference
+// creates it NC_NETCDF4 with contri_S chunked (16, estimators, schemes,
+// steps) and id chunked (1, order) (netcdf-c's default for a record
+// variable); cdf5 = true writes the same dims, vars and attrs as CDF-5.
 void write_entcontri(const std::string& path, const std::string& content, int order,
                      bool upper, const std::vector<uint64_t>& dim_lens,
                      const std::vector<uint8_t>& schemes, int nsteps, int nest,
-                     const std::vector<uint32_t>& ids, const std::vector<double>& contri) {
+                     const std::vector<uint32_t>& ids, const std::vector<double>& contri,
+                     bool netcdf4) {
     nc::Writer w(5);
     const uint64_t nrec = ids.size() / order;
     const int dr = w.add_dim("records", nrec, true);
@@ -193,7 +215,13 @@ void write_entcontri(const std::string& path, const std::string& content, int or
     w.add_attr("binSchemes", nc::Attr::of<uint8_t>(nc::UBYTE, schemes));
     w.add_var("id", nc::UINT, {dr, di}, ids.data());
     w.add_var("contri_S", nc::DOUBLE, {dr, de, db, dn}, contri.data());
-    w.write(path);
+    if (!netcdf4) {
+        w.write(path);
+        return;
+    }
+    w.set_chunks("id", {1, (uint64_t)order});
+    w.set_chunks("contri_S", {16, (uint64_t)nest, schemes.size(), (uint64_t)nsteps});
+    ncf::write_netcdf4(w, path);
 }
 
 // Bin mids as binData1D / binData2D accumulate them (Estimators.cpp:67-77,
@@ -507,7 +535,8 @@ void Host::run_entropy() {
         std::map<uint32_t, uint32_t> ra, rb;
         for (size_t i = 0; i < td[KINDS[k].a].subset.size(); ++i) ra[td[KINDS[k].a].subset[i]] = (uint32_t)i;
         for (size_t i = 0; i < td[KINDS[k].b].subset.size(); ++i) rb[td[KINDS[k].b].subset[i]] = (uint32_t)i;
-        for (auto& pr : record_order(nm[k], td[KINDS[k].b].subset, block)) {
+        for (auto& pr : record_order(nm[k], td[KINDS[k].b].subset, block,
+                                     (unsigned)std::max(1, in.procs_for_order))) {
             if (!ra.count(pr.first) || !rb.count(pr.second)) continue;
             ids2[k].push_back(pr.first);
             ids2[k].push_back(pr.second);
@@ -638,14 +667,14 @@ void Host::run_entropy() {
         std::vector<double> c(s1.begin() + (size_t)base[ty] * nest * nsch * S,
                               s1.begin() + (size_t)(base[ty] + n) * nest * nsch * S);
         write_entcontri(out(std::string("entcontri_") + ONE_TAG[ty] + "-1d.nc"), ONE_CONTENT[ty], 1,
-                        false, {n}, schemes, S, nest, td[ty].subset, c);
+                        false, {n}, schemes, S, nest, td[ty].subset, c, in.entcontri_netcdf4);
     }
     for (int k = 0; k < 6; ++k) {
         if (!(in.workset & KINDS[k].set)) continue;
         const bool same = KINDS[k].a == KINDS[k].b;
         write_entcontri(out(std::string("entcontri_") + KINDS[k].tag + "-2d.nc"), KINDS[k].content, 2,
                         same, {td[KINDS[k].a].subset.size(), td[KINDS[k].b].subset.size()}, schemes,
-                        S, nest, ids2[k], s2[k]);
+                        S, nest, ids2[k], s2[k], in.entcontri_netcdf4);
     }
     // count files (EntropyCalculator.cpp:61-92, 173-214, 362-372, 575-617,
     // 786-795): 1-D FULL, same-type 2-D FULL, cross-type 2-D UPPER
@@ -838,7 +867,27 @@ void usage() {
                  "  print the format, global text attributes and, per variable,\n"
                  "  its shape and the sum of its values (no GPU is used)\n"
                  "       centre_hip --print-config <input.cfg> [frames in the trajectory]\n"
-                 "  print the parsed input parameters (no GPU is used)\n";
+                 "  print the parsed input parameters (no GPU is used)\n"
+                 "       centre_hip --convert <entcontri.nc> <out.nc> [netcdf4|cdf5]\n"
+                 "  rewrite an entropy-contribution file of either format in the\n"
+                 "  layout the entropy stage writes (default netcdf4; no GPU is used)\n";
+}
+
+int convert(const std::string& src, const std::string& dst, const std::string& fmt) {
+    if (fmt != "netcdf4" && fmt != "cdf5") throw Fail("format must be netcdf4 or cdf5");
+    auto f = ncf::open(src);
+    if (f->text_attr("convention") != "EntropyCont" || !f->has_var("contri_S") || !f->has_var("id"))
+        throw Fail(src + ": not an entropy-contribution file");
+    const auto shp = f->shape("contri_S");   // records, estimators, schemes, steps
+    const auto order = f->uint_attr("order"), st = f->uint_attr("storagetype");
+    const auto dl = f->uint_attr("dimLengths"), bs = f->uint_attr("binSchemes");
+    if (shp.size() != 4 || order.size() != 1 || st.size() != 1 || bs.size() != shp[2])
+        throw Fail(src + ": attributes do not match contri_S");
+    const std::vector<uint8_t> schemes(bs.begin(), bs.end());
+    write_entcontri(dst, f->text_attr("content"), (int)order[0], st[0] == 3, dl, schemes,
+                    (int)shp[3], (int)shp[1], f->read_u32("id"), f->read_f64("contri_S"),
+                    fmt == "netcdf4");
+    return 0;
 }
 
 int inspect(int argc, char** argv, int first) {
@@ -887,7 +936,8 @@ int print_config(const char* path, uint64_t frames_in_file) {
     for (auto e : in.estimators) std::cout << " " << EST_NAME[e];
     std::cout << "\nmethod " << (int)in.method << "\nkappa " << in.kappa << "\nnmaxconf "
               << in.nmaxconf << "\ncacheentcdimspercpu " << in.cacheentcdimspercpu
-              << "\nstartframe " << in.startframe << "\nstrideframe " << in.strideframe
+              << "\nthreadsfororder " << in.threads_for_order << "\nprocsfororder "
+              << in.procs_for_order << "\nstartframe " << in.startframe << "\nstrideframe " << in.strideframe
               << "\nnframe " << in.nframe << "\n";
     return in.ok() ? 0 : 1;
 }
@@ -901,6 +951,8 @@ int main(int argc, char** argv) {
         if (argc >= 3 && !std::strcmp(argv[1], "--inspect")) return inspect(argc, argv, 2);
         if (argc >= 3 && !std::strcmp(argv[1], "--print-config"))
             return print_config(argv[2], argc >= 4 ? std::strtoull(argv[3], nullptr, 10) : 0);
+        if (argc >= 4 && !std::strcmp(argv[1], "--convert"))
+            return convert(argv[2], argv[3], argc >= 5 ? argv[4] : "netcdf4");
     } catch (const std::exception& e) {
         std::cerr << "centre_hip: " << e.what() << "\n";
         return 2;
@@ -922,6 +974,9 @@ int main(int argc, char** argv) {
             for (auto& e : h.in.errors) std::cerr << "InputReadError:: " << e << "\n";
             return 1;
         }
+        if (h.in.calcentropy && h.in.entcontri_netcdf4 && !ncf::netcdf4_available())
+            throw Fail("NetCDF-4 output needs libhdf5 and libhdf5_hl (or set "
+                       "mi355x.entcontriformat = cdf5)");
         if (h.in.discretize) {
             const std::string batp = join(h.in.infilepath, h.in.fnames[0]);
             auto bat = ncf::open(batp);

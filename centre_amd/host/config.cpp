@@ -148,7 +148,17 @@ Inputs inputs_from_symbols(const Symbols& sym) {
     }
     // options of the MI355X host (no counterpart in the reference's cfg)
     i1("mi355x.threadsfororder", in.threads_for_order);
+    i1("mi355x.procsfororder", in.procs_for_order);
+    if (in.threads_for_order < 1) in.errors.push_back("mi355x.threadsfororder must be at least 1");
+    if (in.procs_for_order < 1) in.errors.push_back("mi355x.procsfororder must be at least 1");
     i1("mi355x.device", in.device);
+    if (auto v = get("mi355x.entcontriformat")) {
+        std::string m = (*v)[0];
+        std::transform(m.begin(), m.end(), m.begin(), ::tolower);
+        if (m == "netcdf4") in.entcontri_netcdf4 = true;
+        else if (m == "cdf5") in.entcontri_netcdf4 = false;
+        else in.errors.push_back("mi355x.entcontriformat must be netcdf4 or cdf5");
+    }
 
     if (in.nbins.empty() && in.pdfmethod == PdfMethod::Histogram)
         in.errors.push_back("histogram.nbins has no default and must be given");

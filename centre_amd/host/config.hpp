@@ -63,7 +63,9 @@ struct Inputs {
     std::vector<int> estimators;            // 0 ML, 1 MM, 2 CS, 3 JS
     // options of this build (no reference counterpart)
     int threads_for_order = 1;              // OpenMP threads the record order emulates
+    int procs_for_order = 1;                // MPI ranks the record order emulates (1: centre_OMP)
     int device = 0;
+    bool entcontri_netcdf4 = true;          // entcontri_* as NetCDF-4 (the reference's format) or CDF-5
 
     std::vector<std::string> errors;
     bool ok() const { return errors.empty(); }

@@ -297,6 +297,16 @@ struct Out {
 };
 }  // namespace
 
+void Writer::set_chunks(const std::string& var, const std::vector<uint64_t>& chunks) {
+    for (auto& v : vars_)
+        if (v.name == var) {
+            if (chunks.size() != v.dims.size()) throw Error("chunk rank mismatch for " + var);
+            v.chunks = chunks;
+            return;
+        }
+    throw Error("no variable " + var);
+}
+
 void Writer::write(const std::string& path) const {
     const uint64_t numrecs = unlimited_ >= 0 ? dims_[unlimited_].second : 0;
     std::vector<const WVar*> fixed, rec;

@@ -93,16 +93,26 @@ public:
     // data: native-endian bytes of the full variable
     void add_var(const std::string& name, int type, const std::vector<int>& dims,
                  const void* data, const std::map<std::string, Attr>& attrs = {});
+    // Chunk shape of a variable for the NetCDF-4 encoding (the classic
+    // formats have no chunks and ignore it).
+    void set_chunks(const std::string& var, const std::vector<uint64_t>& chunks);
     void write(const std::string& path) const;
 
-private:
     struct WVar {
         std::string name;
         int type;
         std::vector<int> dims;
         std::vector<uint8_t> data;
         std::vector<std::pair<std::string, Attr>> attrs;
+        std::vector<uint64_t> chunks;
     };
+    // The declared file, for other encoders of the same description.
+    const std::vector<std::pair<std::string, uint64_t>>& dims() const { return dims_; }
+    int unlimited() const { return unlimited_; }
+    const std::vector<std::pair<std::string, Attr>>& attrs() const { return gatts_; }
+    const std::vector<WVar>& vars() const { return vars_; }
+
+private:
     int version_;
     int unlimited_ = -1;
     std::vector<std::pair<std::string, uint64_t>> dims_;

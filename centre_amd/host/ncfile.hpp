@@ -15,6 +15,8 @@
 #include <string>
 #include <vector>
 
+#include "nc_classic.hpp"
+
 namespace ncf {
 
 class File {
@@ -32,10 +34,25 @@ public:
                                            std::vector<uint64_t> count = {}) const = 0;
     // Text of a global attribute, "" when absent or not text.
     virtual std::string text_attr(const std::string& name) const = 0;
+    // Values of an integer global attribute, empty when absent or not integer.
+    virtual std::vector<uint64_t> uint_attr(const std::string& name) const = 0;
     virtual const char* format() const = 0;
 };
 
 // Opens by content (magic bytes), not by file name.
 std::unique_ptr<File> open(const std::string& path);
+
+// Writes the file a classic-format Writer describes as NetCDF-4 (HDF5 with
+// netcdf-c's conventions: one dimension scale per dimension carrying
+// _Netcdf4Dimid, DIMENSION_LIST / REFERENCE_LIST links, _Netcdf4Coordinates
+// on variables of rank > 1, creation order tracked, NetCDF default fill
+// values, _NCProperties), the way the reference creates its
+// entropy-contribution files (Netcdf_EntContri.cpp:577-760, NC_NETCDF4).
+// Variables with the unlimited dimension are chunked (1, full extent, ...)
+// unless the Writer names their chunks.  Loads libhdf5 and libhdf5_hl at run
+// time; throws nc::Error when they are absent.
+void write_netcdf4(const nc::Writer& w, const std::string& path);
+// True when the libraries write_netcdf4 needs can be loaded.
+bool netcdf4_available();
 
 }  // namespace ncf

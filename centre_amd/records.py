@@ -84,21 +84,38 @@ def neighbors_from_subset(subset: Dict[int, Sequence[int]]) -> Dict[str, Dict[in
 
 
 def record_order(neigh: Dict[int, List[int]], col_members: Sequence[int],
-                 block: int) -> List[Tuple[int, int]]:
-    """(row id, column id) records in the order the reference writes them."""
-    if block < 1:
-        raise ValueError("block must be >= 1")
+                 block: int, procs: int = 1) -> List[Tuple[int, int]]:
+    """(row id, column id) records in the order the reference writes them.
+
+    Rows come in blocks of ``block`` keys; columns in blocks of
+    ``block * procs`` members, each split among ``procs`` MPI ranks: ranks
+    1.. first (``ceil(bc / procs)`` columns each when the block is partial),
+    the master last.  ``procs = 1`` is centre_OMP
+    (EntropyCalculator.cpp:406-473); ``procs > 1`` is centre_OMPMPI
+    (EntropyCalculator_mpi.cpp:815-1110), the build that wrote the stored
+    example outputs (two ranks)."""
+    if block < 1 or procs < 1:
+        raise ValueError("block and procs must be >= 1")
     keys = list(neigh.keys())
     cols = list(col_members)
+    cached = block * procs
     out: List[Tuple[int, int]] = []
     for bi in range(0, len(keys), block):
         rows = keys[bi:bi + block]
-        for bj in range(0, len(cols), block):
-            lo, hi = cols[bj], cols[min(bj + block, len(cols)) - 1]
-            for r in rows:
-                for cj in neigh[r]:
-                    if lo <= cj <= hi:
-                        out.append((r, cj))
+        for bj in range(0, len(cols), cached):
+            bc = min(cached, len(cols) - bj)
+            per, ranks = block, procs
+            if bc < cached:
+                per = -(-bc // procs)
+                ranks = -(-bc // per)
+            for p in range(1, ranks + 1):
+                c0 = bj + (p - 1) * per
+                c1 = bj + p * per if p < ranks else bj + bc
+                lo, hi = cols[c0], cols[c1 - 1]
+                for r in rows:
+                    for cj in neigh[r]:
+                        if lo <= cj <= hi:
+                            out.append((r, cj))
     return out
 
 

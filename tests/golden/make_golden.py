@@ -13,8 +13,9 @@ and outputs for ``test_examples/example-{1,2}``:
 * the convergence CSVs, the MIST tree CSVs and ``vmkde_torsions_minimas.out``;
 * one stored NetCDF-4 file copied as is (``ex1h_entcontri_ang-1d.nc``, 18 KB),
   the format fixture of the host's NetCDF-4 reader;
-* the header of example-2's ``hist_tor-1d.nc`` as JSON
-  (``ex2h_hist_tor-1d_header.json``), the format fixture of the count files.
+* the header of example-2's ``hist_tor-1d.nc`` and of example-1's
+  ``bin_bonds.nc`` as JSON, and the h5dump headers of example-1's
+  ``entcontri_{ang,ba}-2d.nc``: the format fixtures of the host's writers.
 
 Usage:  python tests/golden/make_golden.py [/root/reference]
 """
@@ -136,13 +137,12 @@ def main(ref: str) -> None:
           f"{len(out)} arrays)")
 
 
-def write_hist_header(ref: str) -> None:
-    """Header of example-2's stored hist_tor-1d.nc (dimension order and
-    sizes, attribute order, types and values, variable dimensions and types),
-    the format fixture of the host's count-file writer."""
+def write_classic_header(src: str, dst: str) -> None:
+    """Header of a stored classic-format file (dimension order and sizes,
+    attribute order, types and values, variable dimensions and types) as
+    JSON."""
     import json
-    ds = open_dataset(os.path.join(ref, "test_examples", "example-2", "outfiles-h-ref",
-                                   "hist_tor-1d.nc"))
+    ds = open_dataset(src)
     attrs = []
     for k, v in ds.attrs.items():
         if isinstance(v, str):
@@ -153,11 +153,34 @@ def write_hist_header(ref: str) -> None:
     hdr = {"version": ds.version, "unlimited": ds.unlimited,
            "dims": [[k, n] for k, n in ds.dims.items()], "attrs": attrs,
            "vars": [[v.name, list(v.dims), v.nc_type] for v in ds.variables.values()]}
-    with open(os.path.join(HERE, "ex2h_hist_tor-1d_header.json"), "w") as f:
+    with open(dst, "w") as f:
         json.dump(hdr, f, indent=1)
         f.write("\n")
 
 
+def write_headers(ref: str) -> None:
+    """Format fixtures of the host's writers:
+
+    * example-2's ``hist_tor-1d.nc`` (CDF-5 count file) and example-1's
+      ``bin_bonds.nc`` (CDF-5 bin-code file) headers as JSON;
+    * the h5dump headers (creation order, storage properties, attribute
+      values; object addresses and the _NCProperties text taken out) of
+      example-1's ``entcontri_ang-2d.nc`` (same-type pairs, UPPER) and
+      ``entcontri_ba-2d.nc`` (cross-type pairs), both NetCDF-4.
+    """
+    sys.path.insert(0, os.path.dirname(HERE))
+    import h5read
+    ex1 = os.path.join(ref, "test_examples", "example-1", "outfiles-h-ref")
+    write_classic_header(os.path.join(ref, "test_examples", "example-2", "outfiles-h-ref",
+                                      "hist_tor-1d.nc"),
+                         os.path.join(HERE, "ex2h_hist_tor-1d_header.json"))
+    write_classic_header(os.path.join(ex1, "bin_bonds.nc"),
+                         os.path.join(HERE, "ex1h_bin_bonds_header.json"))
+    for tag in ("ang", "ba"):
+        with open(os.path.join(HERE, f"ex1h_entcontri_{tag}-2d_header.txt"), "w") as f:
+            f.write(h5read.header(os.path.join(ex1, f"entcontri_{tag}-2d.nc")) + "\n")
+
+
 if __name__ == "__main__":
     main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
-    write_hist_header(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
+    write_headers(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")

@@ -142,3 +142,105 @@ def test_zero_frame_stride_is_refused(tmp_path):
     rc, out, _ = run("--print-config", str(cfg), "1000")
     assert rc == 1
     assert "error entropy.strideframe must be at least 1" in out
+
+
+# ---- NetCDF-4 entropy-contribution writer (no GPU: the host's --convert) ----
+
+import h5read  # noqa: E402
+
+H5OK = h5read.lib() is not None and os.path.exists(h5read.H5DUMP)
+needs_h5 = pytest.mark.skipif(not H5OK, reason="libhdf5 or h5dump absent")
+
+
+@needs_h5
+def test_netcdf4_writer_reproduces_the_reference_file(tmp_path):
+    """The reference's stored example-1 entcontri_ang-1d.nc (NetCDF-4 from
+    netcdf-c), rewritten by the host's NetCDF-4 writer: the h5dump header
+    (creation order of every object and attribute, dimension scales and
+    their links, chunking, fill values, attribute types and values) is the
+    same apart from the _NCProperties provenance text, and so is the data."""
+    dst = str(tmp_path / "a.nc")
+    rc, _, err = run("--convert", NC4, dst)
+    assert rc == 0, err
+    assert h5read.header(dst) == h5read.header(NC4)
+    with h5read.File(NC4) as a, h5read.File(dst) as b:
+        for v in ("id", "contri_S"):
+            np.testing.assert_array_equal(a.read(v), b.read(v))
+        assert b.attr("_NCProperties").startswith("version=2,")
+
+
+def entcontri_cdf5(path, ids, S, content, upper, dim_lens):
+    """An entropy-contribution file in CDF-5 with the reference's dims,
+    attribute order and variables (one estimator, one 30-bin scheme)."""
+    n, steps = S.shape
+    cdf.write_dataset(
+        path,
+        [("records", n), ("estimators", 1), ("nbin_schemes", 1), ("nsteps", steps),
+         ("ids", ids.shape[1])],
+        [cdf.VarSpec("id", ("records", "ids"), ids.astype(np.uint32)),
+         cdf.VarSpec("contri_S", ("records", "estimators", "nbin_schemes", "nsteps"),
+                     S[:, None, None, :])],
+        attrs={"title": np.array([1 if upper else 0], np.uint8), "application": "CENTRE",
+               "program": "Real2Int", "programVersion": "1.0.0", "convention": "EntropyCont",
+               "conventionVersion": "1.0", "content": content,
+               "order": np.array([ids.shape[1]], np.uint8), "nsteps": np.array([steps], np.uint8),
+               "storagetype": np.array([3 if upper else 0], np.uint8),
+               "ndims": np.array([ids.shape[1]], np.uint8),
+               "dimLengths": np.array(dim_lens, np.uint64),
+               "nbinSchemes": np.array([1], np.uint8), "binSchemes": np.array([30], np.uint8)},
+        version=5, unlimited="records")
+
+
+@needs_h5
+@pytest.mark.parametrize("tag,content,upper,lens", [("ang", "A/A", True, ("ang", "ang")),
+                                                     ("ba", "B/A", False, ("bnd", "ang"))])
+def test_netcdf4_writer_2d_layout_matches_reference(tmp_path, golden, tag, content, upper, lens):
+    """Example-1's 2-D records (same-type UPPER and cross-type FULL) written
+    through the host's NetCDF-4 writer give the h5dump header of the
+    reference's stored entcontri_{ang,ba}-2d.nc, and the same values."""
+    ids, S = golden[f"ex1_id_{tag}2d"], golden[f"ex1_S_{tag}2d"]
+    dims = [len(golden[f"ex1_id_{t}1d"]) for t in lens]
+    src, dst = str(tmp_path / "in.nc"), str(tmp_path / "out.nc")
+    entcontri_cdf5(src, ids, S, content, upper, dims)
+    rc, _, err = run("--convert", src, dst, "netcdf4")
+    assert rc == 0, err
+    with open(os.path.join(ROOT, "tests", "golden", f"ex1h_entcontri_{tag}-2d_header.txt")) as f:
+        assert h5read.header(dst) == f.read().rstrip("\n")
+    with h5read.File(dst) as b:
+        np.testing.assert_array_equal(b.read("id"), ids)
+        np.testing.assert_array_equal(b.read("contri_S")[:, 0, 0, :], S)
+
+
+@needs_h5
+def test_cdf5_output_of_a_netcdf4_file(tmp_path):
+    """mi355x.entcontriformat = cdf5 keeps the same dims, attributes and
+    values in the classic format."""
+    dst = str(tmp_path / "a.nc")
+    rc, _, err = run("--convert", NC4, dst, "cdf5")
+    assert rc == 0, err
+    ds = cdf.open_dataset(dst)
+    assert ds.version == 5 and ds.unlimited == "records"
+    assert list(ds.dims) == ["records", "estimators", "nbin_schemes", "nsteps", "ids"]
+    assert list(ds.attrs)[:3] == ["title", "application", "program"]
+    with h5read.File(NC4) as a:
+        np.testing.assert_array_equal(ds.read("contri_S"), a.read("contri_S"))
+        np.testing.assert_array_equal(ds.read("id"), a.read("id"))
+        for k in ("order", "storagetype", "dimLengths", "binSchemes", "nsteps"):
+            np.testing.assert_array_equal(np.asarray(ds.attrs[k]), a.attr(k), err_msg=k)
+
+
+def test_convert_refuses_other_files(tmp_path):
+    p = tmp_path / "b.nc"
+    cdf.write_dataset(str(p), [("dims", 2)], [cdf.VarSpec("dimid", ("dims",), np.arange(2, dtype=np.uint32))],
+                      attrs={"convention": "CENTRE_Real"}, version=5)
+    rc, _, err = run("--convert", str(p), str(tmp_path / "o.nc"))
+    assert rc == 2 and "not an entropy-contribution file" in err
+
+
+def test_entcontri_format_option(tmp_path):
+    cfg = tmp_path / "in.cfg"
+    base = ("[control]\n nsteps = 3\n[discretization]\n fname = a.nc\n"
+            "[histogram]\n referencenbins = 30\n nbins = 30\n[mi355x]\n")
+    cfg.write_text(base + " entcontriformat = hdf9\n")
+    rc, out, _ = run("--print-config", str(cfg))
+    assert rc == 1 and "error mi355x.entcontriformat must be netcdf4 or cdf5" in out

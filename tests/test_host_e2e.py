@@ -20,6 +20,8 @@ import pytest
 import oracle as O
 from centre_amd import cdf, hip
 
+import h5read  # noqa: E402
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HOST = os.path.join(ROOT, "centre_amd", "centre_hip")
 TYPES = (("b", "bin_bonds.nc", "B", "bond", "bnd"), ("a", "bin_angles.nc", "A", "angle", "ang"),
@@ -89,6 +91,8 @@ def write_inputs(d, g, hist=""):
     neighborfile         = neigh.txt
     scoringmethod        = MIST
     estimator            = ML
+[mi355x]
+    procsfororder        = 2
 """)
     return cfg, out
 
@@ -107,25 +111,30 @@ def test_host_example1_entropy_and_scoring(tmp_path, golden):
     r = subprocess.run([HOST, "-i", cfg], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
 
-    # entropy contributions: same records, same values
+    # entropy contributions: NetCDF-4 as the reference writes them, same
+    # layout (the reference's stored headers), same records, same values
+    for tag in ("ang", "ba"):
+        p = os.path.join(out, f"entcontri_{tag}-2d.nc")
+        assert h5read.is_hdf5(p), p
+        got = h5read.header(p)
+        if got is not None:
+            with open(os.path.join(ROOT, "tests", "golden", f"ex1h_entcontri_{tag}-2d_header.txt")) as f:
+                assert got == f.read().rstrip("\n"), f"{tag}-2d header differs from the reference's"
     for tag in ("bnd", "ang", "tor"):
-        ds = cdf.open_dataset(os.path.join(out, f"entcontri_{tag}-1d.nc"))
+        ds = h5read.open_any(os.path.join(out, f"entcontri_{tag}-1d.nc"))
         assert ds.attrs["convention"] == "EntropyCont" and ds.attrs["order"][0] == 1
         ids, S = ds.read("id")[:, 0], ds.read("contri_S")[:, 0, 0, :]
         np.testing.assert_array_equal(ids, golden[f"ex1_id_{tag}1d"][:, 0])
         np.testing.assert_allclose(S, golden[f"ex1_S_{tag}1d"], rtol=1e-10, atol=1e-12)
     for tag in KIND_WORD:
-        ds = cdf.open_dataset(os.path.join(out, f"entcontri_{tag}-2d.nc"))
+        ds = h5read.open_any(os.path.join(out, f"entcontri_{tag}-2d.nc"))
         same = tag in ("bnd", "ang", "tor")
         assert ds.attrs["storagetype"][0] == (3 if same else 0)
         assert ds.attrs["title"][0] == (1 if same else 0)
         ids, S = ds.read("id"), ds.read("contri_S")[:, 0, 0, :]
-        want = {(int(i), int(j)): v for (i, j), v in zip(golden[f"ex1_id_{tag}2d"], golden[f"ex1_S_{tag}2d"])}
-        got = {(int(i), int(j)): v for (i, j), v in zip(ids, S)}
-        assert got.keys() == want.keys(), tag
-        a = np.array([got[k] for k in want])
-        b = np.array([want[k] for k in want])
-        np.testing.assert_allclose(a, b, rtol=1e-10, atol=1e-12, err_msg=tag)
+        # the stored files' own record order (the MPI build's, two ranks)
+        np.testing.assert_array_equal(ids, golden[f"ex1_id_{tag}2d"], err_msg=tag)
+        np.testing.assert_allclose(S, golden[f"ex1_S_{tag}2d"], rtol=1e-10, atol=1e-12, err_msg=tag)
 
     # convergence tables (4 decimals as printed)
     for meth in ("MIE", "MIST"):
@@ -211,7 +220,7 @@ def test_host_example1_count_files(tmp_path, golden):
         assert ds.attrs["content"] == content
         assert list(ds.attrs["storagetype"]) == [0 if x == y else 3]
         assert dict(ds.dims)["bin_pow_dims_sum"] == 900 and dict(ds.dims)["dimsXbin_sum"] == 60
-        ent = cdf.open_dataset(os.path.join(out, f"entcontri_{tag}-2d.nc"))
+        ent = h5read.open_any(os.path.join(out, f"entcontri_{tag}-2d.nc"))
         ids = ds.read("id")
         np.testing.assert_array_equal(ids, ent.read("id"))
         ex = ds.read("extrema")
@@ -273,26 +282,137 @@ def test_host_all_pairs_from_bat_file(tmp_path):
     r = subprocess.run([HOST, "-i", cfg], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     codes, ext = {}, {}
-    for t, fname, _, _, _ in TYPES:
+    # bin files: the layout of the reference's stored example-1 bin_bonds.nc
+    # (CDF-5; dims, attribute order and types, fixed texts, variables)
+    with open(os.path.join(ROOT, "tests", "golden", "ex1h_bin_bonds_header.json")) as f:
+        ref = json.load(f)
+    sizes = {"b": nbd, "a": nan, "t": ndh}
+    for t, fname, content, _, _ in TYPES:
         ds = cdf.open_dataset(os.path.join(out, fname))
+        assert ds.version == ref["version"] and ds.unlimited == ref["unlimited"]
+        assert list(ds.dims) == [k for k, _ in ref["dims"]]
+        assert dict(ds.dims) == {"dims": sizes[t], "extremes": 4, "bin_scheme_count": 1, "frames": F}
+        assert list(ds.attrs) == [a[0] for a in ref["attrs"]]
+        for k, typ, val in ref["attrs"]:
+            got = ds.attrs[k]
+            if typ == "text":
+                want = {"title": f"{content} Integer Vectors", "content": content}.get(k, val)
+                assert got == want, k
+            else:
+                assert np.asarray(got).dtype.name == typ, k
+        assert list(ds.attrs["ncoordims"]) == [sizes[t]] and list(ds.attrs["binSchemes"]) == [nb]
+        assert [(v.name, list(v.dims), v.nc_type) for v in ds.variables.values()] == \
+            [(v[0], v[1], v[2]) for v in ref["vars"]]
         codes[t] = ds.read("coord")[:, 0, :]
         ext[t] = ds.read("extreme")
     tnum = {"b": 1, "a": 2, "t": 3}
     for tag, t in (("bnd", "b"), ("ang", "a"), ("tor", "t")):
-        ds = cdf.open_dataset(os.path.join(out, f"entcontri_{tag}-1d.nc"))
+        ds = h5read.open_any(os.path.join(out, f"entcontri_{tag}-1d.nc"))
         want = O.dof_entropies(codes[t], np.full(len(codes[t]), tnum[t], np.uint8), ext[t], S,
                                [nb], (0, 1))
         np.testing.assert_allclose(ds.read("contri_S")[:, :, 0, :], want[:, :, 0, :],
                                    rtol=1e-10, atol=1e-12)
     kinds = {"bnd": "bb", "ang": "aa", "tor": "tt", "ba": "ba", "bd": "bt", "ad": "at"}
     for tag, (x, y) in kinds.items():
-        ds = cdf.open_dataset(os.path.join(out, f"entcontri_{tag}-2d.nc"))
+        ds = h5read.open_any(os.path.join(out, f"entcontri_{tag}-2d.nc"))
         ids = ds.read("id").astype(np.int64)
         n_x, n_y = len(codes[x]), len(codes[y])
         want_n = n_x * (n_x - 1) // 2 if x == y else n_x * n_y
         assert len(ids) == want_n, tag
         both = np.concatenate([codes[x], codes[y]])
         types = np.array([tnum[x]] * n_x + [tnum[y]] * n_y, np.uint8)
+        e = np.concatenate([ext[x], ext[y]])
+        pr = np.stack([ids[:, 0], ids[:, 1] + n_x], 1)
+        want = O.pair_entropies(both, types, e, pr, S, [nb], (0, 1))
+        np.testing.assert_allclose(ds.read("contri_S")[:, :, 0, :], want[:, :, 0, :],
+                                   rtol=1e-10, atol=1e-12, err_msg=tag)
+
+
+@pytest.mark.gpu
+def test_host_netcdf4_bat_shuffled_and_strided(tmp_path):
+    """The input side: a NetCDF-4 BAT file in the reference's layout
+    (Netcdf_BAT.cpp:440-670: float variables bond/angle/dihedral[frame][n],
+    frame unlimited, dimension scales named _nc4_non_coord_* where a variable
+    shares the dimension's name), discretised with frame shuffling
+    (main_smp.cpp:312-357: std::shuffle with mt19937(randseed + 1000), then
+    Real2Int::shuffleArray's swap loop, Real2Int.cpp:12-25) and a start /
+    stride / nframe entropy range (Netcdf_TrjInt.cpp:353-359).  The bin files
+    equal the oracle's codes of the same float32 columns passed through the
+    same swap loop, and every 1-D and 2-D contribution equals the oracle's
+    on the selected frames."""
+    nbd, nan, ndh, F, nb, S = 6, 5, 7, 5000, 30, 3
+    start, stride, nframe = 3, 2, 4501
+    bat = hip.synth_bat(0xCE27E + 91, nbd, nan, ndh, F).astype(np.float32)
+    d = str(tmp_path)
+    out = os.path.join(d, "out")
+    os.makedirs(out)
+    cols = {"bond": bat[:, :nbd], "angle": bat[:, nbd:nbd + nan], "dihedral": bat[:, nbd + nan:]}
+    h5read.write_netcdf4(
+        os.path.join(d, "bat.nc"),
+        [("tuple2", 2, False), ("pseudos", 1, False), ("frame", F, True), ("bond", nbd, False),
+         ("angle", nan, False), ("dihedral", ndh, False)],
+        [("pseudoBonds", ("pseudos", "tuple2"), np.zeros((1, 2), np.int32), None),
+         ("bond", ("frame", "bond"), cols["bond"], (4096, 4)),
+         ("angle", ("frame", "angle"), cols["angle"], None),
+         ("dihedral", ("frame", "dihedral"), cols["dihedral"], None)],
+        {"title": "synthetic", "application": "CENTRE", "program": "Cart2BAT",
+         "programVersion": "1.0", "convention": "CENTRE", "conventionVersion": "1.0"})
+    cfg = os.path.join(d, "centre.cfg")
+    with open(cfg, "w") as f:
+        f.write(f"""[control]
+    infilepath           = {d}/
+    outfilepath          = {out}/
+    nsteps               = {S}
+    discretize           = true
+    calcentropy          = true
+    genconvgdata         = false
+[discretization]
+    fname                = bat.nc
+    pdfmethod            = histogram
+    shuffleframes        = true
+    randseed             = 7
+[histogram]
+    referencenbins       = {nb}
+    nbins                = {nb}
+[entropy]
+    workset              = 2D
+    jacobian             = true
+    estimator            = ML, MM
+    startframe           = {start}
+    strideframe          = {stride}
+    nframe               = {nframe}
+""")
+    print("running host on a NetCDF-4 BAT file (shuffled, strided)", flush=True)
+    r = subprocess.run([HOST, "-i", cfg], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    idx = np.fromfile(os.path.join(out, "shuffle_indices.bin"), np.uint64)
+    assert np.array_equal(np.sort(idx), np.arange(F)) and not np.array_equal(idx, np.arange(F))
+    tnum = {"bond": 1, "angle": 2, "dihedral": 3}
+    codes, ext = {}, {}
+    for (var, x), (t, fname, _, _, _) in zip(cols.items(), TYPES):
+        ds = cdf.open_dataset(os.path.join(out, fname))
+        got = ds.read("coord")[:, 0, :]
+        want_c, want_e = [], []
+        for k in range(x.shape[1]):
+            c, e = O.real2int(x[:, k], tnum[var], [nb], nb, True)
+            want_c.append(O.shuffle(idx, c[0]))
+            want_e.append(e)
+        np.testing.assert_array_equal(got, np.array(want_c), err_msg=var)
+        np.testing.assert_array_equal(ds.read("extreme"), np.array(want_e), err_msg=var)
+        codes[t], ext[t] = got[:, start:nframe:stride], ds.read("extreme")
+    for tag, t in (("bnd", "b"), ("ang", "a"), ("tor", "t")):
+        ds = h5read.open_any(os.path.join(out, f"entcontri_{tag}-1d.nc"))
+        want = O.dof_entropies(np.ascontiguousarray(codes[t]),
+                               np.full(len(codes[t]), "bat".index(t) + 1, np.uint8), ext[t], S, [nb], (0, 1))
+        np.testing.assert_allclose(ds.read("contri_S")[:, :, 0, :], want[:, :, 0, :],
+                                   rtol=1e-10, atol=1e-12, err_msg=tag)
+    kinds = {"bnd": "bb", "ang": "aa", "tor": "tt", "ba": "ba", "bd": "bt", "ad": "at"}
+    for tag, (x, y) in kinds.items():
+        ds = h5read.open_any(os.path.join(out, f"entcontri_{tag}-2d.nc"))
+        ids = ds.read("id").astype(np.int64)
+        n_x = len(codes[x])
+        both = np.ascontiguousarray(np.concatenate([codes[x], codes[y]]))
+        types = np.array([" bat".index(x)] * n_x + [" bat".index(y)] * len(codes[y]), np.uint8)
         e = np.concatenate([ext[x], ext[y]])
         pr = np.stack([ids[:, 0], ids[:, 1] + n_x], 1)
         want = O.pair_entropies(both, types, e, pr, S, [nb], (0, 1))

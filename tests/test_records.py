@@ -1,17 +1,15 @@
 """Record order of the pair stage against the stored examples.
 
-The stored entcontri_*-2d.nc files list their records in the order
-EntropyCalculator writes them (blocks of cacheentcdimspercpu * threads / 2 = 5
-ids).  Rebuilding each row's neighbour list from the stored pairs (ascending,
-as the neighbour files list them) and regenerating the order gives the stored
-sequence back for every record whose column lies in a full column block.
-Near the end of some column ranges the stored files place block boundaries
-one column earlier than the source as written would (example-1 bonds and
-angles, example-2 angles): the column bounds there come from
-dimtypes_v[bj + bl_cols - 1] (EntropyCalculator.cpp:444-460), and the build
-that wrote the files evidently read them past the end of that vector.  The
-order is therefore checked exactly up to the first such record, and as a
-set, with the row-block grouping, for the rest."""
+The stored entcontri_*-2d.nc files list their records in the order the
+reference's MPI build (centre_OMPMPI, EntropyCalculator_mpi.cpp:815-1110)
+writes them with two ranks of one thread each: row blocks of
+cacheentcdimspercpu * threads / 2 = 5 neighbour keys, column blocks of 5 * 2
+subset members split between the ranks (rank 1 first, the master last; a
+partial last block split ceil(n / 2) : rest).  Rebuilding each row's
+neighbour list from the stored pairs (ascending, as the neighbour files list
+them) and regenerating the order with two ranks gives every stored file back
+exactly; the single-process order of centre_OMP (EntropyCalculator.cpp:
+406-473) differs only where the last column block is partial."""
 import numpy as np
 import pytest
 
@@ -20,9 +18,7 @@ from centre_amd import records as rec
 ONE = {0: "bnd", 1: "ang", 2: "tor"}
 
 
-@pytest.mark.parametrize("ex", ["ex1", "ex2"])
-@pytest.mark.parametrize("kind", list(rec.KINDS))
-def test_record_order_matches_stored_files(golden, ex, kind):
+def stored(golden, ex, kind):
     ids = golden[f"{ex}_id_{kind}2d"].astype(int)
     a, b = rec.KINDS[kind]
     cols = golden[f"{ex}_id_{ONE[b]}1d"][:, 0].astype(int).tolist()
@@ -30,25 +26,40 @@ def test_record_order_matches_stored_files(golden, ex, kind):
     for r, c in ids:
         neigh.setdefault(int(r), []).append(int(c))
     neigh = {k: sorted(v) for k, v in sorted(neigh.items())}
+    return neigh, cols, [tuple(x) for x in ids.tolist()]
+
+
+@pytest.mark.parametrize("ex", ["ex1", "ex2"])
+@pytest.mark.parametrize("kind", list(rec.KINDS))
+def test_record_order_matches_stored_files(golden, ex, kind):
+    neigh, cols, want = stored(golden, ex, kind)
     blk = rec.block_size(10, 1)
-    got = rec.record_order(neigh, cols, blk)
-    want = [tuple(x) for x in ids.tolist()]
-    assert sorted(got) == sorted(want)
-    same = next((i for i, (g, w) in enumerate(zip(got, want)) if g != w), len(got))
-    assert same >= min(len(got), 20), (kind, same, len(got))
-    keys = list(neigh)
-    block_of = {k: i // blk for i, k in enumerate(keys)}
-    rb = [block_of[r] for r, _ in want]
-    assert rb == sorted(rb)                       # row blocks in order
-    # each row block holds the same records in both
-    for b in set(rb):
-        assert sorted(p for p in got if block_of[p[0]] == b) == \
-            sorted(p for p in want if block_of[p[0]] == b)
-    if kind == "tor":
-        assert got == want                        # full subsets: exact
+    assert rec.record_order(neigh, cols, blk, procs=2) == want
+    # one process: same records; the order can differ only where the last
+    # column block is partial (23, 22, 258 and 259 members)
+    one = rec.record_order(neigh, cols, blk)
+    assert sorted(one) == sorted(want)
+    if len(cols) % (2 * blk) == 0:
+        assert one == want
     # a different thread count changes the order but not the set
     got4 = rec.record_order(neigh, cols, rec.block_size(10, 4))
-    assert sorted(got4) == sorted(got)
+    assert sorted(got4) == sorted(want)
+
+
+def test_partial_column_block_split_between_ranks():
+    """A partial last column block of bc members goes ceil(bc / P) to each
+    of ranks 1.. and the rest to the master (EntropyCalculator_mpi.cpp:
+    895-906, 1068-1069)."""
+    cols = list(range(8))
+    neigh = {0: list(range(1, 8))}
+    # block 2, 3 ranks: full blocks of 6 = [0,1][2,3][4,5]; last block of 2
+    # members: ceil(2/3) = 1 each, ranks = 2 -> [6] then [7]
+    assert rec.record_order(neigh, cols, 2, 3) == [(0, c) for c in range(1, 8)]
+    neigh = {0: [5, 6, 7], 1: [5, 6, 7]}
+    assert rec.record_order(neigh, cols, 2, 3) == \
+        [(0, 5), (1, 5), (0, 6), (1, 6), (0, 7), (1, 7)]
+    assert rec.record_order(neigh, cols, 2, 1) == \
+        [(0, 5), (1, 5), (0, 6), (0, 7), (1, 6), (1, 7)]
 
 
 def test_subset_and_neighbor_file_parsing():
