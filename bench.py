@@ -505,15 +505,17 @@ def batch_workload(args, hip):
                              s["s1"].ptr, s["s2"].ptr, s["mi"].ptr) for s in systems]
     ctx.device_sync()
     t0 = time.perf_counter()
-    sys_ms, kern_ms, order = ctx.batch_pair_entropy(descs, nb, S, [0], order=0)
+    sys_ms, kern_ms, order, shape = ctx.batch_pair_entropy(descs, nb, S, [0], order=0, shape_used=True)
     ctx.device_sync()
     t_batch = time.perf_counter() - t0
     same = all(np.allclose(systems[k]["mi"].numpy()[:, S - 1], r, rtol=1e-12, atol=1e-15)
                for k, r in zip((0, n_sys - 1), ref))
-    per_order = {}
-    for o in (1, 2, 3):
-        _, ms, _ = ctx.batch_pair_entropy(descs, nb, S, [0], order=o)
-        per_order[str(o)] = ms
+    per_choice = {}
+    R = int(hip.lib().centre_pair_dofs_per_tile(nb))
+    for r in (R, R - 1):
+        for o in (1, 2, 3):
+            _, ms, _ = ctx.batch_pair_entropy(descs, nb, S, [0], order=o, tile_dofs=r)
+            per_choice[f"order{o}_dofs{r}"] = ms
     # MIST of every system on its last-step MI
     t0 = time.perf_counter()
     mist = []
@@ -531,7 +533,7 @@ def batch_workload(args, hip):
                                f"{S} steps, ML, MIE + MIST", "pairs": n_pairs,
                    "frames_total": int(sum(F for _, F in dims))},
         "batch": {"seconds": t_batch, "kernel_ms": kern_ms, "order_used": order,
-                  "kernel_ms_by_order": per_order,
+                  "tile_dofs_used": shape, "kernel_ms_by_choice": per_choice,
                   "system_latency_ms": {"min": float(sys_ms.min()), "median": float(np.median(sys_ms)),
                                         "max": float(sys_ms.max())}},
         "one_launch_per_system": {"seconds": t_seq, "pairs_per_s": n_pairs / t_seq,

@@ -609,32 +609,24 @@ struct BatchRun {
     std::vector<double> sys_ms;
 };
 
-// Launches over a prepared tile list: tiles of a system's last, partial
-// block first (persistent four-wave tile), then the full tiles (eight-wave
-// tile, one workgroup per tile in list order), each list in the given order;
-// at nb <= 8 every tile runs on the eight-wave small-bin tile.  Every
-// system's completion stamp is taken by whichever launch finishes its last
+// One persistent launch over a prepared tile list, in the given order: the
+// eight-wave tile when the bin count allows it (every tile, those of a
+// system's partial last block included), else the four-wave tile.  Every
+// system's completion stamp is taken by the workgroup finishing its last
 // tile.
 int batch_launch(centre_ctx* ctx, const PairKernelArgs& base, const std::vector<BatchSystem>& bs,
                  const std::vector<BatchTile>& tl, int grid, BatchRun* run) {
     const int n_sys = (int)bs.size();
     const bool w8 = pair_tile8_supported(base.R, base.nb);
-    const bool w8all = w8 && pair_tile8_small(base.nb);   // partial tiles too
-    auto eight = [&](const BatchTile& q) {
-        const int nfull = bs[q.sys].n_dof / base.R;
-        return w8all || (w8 && q.I < nfull && q.J < nfull);
-    };
     std::vector<int2> tiles;
     std::vector<int> tsys, per(n_sys, 0);
-    for (int pass = 0; pass < 2; ++pass)
-        for (const BatchTile& q : tl) {
-            if (eight(q) != (pass == 1)) continue;
-            tiles.push_back(make_int2(q.I, q.J));
-            tsys.push_back(q.sys);
-            per[q.sys]++;
-        }
-    int n_part = 0;
-    for (const BatchTile& q : tl) n_part += !eight(q);
+    tiles.reserve(tl.size());
+    tsys.reserve(tl.size());
+    for (const BatchTile& q : tl) {
+        tiles.push_back(make_int2(q.I, q.J));
+        tsys.push_back(q.sys);
+        per[q.sys]++;
+    }
     DevBuf dt, ds, dsys, dper, dcnt, dtime, dwork;
     HIPCHK(dt.alloc(sizeof(int2) * tiles.size()));
     HIPCHK(ds.alloc(sizeof(int) * tsys.size()));
@@ -642,33 +634,27 @@ int batch_launch(centre_ctx* ctx, const PairKernelArgs& base, const std::vector<
     HIPCHK(dper.alloc(sizeof(int) * n_sys));
     HIPCHK(dcnt.alloc(sizeof(int) * n_sys));
     HIPCHK(dtime.alloc(sizeof(unsigned long long) * (n_sys + 1)));
-    HIPCHK(dwork.alloc(2 * sizeof(int)));
+    HIPCHK(dwork.alloc(sizeof(int)));
     HIPCHK(hipMemcpyAsync(dt.p, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(ds.p, tsys.data(), sizeof(int) * tsys.size(), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(dsys.p, bs.data(), sizeof(BatchSystem) * n_sys, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(dper.p, per.data(), sizeof(int) * n_sys, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemsetAsync(dcnt.p, 0, sizeof(int) * n_sys, ctx->stream));
     HIPCHK(hipMemsetAsync(dtime.p, 0, sizeof(unsigned long long) * (n_sys + 1), ctx->stream));
-    HIPCHK(hipMemsetAsync(dwork.p, 0, 2 * sizeof(int), ctx->stream));
+    HIPCHK(hipMemsetAsync(dwork.p, 0, sizeof(int), ctx->stream));
     PairKernelArgs a = base;
     a.systems = dsys.as<BatchSystem>();
     a.tiles = dt.as<int2>();
     a.tile_sys = ds.as<int>();
     a.work = dwork.as<int>();
-    a.n_work = n_part;
+    a.n_work = (int)tiles.size();
     a.sys_count = dcnt.as<int>();
     a.sys_tiles = dper.as<int>();
     a.sys_time = dtime.as<unsigned long long>() + 1;
     a.t0 = dtime.as<unsigned long long>();
-    PairKernelArgs f = a;
-    f.tiles = a.tiles + n_part;
-    f.tile_sys = a.tile_sys + n_part;
-    f.work = a.work + 1;
-    f.n_work = (int)tiles.size() - n_part;
-    if (n_part > 0) f.t0 = nullptr;
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-    HIPCHK(launch_pair_persistent(a, grid, ctx->stream));
-    HIPCHK(launch_pair_batch8(f, ctx->stream));
+    if (w8) HIPCHK(launch_pair_batch8(a, grid, ctx->stream));
+    else HIPCHK(launch_pair_persistent(a, grid, ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
     std::vector<unsigned long long> stamps(n_sys + 1);
     HIPCHK(hipMemcpyAsync(stamps.data(), dtime.p, sizeof(unsigned long long) * (n_sys + 1),
@@ -691,14 +677,19 @@ int batch_launch(centre_ctx* ctx, const PairKernelArgs& base, const std::vector<
 
 extern "C" int centre_batch_pair_entropy(centre_ctx* ctx, const centre_batch_system* sys, int n_sys,
                                          int nbins, int nsteps, const int* ests, int nest,
-                                         int jacobian, int kde, int order, double* sys_ms,
-                                         double* total_ms, int* order_used) {
+                                         int jacobian, int kde, int order, int tile_dofs,
+                                         double* sys_ms, double* total_ms, int* order_used,
+                                         int* tile_dofs_used) {
     if (!ctx) return -3;
     if (!sys || n_sys < 1) return fail(ctx, -3, "no systems");
     if (nbins < 1 || nbins > 255) return fail(ctx, -3, "nbins must be in 1..255");
     if (nsteps < 1 || nsteps > 255) return fail(ctx, -3, "nsteps must be in 1..255");
     if (!ests || nest < 1) return fail(ctx, -3, "no estimators");
     if (order < 0 || order > 3) return fail(ctx, -3, "order must be 0 (autotune) .. 3");
+    const int r_max = dofs_per_tile(nbins);
+    if (tile_dofs < 0 || tile_dofs > r_max || (pair_tile8_small(nbins) && tile_dofs != 0 && tile_dofs != r_max))
+        return fail(ctx, -3, "tile_dofs must be 0 (autotune) or 1.." + std::to_string(r_max) +
+                                 (pair_tile8_small(nbins) ? " (32 at nbins <= 8)" : ""));
     int slot_ml = -1, slot_mm = -1;
     for (int e = 0; e < nest; ++e) {
         if (ests[e] == 0 && slot_ml < 0) slot_ml = e;
@@ -749,7 +740,7 @@ extern "C" int centre_batch_pair_entropy(centre_ctx* ctx, const centre_batch_sys
     }
     PairKernelArgs base{};
     base.nb = nbins;
-    base.R = dofs_per_tile(nbins);
+    base.R = r_max;
     base.nsteps = nsteps;
     base.dof_w = nullptr;
     base.slot_ml = slot_ml;
@@ -760,33 +751,50 @@ extern "C" int centre_batch_pair_entropy(centre_ctx* ctx, const centre_batch_sys
     int dev_cu = 0;
     HIPCHK(hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, ctx->device));
     const int grid = dev_cu;   // one workgroup per CU: the tile's LDS fills most of a CU
-    // autotune: time each order on a calibration share of the batch (2% of
-    // each system's tiles, at most 1,024 frames per step: about 0.6% of the
-    // full run's work for the three orders together)
-    int chosen = order;
-    if (order == 0) {
-        double best = 1e300;
-        for (int cand = 1; cand <= 3; ++cand) {
-            auto tl = batch_tiles(sys, n_sys, base.R, nsteps, cand, 1024, 0.02);
-            // calibration outputs land in the real output buffers and are
-            // overwritten by the full run below
-            BatchRun run;
-            std::vector<BatchSystem> cbs = bs;
-            for (auto& c : cbs) c.ss = std::min<uint64_t>(c.ss, 1024);
-            if (int r = batch_launch(ctx, base, cbs, tl, grid, &run)) return r;
-            if (run.total_ms < best) {
-                best = run.total_ms;
-                chosen = cand;
-            }
-        }
+    // Autotuner over the unset knobs: the tile-list order (1..3) and the
+    // tile shape, R DOFs a side (the largest that fits the 256-row tile, or
+    // one fewer: more, smaller tiles spread a batch of small systems wider).
+    // Each candidate runs on a calibration share of the batch (2% of each
+    // system's tiles, at most 1,024 frames per step); about 0.2% of the full
+    // run's work per candidate.
+    std::vector<int> orders, shapes;
+    if (order) orders.push_back(order);
+    else orders = {1, 2, 3};
+    if (tile_dofs) shapes.push_back(tile_dofs);
+    else {
+        shapes.push_back(r_max);
+        if (!pair_tile8_small(nbins) && r_max > 1) shapes.push_back(r_max - 1);
     }
-    auto tl = batch_tiles(sys, n_sys, base.R, nsteps, chosen, ~0ull, 1.0);
+    int chosen = orders[0], chosen_R = shapes[0];
+    if (orders.size() * shapes.size() > 1) {
+        double best = 1e300;
+        for (int R : shapes)
+            for (int cand : orders) {
+                PairKernelArgs cb = base;
+                cb.R = R;
+                auto tl = batch_tiles(sys, n_sys, R, nsteps, cand, 1024, 0.02);
+                // calibration outputs land in the real output buffers and
+                // are overwritten by the full run below
+                BatchRun run;
+                std::vector<BatchSystem> cbs = bs;
+                for (auto& c : cbs) c.ss = std::min<uint64_t>(c.ss, 1024);
+                if (int r = batch_launch(ctx, cb, cbs, tl, grid, &run)) return r;
+                if (run.total_ms < best) {
+                    best = run.total_ms;
+                    chosen = cand;
+                    chosen_R = R;
+                }
+            }
+    }
+    base.R = chosen_R;
+    auto tl = batch_tiles(sys, n_sys, chosen_R, nsteps, chosen, ~0ull, 1.0);
     BatchRun run;
     if (int r = batch_launch(ctx, base, bs, tl, grid, &run)) return r;
     if (sys_ms)
         for (int k = 0; k < n_sys; ++k) sys_ms[k] = run.sys_ms[k];
     if (total_ms) *total_ms = run.total_ms;
     if (order_used) *order_used = chosen;
+    if (tile_dofs_used) *tile_dofs_used = chosen_R;
     ctx->last_pair_ms = (float)run.total_ms;
     return check_flag(ctx);
 }

@@ -148,7 +148,7 @@ bool pair_tile8_supported(int R, int nb);
 hipError_t launch_pair_tiles8(const PairKernelArgs& a, int n_tiles, hipStream_t st);
 // Batch form (full tiles only, step-aligned codes): one workgroup per entry
 // of a.tiles[0 .. a.n_work), dispatched in list order.
-hipError_t launch_pair_batch8(const PairKernelArgs& a, hipStream_t st);
+hipError_t launch_pair_batch8(const PairKernelArgs& a, int grid, hipStream_t st);
 hipError_t launch_pair_tiles(const PairKernelArgs& a, int n_tiles, bool dump,
                              hipStream_t st);
 // S2 (ML / MM slots) and MI of every pair of n_tiles tiles from the

@@ -237,14 +237,49 @@ __host__ __device__ inline Lay8 lay8(int R, bool small, bool nz) {
     L.log = off;   // log(1 + j/256), 1/(1 + j/256), then 8 series coefficients
     off += (2 * 256 + 8) * sizeof(double);
     L.tab = off;
-    const size_t room = off < kLdsCap ? (kLdsCap - off) / sizeof(double) : 0;
+    // 16 bytes stay free past the layout: the persistent batch kernel's
+    // tile counter word
+    const size_t room = off + 16 < kLdsCap ? (kLdsCap - off - 16) / sizeof(double) : 0;
     L.ntab = (int)((room < (size_t)kTab ? room : (size_t)kTab) / 256 * 256);
     off += sizeof(double) * L.ntab;
     L.total = off;
     return L;
 }
 
-template <bool kBatch, bool kNz, bool kSmall>
+// Tile-independent LDS contents of the eight-wave tile: the row/column DOF
+// maps, log(1 + j/256) and 1/(1 + j/256), the log-series coefficients and
+// the c log c table.  The persistent batch kernel writes them once before
+// its tile loop (computed inside the loop they would be hoisted out of it
+// and held in registers across every tile).
+template <bool kNz, bool kSmall>
+__device__ __forceinline__ void tile8_tables(int R, int nb) {
+    extern __shared__ __align__(16) uint8_t smem_raw[];
+    const Lay8 lay = lay8(R, kSmall, kNz);
+    uint8_t* const row2dof = smem_raw + lay.map;
+    uint8_t* const col2dof = smem_raw + lay.map + 256;
+    double* const logt = reinterpret_cast<double*>(smem_raw + lay.log);
+    double* const invt = logt + 256;
+    double* const ctab = reinterpret_cast<double*>(smem_raw + lay.tab);
+    const int tid = threadIdx.x;
+    if (tid < 256) {
+        const int d = tid / (kSmall ? 8 : nb);
+        row2dof[tid] = (uint8_t)(d < R ? d : 0xFF);
+        col2dof[tid] = (uint8_t)(d < R ? d : 0xFF);
+        const double t = 1.0 + tid * (1.0 / 256.0);
+        logt[tid] = log(t);
+        invt[tid] = 1.0 / t;
+        if (tid < 8) {
+            const double cf[8] = {1.0, -0.5, 1.0 / 3.0, -0.25, 0.2, -1.0 / 6.0,
+                                  0.6931471805599453, 0.0};
+            invt[256 + tid] = cf[tid];
+        }
+    }
+    for (int c = tid; c < lay.ntab; c += 512) ctab[c] = c > 1 ? (double)c * log((double)c) : 0.0;
+}
+
+// kTables: also write the tile-independent LDS contents (tile8_tables);
+// false when the caller wrote them already.
+template <bool kBatch, bool kNz, bool kSmall, bool kTables = true>
 __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) {
     extern __shared__ __align__(16) uint8_t smem_raw[];
     const PairKernelArgs& a = a_in;
@@ -270,7 +305,11 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     sm.ctab = reinterpret_cast<double*>(smem_raw + lay.tab);
     uint8_t* const trash = smem_raw + lay.map + 512;
 
-    const int tid = threadIdx.x, lane = tid & 63;
+    int tid = threadIdx.x;
+    // in the persistent loop, values derived from the thread index alone
+    // would be hoisted out of the loop and held across tiles: hide it
+    if constexpr (!kTables) asm volatile("" : "+v"(tid));
+    const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wr = wave >> 2, wc = wave & 3;
     const int2 tile = a.tiles[tix];
@@ -281,24 +320,11 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
 
     for (int i = tid; i < kBuf8 * 2 * kOpBytes / 16; i += 512)
         reinterpret_cast<uint4*>(sm.oh)[i] = make_uint4(0, 0, 0, 0);
-    if (tid < 256) {
-        const int d = tid / (kSmall ? 8 : nb);
-        sm.row2dof[tid] = (uint8_t)(d < R ? d : 0xFF);
-        sm.col2dof[tid] = (uint8_t)(d < R ? d : 0xFF);
-        const double t = 1.0 + tid * (1.0 / 256.0);
-        sm.logt[tid] = log(t);
-        sm.invt[tid] = 1.0 / t;
-        if (tid < 8) {
-            const double cf[8] = {1.0, -0.5, 1.0 / 3.0, -0.25, 0.2, -1.0 / 6.0,
-                                  0.6931471805599453, 0.0};
-            sm.invt[256 + tid] = cf[tid];
-        }
-    }
+    if constexpr (kTables) tile8_tables<kNz, kSmall>(R, nb);
     for (int i = tid; i < R * R * nstr; i += 512) {
         sm.slotT[i] = 0.0;
         if constexpr (kNz) sm.slotN[i] = 0;
     }
-    for (int c = tid; c < ntab; c += 512) sm.ctab[c] = c > 1 ? (double)c * log((double)c) : 0.0;
     if (tid == 0) *reinterpret_cast<int*>(trash) = *reinterpret_cast<volatile int*>(a.flag);
     // small-bin tile: bit 8*rb + cb marks a 32x32 MFMA block (row DOFs
     // 4rb..4rb+3, column DOFs 4cb..4cb+3) holding a pair to evaluate; the
@@ -331,7 +357,10 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     const int q = (wave * 64 + lane) % nlive;
     const int ld = q / kGroups, k4 = (q - ld * kGroups) * 4;
     const bool isA = ld < R;
-    const uint8_t* const src = codes + (uint64_t)(isA ? r0 + ld : c0 + (ld - R)) * cstride + k4;
+    // rows past the system's last DOF (a partial last block) read the last
+    // row; their pairs are never written
+    const long long grow = isA ? r0 + ld : c0 + (ld - R);
+    const uint8_t* const src = codes + (uint64_t)(grow < D ? grow : D - 1) * cstride + k4;
     const uint32_t dst = (uint32_t)((isA ? 0 : kOpBytes) + (isA ? ld : ld - R) * nb * kLdsRow + k4);
     const uint32_t trash_off = (uint32_t)(lay.map + 512 + 16 + k4);
     // what this slot set in the buffers staged one, two and three chunks
@@ -677,21 +706,33 @@ __global__ __launch_bounds__(512) void pair_tile8_kernel(PairKernelArgs a) {
     tile8_body<false, kNz, kSmall>(a, blockIdx.x);
 }
 
-// Batch form of the eight-wave tile: one workgroup per entry of the ordered
-// batch tile list.  Workgroups are dispatched in list order as CUs free up,
-// which is the work queue of the persistent form; a loop around the tile
-// body would push it past the two-waves-per-SIMD register budget.  The
-// workgroup finishing a system's last tile stamps the system's completion.
+// Batch form of the eight-wave tile, persistent: one workgroup per CU takes
+// entries of the ordered batch tile list (every system's tiles, those of a
+// partial last block included) from a shared counter until the list is
+// exhausted.  The counter value is handed to the workgroup through the word
+// after the tile body's LDS.  The workgroup finishing a system's last tile
+// stamps the system's completion (100 MHz wall clock).
+// Persistent batch kernel: the tile-independent LDS tables are written once,
+// then each tile of the loop writes only its operand buffers and pair slots.
 template <bool kNz, bool kSmall>
-__global__ __launch_bounds__(512) void pair_batch8_kernel(PairKernelArgs a) {
-    const int t = blockIdx.x;
-    if (t == 0 && threadIdx.x == 0 && a.t0) *a.t0 = wall_clock64();
-    tile8_body<true, kNz, kSmall>(a, t);
-    if (threadIdx.x == 0 && a.sys_count) {
-        const int sys = a.tile_sys[t];
-        __threadfence();
-        if (atomicAdd(&a.sys_count[sys], 1) == a.sys_tiles[sys] - 1)
-            a.sys_time[sys] = wall_clock64();
+__global__ __launch_bounds__(512) void pair_batch8_kernel(PairKernelArgs a, uint32_t lds_next) {
+    extern __shared__ __align__(16) uint8_t smem_raw[];
+    int* const s_t = reinterpret_cast<int*>(smem_raw + lds_next);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && a.t0) *a.t0 = wall_clock64();
+    tile8_tables<kNz, kSmall>(a.R, a.nb);
+    for (;;) {
+        __syncthreads();   // the previous tile is done with LDS and s_t
+        if (threadIdx.x == 0) *s_t = atomicAdd(a.work, 1);
+        __syncthreads();
+        const int t = __builtin_amdgcn_readfirstlane(*s_t);
+        if (t >= a.n_work) break;
+        tile8_body<true, kNz, kSmall, false>(a, t);
+        if (threadIdx.x == 0 && a.sys_count) {
+            const int sys = a.tile_sys[t];
+            __threadfence();
+            if (atomicAdd(&a.sys_count[sys], 1) == a.sys_tiles[sys] - 1)
+                a.sys_time[sys] = wall_clock64();
+        }
     }
 }
 
@@ -733,12 +774,15 @@ hipError_t launch_pair_tiles8(const PairKernelArgs& a, int n_tiles, hipStream_t 
     return hipGetLastError();
 }
 
-hipError_t launch_pair_batch8(const PairKernelArgs& a, hipStream_t st) {
-    if (a.n_work <= 0) return hipSuccess;
+hipError_t launch_pair_batch8(const PairKernelArgs& a, int grid, hipStream_t st) {
+    if (a.n_work <= 0 || grid <= 0) return hipSuccess;
     if (!pair_tile8_supported(a.R, a.nb)) return hipErrorInvalidValue;
     const bool small = pair_tile8_small(a.nb);
     const bool nz = a.slot_mm >= 0 || a.mi_from_mm;
-    const size_t lds = lay8(a.R, small, nz).total;
+    const size_t body = lay8(a.R, small, nz).total;
+    const uint32_t next = (uint32_t)((body + 15) & ~(size_t)15);
+    const size_t lds = next + 16;
+    if (lds > kLdsCap) return hipErrorInvalidValue;
     const void* fn;
     if (small) fn = nz ? reinterpret_cast<const void*>(&pair_batch8_kernel<true, true>)
                        : reinterpret_cast<const void*>(&pair_batch8_kernel<false, true>);
@@ -747,11 +791,11 @@ hipError_t launch_pair_batch8(const PairKernelArgs& a, hipStream_t st) {
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     if (small) {
-        if (nz) hipLaunchKernelGGL((pair_batch8_kernel<true, true>), dim3(a.n_work), dim3(512), lds, st, a);
-        else hipLaunchKernelGGL((pair_batch8_kernel<false, true>), dim3(a.n_work), dim3(512), lds, st, a);
+        if (nz) hipLaunchKernelGGL((pair_batch8_kernel<true, true>), dim3(grid), dim3(512), lds, st, a, next);
+        else hipLaunchKernelGGL((pair_batch8_kernel<false, true>), dim3(grid), dim3(512), lds, st, a, next);
     } else {
-        if (nz) hipLaunchKernelGGL((pair_batch8_kernel<true, false>), dim3(a.n_work), dim3(512), lds, st, a);
-        else hipLaunchKernelGGL((pair_batch8_kernel<false, false>), dim3(a.n_work), dim3(512), lds, st, a);
+        if (nz) hipLaunchKernelGGL((pair_batch8_kernel<true, false>), dim3(grid), dim3(512), lds, st, a, next);
+        else hipLaunchKernelGGL((pair_batch8_kernel<false, false>), dim3(grid), dim3(512), lds, st, a, next);
     }
     return hipGetLastError();
 }

@@ -27,7 +27,8 @@ _SIGS = {
     "centre_sync": ([P], I32),
     "centre_stream": ([P], P),
     "centre_last_pair_kernel_ms": ([P], C.c_float),
-    "centre_batch_pair_entropy": ([P, P, I32, I32, I32, P, I32, I32, I32, I32, P, P, P], I32),
+    "centre_batch_pair_entropy": ([P, P, I32, I32, I32, P, I32, I32, I32, I32, I32, P, P, P, P],
+                                  I32),
     "centre_screen_n_params": ([], I32),
     "centre_screen_tiles": ([P, P, U32, I32, C.c_float, I32, P, P], I32),
     "centre_pair_plan_from_tiles": ([P, U32, I32, P, U64, P], I32),
@@ -179,16 +180,21 @@ class Context:
         return d
 
     def batch_pair_entropy(self, systems, nbins, nsteps, ests=(0,), jacobian=True, kde=False,
-                           order=0):
-        """systems: list of BatchSystem -> (per-system ms, total ms, order used)."""
+                           order=0, tile_dofs=0, shape_used=False):
+        """systems: list of BatchSystem -> (per-system ms, total ms, order
+        used[, DOFs per tile side used when shape_used])."""
         arr = (BatchSystem * len(systems))(*systems)
         ests = _c(np.atleast_1d(ests), np.int32)
         sys_ms = np.zeros(len(systems), np.float64)
         tot = C.c_double(0.0)
         used = C.c_int(0)
+        rused = C.c_int(0)
         self.check(self._L.centre_batch_pair_entropy(
             self.h, C.cast(arr, C.c_void_p), len(systems), nbins, nsteps, _p(ests), len(ests),
-            int(jacobian), int(kde), order, _p(sys_ms), C.byref(tot), C.byref(used)))
+            int(jacobian), int(kde), order, tile_dofs, _p(sys_ms), C.byref(tot), C.byref(used),
+            C.byref(rused)))
+        if shape_used:
+            return sys_ms, tot.value, used.value, rused.value
         return sys_ms, tot.value, used.value
 
     # -- learned pair screen (device pointers) ---------------------------------

@@ -245,12 +245,19 @@ int centre_device_sync(centre_ctx *ctx);
  * persistent launch.  Every system gets all its i < j pairs (triangle order)
  * and its marginals, exactly as centre_dev_pair_entropy with an all-pairs
  * plan would give them; ML and MM only.
- * order: tile order of the shared work queue -- 1 longest tiles first,
- * 2 system by system, 3 systems by total work, largest first; 0 = pick by
- * timing the three on a calibration share of the batch (5% of each system's
- * tiles, steps cut to 4,096 frames).  sys_ms[k] (optional): time from the
- * launch to the completion of system k; total_ms: the whole launch;
- * order_used: the order run. */
+ * One persistent launch runs every tile of every system from one work
+ * queue (one workgroup per CU taking tiles from a shared counter).
+ * order: tile order of the queue -- 1 longest tiles first, 2 system by
+ * system, 3 systems by total work, largest first.
+ * tile_dofs: DOFs a tile takes per side (its shape: tile_dofs * nbins of the
+ * 256 one-hot rows), 1 .. centre_pair_dofs_per_tile(nbins) (exactly that at
+ * nbins <= 8).  0 in either knob = autotune it: every candidate (the three
+ * orders; the largest shape and one DOF fewer) runs on a calibration share
+ * of the batch (2% of each system's tiles, steps cut to 1,024 frames) and
+ * the fastest is run.  The frame chunk (64 frames per pipeline stage) is
+ * fixed by the tile's LDS budget.
+ * sys_ms[k] (optional): time from the launch to the completion of system k;
+ * total_ms: the whole launch; order_used / tile_dofs_used: what was run. */
 typedef struct centre_batch_system {
     const uint8_t *d_codes;    /* [n_dof][code_stride] */
     uint64_t code_stride;
@@ -265,8 +272,9 @@ typedef struct centre_batch_system {
 int centre_batch_pair_entropy(centre_ctx *ctx, const centre_batch_system *sys,
                               int n_sys, int nbins, int nsteps,
                               const int *ests, int nest, int jacobian,
-                              int kde, int order, double *sys_ms,
-                              double *total_ms, int *order_used);
+                              int kde, int order, int tile_dofs,
+                              double *sys_ms, double *total_ms,
+                              int *order_used, int *tile_dofs_used);
 
 /* ---- learned pair screen ------------------------------------------------------
  * No reference counterpart (the reference restricts pairs only by a user

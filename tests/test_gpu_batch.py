@@ -1,6 +1,7 @@
 """Batched pair stage (one persistent launch over many systems) against the
 single-system path: the same S1, S2 and MI for every system, whatever the
-tile order of the shared work queue, including the autotuned one."""
+tile order of the shared work queue and the tile shape (DOFs per tile side),
+including the autotuned ones; and against the oracle."""
 import numpy as np
 import pytest
 
@@ -47,8 +48,10 @@ def test_batch_matches_single_system_path(nb, ests):
                              True, False, d1.ptr, d2.ptr, dm.ptr)
         plan.close()
         want.append((d1.numpy(), d2.numpy(), dm.numpy()))
-    for order in (1, 2, 3, 0):
-        print(f"batch nb={nb} order={order} sizes={sizes}", flush=True)
+    R = int(hip.lib().centre_pair_dofs_per_tile(nb))
+    shapes = [R] if nb <= 8 else [R, R - 1, 2]
+    for order, shape in [(1, 0), (2, 0), (3, 0), (0, 0)] + [(1, r) for r in shapes]:
+        print(f"batch nb={nb} order={order} tile_dofs={shape} sizes={sizes}", flush=True)
         outs, descs = [], []
         for s in systems:
             D, F = s["D"], s["F"]
@@ -58,8 +61,10 @@ def test_batch_matches_single_system_path(nb, ests):
             outs.append(o)
             descs.append(hip.BatchSystem(s["codes"].ptr, F, D, F, s["types"].ptr, s["ext"].ptr,
                                          o[0].ptr, o[1].ptr, o[2].ptr))
-        sys_ms, tot, used = ctx.batch_pair_entropy(descs, nb, S, ests, order=order)
+        sys_ms, tot, used, rused = ctx.batch_pair_entropy(descs, nb, S, ests, order=order,
+                                                          tile_dofs=shape, shape_used=True)
         assert used in (1, 2, 3) and (order == 0 or used == order)
+        assert rused == shape if shape else rused in (R, R - 1)
         assert tot > 0 and np.all(sys_ms > 0) and np.all(sys_ms <= tot * 1.05 + 0.05)
         for (a1, a2, am), o in zip(want, outs):
             # S1 comes from the same sequential per-DOF sums: identical.  Pair
@@ -69,4 +74,36 @@ def test_batch_matches_single_system_path(nb, ests):
             np.testing.assert_array_equal(o[0].numpy(), a1)
             np.testing.assert_allclose(o[1].numpy(), a2, rtol=1e-12, atol=1e-14)
             np.testing.assert_allclose(o[2].numpy(), am, rtol=1e-12, atol=1e-14)
+    ctx.close()
+
+
+def test_batch_vs_oracle_and_bad_shape():
+    """Partial last blocks on the persistent eight-wave tile (D not a
+    multiple of the tile's DOFs) against the oracle directly; tile shapes
+    outside 1..R are refused."""
+    import oracle as O
+    ctx = hip.Context(0)
+    nb, S = 50, 2
+    systems = [make_system(ctx, 0xCE27E + 300 + k, 4, 3, 4 + k, 4000 + 1000 * k, nb) for k in range(3)]
+    outs, descs = [], []
+    for s in systems:
+        D, F = s["D"], s["F"]
+        P = D * (D - 1) // 2
+        o = (ctx.alloc((D, 1, S), np.float64), ctx.alloc((P, 1, S), np.float64), ctx.alloc((P, S), np.float64))
+        outs.append(o)
+        descs.append(hip.BatchSystem(s["codes"].ptr, F, D, F, s["types"].ptr, s["ext"].ptr,
+                                     o[0].ptr, o[1].ptr, o[2].ptr))
+    ctx.batch_pair_entropy(descs, nb, S, [0], order=1, tile_dofs=5)
+    for s, o in zip(systems, outs):
+        D = s["D"]
+        codes, types, ext = s["codes"].numpy(), s["types"].numpy(), s["ext"].numpy()
+        iu, ju = np.triu_indices(D, 1)
+        want = O.pair_entropies(codes, types, ext, np.stack([iu, ju], 1), S, [nb])
+        np.testing.assert_allclose(o[1].numpy()[:, 0, :], want[:, 0, 0, :], rtol=1e-10, atol=1e-12)
+        s1 = O.dof_entropies(codes, types, ext, S, [nb])
+        mi = s1[iu, 0, 0, :] + s1[ju, 0, 0, :] - want[:, 0, 0, :]
+        np.testing.assert_allclose(o[2].numpy(), mi, rtol=1e-9, atol=1e-11)
+    for bad in (6, -1):
+        with pytest.raises(RuntimeError):
+            ctx.batch_pair_entropy(descs, nb, S, [0], order=1, tile_dofs=bad)
     ctx.close()
