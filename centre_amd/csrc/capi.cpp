@@ -174,7 +174,7 @@ int centre_dev_discretize_hist(centre_ctx* ctx, const double* d_bat, uint64_t ld
         return fail(ctx, -3, "nbins_ref required for torsion gap optimisation");
     if (n_dof == 0) return 0;
     HIPCHK(hipSetDevice(ctx->device));
-    const int nchunk = disc_chunks(n_frame);
+    const int nchunk = disc_chunks(n_frame, n_dof);
     DevBuf sch, part, gh, shift;
     HIPCHK(sch.alloc(n_schemes));
     HIPCHK(part.alloc(sizeof(double) * 8 * (size_t)nchunk * n_dof));
@@ -1180,6 +1180,29 @@ int check_pair_args(centre_ctx* ctx, const centre_pair_plan* plan, const uint8_t
     return 0;
 }
 
+// The fused pair stage's temporaries laid out back to back from `base`, each
+// part on a 256-byte boundary, in base_total()'s order; the split dumps follow
+// them when `with_split` is set.
+PairWork carve_pair_work(uint8_t* base, const PairLayout& l, bool with_split) {
+    uint8_t* next = base;
+    auto take = [&next](size_t bytes) -> void* {
+        if (!bytes) return nullptr;
+        void* p = next;
+        next += (bytes + 255) & ~(size_t)255;
+        return p;
+    };
+    PairWork ws;
+    ws.s1 = static_cast<double*>(take(l.s1));
+    ws.L = static_cast<double*>(take(l.L));
+    ws.w = static_cast<double*>(take(l.w));
+    ws.s1e0 = static_cast<double*>(take(l.s1e0));
+    ws.packed = static_cast<uint8_t*>(take(l.packed));
+    ws.cs2 = static_cast<double*>(take(l.cs2));
+    ws.cmi = static_cast<double*>(take(l.cmi));
+    if (with_split) ws.split = static_cast<uint32_t*>(take(l.split));
+    return ws;
+}
+
 bool fused_estimators(const int* ests, int nest) {
     bool ml = false, mm = false;
     for (int e = 0; e < nest; ++e) {
@@ -1209,30 +1232,11 @@ extern "C" int centre_dev_pair_entropy(centre_ctx* ctx, const centre_pair_plan* 
                                 nsteps, ests, nest, jacobian, kde, d_s1, d_s2, d_mi, d_counts);
     PairLayout l = pair_layout(plan, d_codes, code_stride, n_frame, nsteps, nest, d_mi != nullptr);
     if (int r = choose_split(ctx, plan, n_frame / nsteps, nsteps, (size_t)1 << 30, &l)) return r;
-    DevBuf s1, L, w, s1e0, packed, cs2, cmi;
-    PairWork ws;
-    HIPCHK(s1.alloc(l.s1));
-    HIPCHK(L.alloc(l.L));
-    HIPCHK(w.alloc(l.w));
-    ws.s1 = s1.as<double>();
-    ws.L = L.as<double>();
-    ws.w = w.as<double>();
-    if (l.s1e0) {
-        HIPCHK(s1e0.alloc(l.s1e0));
-        ws.s1e0 = s1e0.as<double>();
-    }
-    if (l.packed) {
-        HIPCHK(packed.alloc(l.packed));
-        ws.packed = packed.as<uint8_t>();
-    }
-    if (l.cs2) {
-        HIPCHK(cs2.alloc(l.cs2));
-        ws.cs2 = cs2.as<double>();
-    }
-    if (l.cmi) {
-        HIPCHK(cmi.alloc(l.cmi));
-        ws.cmi = cmi.as<double>();
-    }
+    // temporaries in the context's grow-only scratch (no allocation once a
+    // call of this size has run)
+    void* base = nullptr;
+    HIPCHK(ctx_scratch(ctx, 5, l.base_total(), &base));
+    PairWork ws = carve_pair_work(static_cast<uint8_t*>(base), l, false);
     if (l.split) {   // allocated before the timed region, kept by the context
         void* sp = nullptr;
         HIPCHK(ctx_scratch(ctx, 2, l.split, &sp));
@@ -1323,22 +1327,7 @@ extern "C" int centre_dev_pair_entropy_noalloc(centre_ctx* ctx, const centre_pai
     // the frame split only where its count dumps fit what is left
     if (int r = choose_split(ctx, plan, n_frame / nsteps, nsteps, kNoAllocBytes - base, &l))
         return r;
-    uint8_t* next = static_cast<uint8_t*>(ctx->noalloc);
-    auto take = [&next](size_t bytes) -> void* {
-        if (!bytes) return nullptr;
-        void* p = next;
-        next += (bytes + 255) & ~(size_t)255;
-        return p;
-    };
-    PairWork ws;
-    ws.s1 = static_cast<double*>(take(l.s1));
-    ws.L = static_cast<double*>(take(l.L));
-    ws.w = static_cast<double*>(take(l.w));
-    ws.s1e0 = static_cast<double*>(take(l.s1e0));
-    ws.packed = static_cast<uint8_t*>(take(l.packed));
-    ws.cs2 = static_cast<double*>(take(l.cs2));
-    ws.cmi = static_cast<double*>(take(l.cmi));
-    ws.split = static_cast<uint32_t*>(take(l.split));
+    PairWork ws = carve_pair_work(static_cast<uint8_t*>(ctx->noalloc), l, true);
     return run_pair_fused(ctx, plan, d_codes, code_stride, n_frame, d_types, d_extrema, nsteps,
                           ests, nest, jacobian, kde, d_s1, d_s2, d_mi, l, ws);
 }
@@ -1544,7 +1533,10 @@ extern "C" int centre_dev_mst(centre_ctx* ctx, const uint32_t* d_src, const uint
     const size_t o_comp = up(sizeof(unsigned long long) * n_node);
     const size_t o_be = o_comp + up(sizeof(uint32_t) * n_node);
     const size_t o_ch = o_be + up(sizeof(unsigned int) * n_node);
-    const size_t o_it = o_ch + 256;
+    const size_t o_ti = o_ch + 256;                       // tree edge indices
+    const size_t o_tw = o_ti + up(sizeof(uint32_t) * n_node);   // their weights
+    const size_t o_pt = o_tw + up(sizeof(double) * n_node);     // per-workgroup maxima
+    const size_t o_it = o_pt + up(mst_part_bytes(n_node));
     void* mem = nullptr;
     HIPCHK(ctx_scratch(ctx, 3, o_it + (n_edge ? n_edge : 1), &mem));
     uint8_t* const base = static_cast<uint8_t*>(mem);
@@ -1559,6 +1551,7 @@ extern "C" int centre_dev_mst(centre_ctx* ctx, const uint32_t* d_src, const uint
     a.best_e = reinterpret_cast<unsigned int*>(base + o_be);
     a.changed = reinterpret_cast<int*>(base + o_ch);
     a.in_tree = base + o_it;
+    a.part = mst_part_bytes(n_node) ? reinterpret_cast<unsigned long long*>(base + o_pt) : nullptr;
     {
         // endpoint range check before any kernel indexes by an endpoint
         int bad = 0;
@@ -1576,25 +1569,22 @@ extern "C" int centre_dev_mst(centre_ctx* ctx, const uint32_t* d_src, const uint
         HIPCHK(hipStreamSynchronize(ctx->stream));
         if (!changed) break;
     }
-    std::vector<uint8_t> flags(n_edge);
-    if (n_edge)
-        HIPCHK(hipMemcpy(flags.data(), a.in_tree, n_edge, hipMemcpyDeviceToHost));
-    std::vector<uint32_t> idx;
-    for (uint64_t e = 0; e < n_edge; ++e)
-        if (flags[e]) idx.push_back((uint32_t)e);
-    if (idx.size() >= n_node) return fail(ctx, -3, "internal: forest larger than n_node - 1");
-    // weights of the tree edges, gathered on the device
-    std::vector<double> wt(idx.size());
-    if (!idx.empty()) {
-        std::vector<uint64_t> idx64(idx.begin(), idx.end());
-        DevBuf di, dw;
-        HIPCHK(di.alloc(sizeof(uint64_t) * idx64.size()));
-        HIPCHK(dw.alloc(sizeof(double) * idx64.size()));
-        HIPCHK(hipMemcpyAsync(di.p, idx64.data(), sizeof(uint64_t) * idx64.size(),
-                              hipMemcpyHostToDevice, ctx->stream));
-        HIPCHK(launch_gather_rec(d_w, di.as<uint64_t>(), idx64.size(), 1, dw.as<double>(),
-                                 ctx->stream));
-        HIPCHK(hipMemcpyAsync(wt.data(), dw.p, sizeof(double) * wt.size(), hipMemcpyDeviceToHost,
+    // tree edges compacted on the device: a forest has at most n_node - 1
+    int n_in = 0;
+    HIPCHK(hipMemsetAsync(a.changed, 0, sizeof(int), ctx->stream));
+    uint32_t* const d_ti = reinterpret_cast<uint32_t*>(base + o_ti);
+    double* const d_tw = reinterpret_cast<double*>(base + o_tw);
+    HIPCHK(mst_collect(a, d_ti, d_tw, n_node, ctx->stream));
+    HIPCHK(hipMemcpyAsync(&n_in, a.changed, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (n_in < 0 || (uint64_t)n_in >= n_node)
+        return fail(ctx, -3, "internal: forest larger than n_node - 1");
+    std::vector<uint32_t> idx(n_in);
+    std::vector<double> wt(n_in);
+    if (n_in) {
+        HIPCHK(hipMemcpyAsync(idx.data(), d_ti, sizeof(uint32_t) * n_in, hipMemcpyDeviceToHost,
+                              ctx->stream));
+        HIPCHK(hipMemcpyAsync(wt.data(), d_tw, sizeof(double) * n_in, hipMemcpyDeviceToHost,
                               ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
     }

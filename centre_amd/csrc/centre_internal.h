@@ -21,8 +21,10 @@ struct centre_ctx {
     // synchronise before returning, so the next call may reuse them; freed
     // by centre_destroy.
     // 4: count dumps of the grouped estimator pass (CS / JS, count output).
-    void* scratch[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-    size_t scratch_bytes[5] = {0, 0, 0, 0, 0};
+    // 5: the fused pair stage's temporaries (marginals, step logs, weights,
+    // repacked codes, canonical outputs of explicit plans).
+    void* scratch[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    size_t scratch_bytes[6] = {0, 0, 0, 0, 0, 0};
     // device address of the library's static scratch array on this device
     // (the allocation-free pair-stage entry point carves it)
     void* noalloc = nullptr;
@@ -231,7 +233,7 @@ struct DiscArgs {
     int* flag;
 };
 hipError_t run_discretize_hist(const DiscArgs& a, int nchunk, hipStream_t st);
-int disc_chunks(uint64_t n_frame);
+int disc_chunks(uint64_t n_frame, uint32_t n_dof);
 
 // ---- vmKDE -------------------------------------------------------------------
 struct VmkdeArgs {
@@ -269,8 +271,16 @@ struct MstArgs {
     unsigned int* best_e;   // [V]
     uint8_t* in_tree;       // [E]
     int* changed;           // device word
+    unsigned long long* part;  // [kMstGroups][V] per-workgroup maxima (V <= kMstLdsNodes)
 };
+// Boruvka's max pass reduces per workgroup in LDS up to this many nodes
+// (8 B each), over kMstGroups workgroups.
+constexpr uint32_t kMstLdsNodes = 16384;
+constexpr int kMstGroups = 256;
+size_t mst_part_bytes(uint32_t n_node);
 hipError_t run_mst_round(const MstArgs& a, hipStream_t st);
+hipError_t mst_collect(const MstArgs& a, uint32_t* idx, double* w, uint32_t cap,
+                       hipStream_t st);
 hipError_t mst_init(const MstArgs& a, hipStream_t st);
 hipError_t mst_check(const MstArgs& a, hipStream_t st);   // sets *changed on a bad endpoint
 

@@ -47,30 +47,42 @@ __global__ __launch_bounds__(256) void stats_kernel(DiscArgs a, int pass,
     double v0 = 1.0e100, v1 = -1.0e100, v2 = 0.0, v3 = 1.0e100, v4 = -1.0e100;
     const double bs = (kTwoPi + kDx) / m;
     const double sh = ok && pass == 1 ? a.shift[d] : 0.0;
-    if (ok) {
-        for (uint64_t f = f0 + ty; f < f1; f += 4) {
-            const double x = (double)(float)a.bat[f * a.ld + d];
-            if (pass == 0) {
-                v2 += x;
-                if (v0 > x) v0 = x;
-                if (v1 < x) v1 = x;
-                if (gap) {
-                    const int t1 = (int)fabs(floor(x / bs));
-                    const uint32_t code = (uint8_t)t1;
-                    if (code < (uint32_t)m) atomicAdd(&gh[tx * m + code], 1u);
-                    else atomicOr(a.flag, 2);
-                }
-            } else {
-                const double p = fmod(x + kPi, kTwoPi);
-                if (p < v0) v0 = p;
-                if (p > v1) v1 = p;
-                if (a.optimize) {
-                    const double o = fmod(x + sh, kTwoPi);
-                    if (o < v3) v3 = o;
-                    if (o > v4) v4 = o;
-                }
+    // four frames of the thread's stride-4 sequence are loaded before any
+    // is used, so each lane keeps four HBM reads in flight
+    auto visit = [&](double x) {
+        if (pass == 0) {
+            v2 += x;
+            if (v0 > x) v0 = x;
+            if (v1 < x) v1 = x;
+            if (gap) {
+                const int t1 = (int)fabs(floor(x / bs));
+                const uint32_t code = (uint8_t)t1;
+                if (code < (uint32_t)m) atomicAdd(&gh[tx * m + code], 1u);
+                else atomicOr(a.flag, 2);
+            }
+        } else {
+            const double p = fmod(x + kPi, kTwoPi);
+            if (p < v0) v0 = p;
+            if (p > v1) v1 = p;
+            if (a.optimize) {
+                const double o = fmod(x + sh, kTwoPi);
+                if (o < v3) v3 = o;
+                if (o > v4) v4 = o;
             }
         }
+    };
+    if (ok) {
+        const double* col = a.bat + d;
+        uint64_t f = f0 + ty;
+        for (; f + 12 < f1; f += 16) {
+            const float x0 = (float)col[f * a.ld], x1 = (float)col[(f + 4) * a.ld];
+            const float x2 = (float)col[(f + 8) * a.ld], x3 = (float)col[(f + 12) * a.ld];
+            visit(x0);
+            visit(x1);
+            visit(x2);
+            visit(x3);
+        }
+        for (; f < f1; f += 4) visit((double)(float)col[f * a.ld]);
     }
     red[ty][tx][0] = v0;
     red[ty][tx][1] = v1;
@@ -150,12 +162,27 @@ __global__ void finalize_kernel(DiscArgs a, int nchunk, int pass) {
     const size_t cs = (size_t)a.n_dof * 8;
     double* e = a.ext + 4 * (size_t)d;
     if (pass == 0) {
-        double mn = 1.0e100, mx = -1.0e100, sum = 0.0;
-        for (int c = 0; c < nchunk; ++c) {
-            mn = fmin(mn, p[c * cs + 0]);
-            mx = fmax(mx, p[c * cs + 1]);
-            sum += p[c * cs + 2];
+        // four chunk records per iteration, in independent accumulators
+        double mn4[4] = {1.0e100, 1.0e100, 1.0e100, 1.0e100};
+        double mx4[4] = {-1.0e100, -1.0e100, -1.0e100, -1.0e100};
+        double sm4[4] = {0.0, 0.0, 0.0, 0.0};
+        int c = 0;
+        for (; c + 4 <= nchunk; c += 4) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                mn4[u] = fmin(mn4[u], p[(c + u) * cs + 0]);
+                mx4[u] = fmax(mx4[u], p[(c + u) * cs + 1]);
+                sm4[u] += p[(c + u) * cs + 2];
+            }
         }
+        for (; c < nchunk; ++c) {
+            mn4[0] = fmin(mn4[0], p[c * cs + 0]);
+            mx4[0] = fmax(mx4[0], p[c * cs + 1]);
+            sm4[0] += p[c * cs + 2];
+        }
+        double mn = fmin(fmin(mn4[0], mn4[1]), fmin(mn4[2], mn4[3]));
+        double mx = fmax(fmax(mx4[0], mx4[1]), fmax(mx4[2], mx4[3]));
+        const double sum = (sm4[0] + sm4[1]) + (sm4[2] + sm4[3]);
         const double avg = sum / (double)a.n_frame;
         if (a.dtype == 1 || a.dtype == 2) {
             mn -= kDx;
@@ -176,6 +203,7 @@ __global__ void finalize_kernel(DiscArgs a, int nchunk, int pass) {
     }
     // pass 1 (torsions): choose the range
     double mn2 = 6.5, mx2 = 0.0, mn3 = 1.0e100, mx3 = 0.0;
+#pragma unroll 4
     for (int c = 0; c < nchunk; ++c) {
         mn2 = fmin(mn2, p[c * cs + 3]);
         mx2 = fmax(mx2, p[c * cs + 4]);
@@ -255,9 +283,16 @@ __global__ __launch_bounds__(256) void bin_kernel(DiscArgs a) {
 
 }  // namespace
 
-int disc_chunks(uint64_t n_frame) {
-    uint64_t c = (n_frame + 2047) / 2048;
-    return (int)(c < 1 ? 1 : (c > 64 ? 64 : c));
+// Frame chunks of the stats passes: about 1,024 workgroups over the chip
+// (four per CU), each chunk at least 512 frames, at most 128 chunks so the
+// per-DOF finalize loop stays short.
+int disc_chunks(uint64_t n_frame, uint32_t n_dof) {
+    const uint64_t groups = (n_dof + 63) / 64;
+    uint64_t c = (1024 + groups - 1) / groups;
+    const uint64_t cap = (n_frame + 511) / 512;
+    if (c > cap) c = cap;
+    if (c > 128) c = 128;
+    return (int)(c < 1 ? 1 : c);
 }
 
 hipError_t run_discretize_hist(const DiscArgs& a, int nchunk, hipStream_t st) {

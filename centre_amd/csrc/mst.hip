@@ -39,14 +39,86 @@ __global__ void mst_reset_kernel(MstArgs a) {
     if (t == 0) *a.changed = 0;
 }
 
+// Largest key over the wave's lanes (inactive lanes pass 0).
+__device__ inline unsigned long long wave_max(unsigned long long k) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(k, off);
+        k = o > k ? o : k;
+    }
+    return k;
+}
+
+// atomicMax of k into table[c] for every active lane; when every lane of the
+// wave names the same c (the source side of a row of the pair triangle, or
+// late rounds with few components) the wave reduces first and one lane
+// writes, instead of 64 atomics on one word.
+__device__ inline void max_into(unsigned long long* table, uint32_t c, bool act,
+                                unsigned long long k) {
+    const uint32_t c0 = __builtin_amdgcn_readfirstlane(c);
+    if (__all(c == c0)) {
+        const unsigned long long m = wave_max(act ? k : 0ull);
+        if ((threadIdx.x & 63) == 0 && m) atomicMax(&table[c0], m);
+    } else if (act) {
+        atomicMax(&table[c], k);
+    }
+}
+
+// Graphs whose node table does not fit in LDS: keys straight into best_w.
 __global__ void mst_max_kernel(MstArgs a) {
     const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= a.n_edge) return;
-    const uint32_t cu = a.comp[a.src[e]], cv = a.comp[a.dst[e]];
-    if (cu == cv) return;
-    const unsigned long long k = order_key(a.w[e]);
-    atomicMax(&a.best_w[cu], k);
-    atomicMax(&a.best_w[cv], k);
+    const bool ok = e < a.n_edge;
+    uint32_t cu = 0, cv = 0;
+    unsigned long long k = 0;
+    if (ok) {
+        cu = a.comp[a.src[e]];
+        cv = a.comp[a.dst[e]];
+        k = order_key(a.w[e]);
+    }
+    const bool act = ok && cu != cv;
+    max_into(a.best_w, cu, act, k);
+    max_into(a.best_w, cv, act, k);
+}
+
+// Graphs of at most kMstLdsNodes nodes: each workgroup takes a grid-strided
+// share of the edges, reduces their keys per component in an LDS table and
+// writes the table whole to its row of `part` ([groups][n_node]); the
+// column maxima then go to best_w without a single global atomic.
+__global__ __launch_bounds__(1024) void mst_max_lds_kernel(MstArgs a, unsigned long long* part) {
+    extern __shared__ unsigned long long tab[];
+    for (uint32_t c = threadIdx.x; c < a.n_node; c += blockDim.x) tab[c] = 0ull;
+    __syncthreads();
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    // every lane runs the same number of iterations (max_into needs whole waves)
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < a.n_edge; base += stride) {
+        const uint64_t e = base + threadIdx.x;
+        const bool ok = e < a.n_edge;
+        uint32_t cu = 0, cv = 0;
+        unsigned long long k = 0;
+        if (ok) {
+            cu = a.comp[a.src[e]];
+            cv = a.comp[a.dst[e]];
+            k = order_key(a.w[e]);
+        }
+        const bool act = ok && cu != cv;
+        max_into(tab, cu, act, k);
+        max_into(tab, cv, act, k);
+    }
+    __syncthreads();
+    unsigned long long* row = part + (size_t)blockIdx.x * a.n_node;
+    for (uint32_t c = threadIdx.x; c < a.n_node; c += blockDim.x) row[c] = tab[c];
+}
+
+__global__ __launch_bounds__(256) void mst_colmax_kernel(MstArgs a, const unsigned long long* part,
+                                                         int groups) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.n_node) return;
+    unsigned long long m = 0ull;
+    for (int g = 0; g < groups; ++g) {
+        const unsigned long long v = part[(size_t)g * a.n_node + c];
+        m = v > m ? v : m;
+    }
+    a.best_w[c] = m;
 }
 
 __global__ void mst_argmin_kernel(MstArgs a) {
@@ -109,6 +181,25 @@ __global__ __launch_bounds__(256) void segsum_kernel(const double* x, const uint
 
 }  // namespace
 
+// Tree edges appended to (idx, w) through a counter in *a.changed (their
+// order is fixed afterwards by a sort on the host).
+__global__ void mst_collect_kernel(MstArgs a, uint32_t* idx, double* w, uint32_t cap) {
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.n_edge || !a.in_tree[e]) return;
+    const uint32_t slot = (uint32_t)atomicAdd(a.changed, 1);
+    if (slot < cap) {
+        idx[slot] = (uint32_t)e;
+        w[slot] = a.w[e];
+    }
+}
+
+hipError_t mst_collect(const MstArgs& a, uint32_t* idx, double* w, uint32_t cap, hipStream_t st) {
+    if (!a.n_edge) return hipSuccess;
+    hipLaunchKernelGGL(mst_collect_kernel, dim3((unsigned)((a.n_edge + 255) / 256)), dim3(256), 0,
+                       st, a, idx, w, cap);
+    return hipGetLastError();
+}
+
 // *a.changed |= 1 when an edge endpoint is not below n_node
 __global__ void mst_check_kernel(MstArgs a) {
     const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -127,11 +218,31 @@ hipError_t mst_init(const MstArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 
+size_t mst_part_bytes(uint32_t n_node) {
+    return n_node <= kMstLdsNodes ? sizeof(unsigned long long) * kMstGroups * (size_t)n_node : 0;
+}
+
 hipError_t run_mst_round(const MstArgs& a, hipStream_t st) {
     const dim3 gv((a.n_node + 255) / 256), ge((unsigned)((a.n_edge + 255) / 256));
     hipLaunchKernelGGL(mst_reset_kernel, gv, dim3(256), 0, st, a);
     if (a.n_edge) {
-        hipLaunchKernelGGL(mst_max_kernel, ge, dim3(256), 0, st, a);
+        if (a.part && a.n_node <= kMstLdsNodes) {
+            // fewer workgroups than edge blocks when the graph is small
+            const uint64_t blocks = (a.n_edge + 1023) / 1024;
+            const int groups = blocks < (uint64_t)kMstGroups ? (int)blocks : kMstGroups;
+            const size_t lds = sizeof(unsigned long long) * a.n_node;
+            if (lds > 65536) {
+                const hipError_t e = hipFuncSetAttribute(
+                    reinterpret_cast<const void*>(&mst_max_lds_kernel),
+                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+                if (e != hipSuccess) return e;
+            }
+            hipLaunchKernelGGL(mst_max_lds_kernel, dim3(groups), dim3(1024), lds, st, a, a.part);
+            hipLaunchKernelGGL(mst_colmax_kernel, gv, dim3(256), 0, st, a,
+                               (const unsigned long long*)a.part, groups);
+        } else {
+            hipLaunchKernelGGL(mst_max_kernel, ge, dim3(256), 0, st, a);
+        }
         hipLaunchKernelGGL(mst_argmin_kernel, ge, dim3(256), 0, st, a);
     }
     hipLaunchKernelGGL(mst_hook_kernel, gv, dim3(256), 0, st, a);

@@ -283,7 +283,8 @@ def test_code_out_of_range_is_an_error(ctx, bad, F):
 
 # ---------------------------------------------------------------------------
 def kruskal_set(n, src, dst, w):
-    order = sorted(range(len(w)), key=lambda e: (-w[e], e))
+    w = np.asarray(w, np.float64) + 0.0          # -0.0 and +0.0 tie
+    order = np.lexsort((np.arange(len(w)), -w)).tolist()
     par = list(range(n))
 
     def find(x):
@@ -297,6 +298,8 @@ def kruskal_set(n, src, dst, w):
         if a != b:
             par[a] = b
             out.append(e)
+            if len(out) == n - 1:
+                break
     return out
 
 
@@ -348,6 +351,42 @@ def test_mst_neighbour_graph_scale(ctx):
     got = ctx.mst(n, src, dst, w)
     want = O.boruvka(n, src, dst, w)
     assert len(got) == n - 1
+    assert sorted(got.tolist()) == sorted(want.tolist())
+    assert got.tolist() == kruskal_set(n, src.tolist(), dst.tolist(), w.tolist())
+
+
+def test_mst_complete_graph_in_pair_order(ctx):
+    """All pairs of 2,500 DOFs in the pair stage's (i < j) order, weights from
+    a small set (heavy ties): rows of the triangle give waves whose edges
+    share a source, which the max pass reduces across the wave before its
+    LDS table; the forest equals the oracle's Boruvka and Kruskal."""
+    rng = np.random.default_rng(5)
+    n = 2500
+    src, dst = np.triu_indices(n, 1)
+    w = rng.integers(0, 200, len(src)).astype(float) / 9.0
+    print(f"mst complete n={n} m={len(src)}", flush=True)
+    got = ctx.mst(n, src, dst, w)
+    want = O.boruvka(n, src, dst, w)
+    assert len(got) == n - 1
+    assert sorted(got.tolist()) == sorted(want.tolist())
+    assert got.tolist() == kruskal_set(n, src.tolist(), dst.tolist(), w.tolist())
+
+
+def test_mst_more_nodes_than_the_lds_table(ctx):
+    """40,000 nodes (beyond the 16,384-node LDS table, so keys go straight to
+    global memory) in a few components, with runs of edges sharing an
+    endpoint and tied weights: the same forest as the oracle's."""
+    rng = np.random.default_rng(11)
+    n, m = 40000, 300000
+    src = rng.integers(0, n, m)
+    dst = rng.integers(0, n, m)
+    src[:64000] = np.repeat(rng.integers(0, n, 1000), 64)   # wave-uniform sources
+    keep = (src != dst) & (src % 3 == dst % 3)                # three components at least
+    src, dst = src[keep], dst[keep]
+    w = rng.integers(0, 40, len(src)).astype(float) / 3.0
+    print(f"mst n={n} m={len(src)}", flush=True)
+    got = ctx.mst(n, src, dst, w)
+    want = O.boruvka(n, src, dst, w)
     assert sorted(got.tolist()) == sorted(want.tolist())
     assert got.tolist() == kruskal_set(n, src.tolist(), dst.tolist(), w.tolist())
 
