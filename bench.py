@@ -302,18 +302,20 @@ def _system(ctx, hip, seed, D, F, nb):
     return d_codes, d_ext, ctx.to_device(types)
 
 
-def _pass(ctx, hip, d_codes, d_types, d_ext, D, F, nb, S, tiles=None):
-    """Pair stage over all pairs (or the given tiles): returns MI of the last
-    step in triangle order (pairs outside the tiles: NaN), S1 and seconds."""
+def _pass(ctx, hip, d_codes, d_types, d_ext, D, F, nb, S, tiles=None, f0=0, nf=None):
+    """Pair stage over all pairs (or the given tiles) of frames f0..f0+nf of
+    rows of F frames: returns MI of the last step in triangle order (pairs
+    outside the tiles: NaN), S1 and seconds."""
     P = D * (D - 1) // 2
+    nf = F - f0 if nf is None else nf
     d_s1 = ctx.alloc((D, 1, S), np.float64)
     d_s2 = ctx.alloc((P, 1, S), np.float64)
     d_mi = ctx.alloc((P, S), np.float64)
     ctx.device_sync()
     t0 = time.perf_counter()
     plan = hip.PairPlan(ctx, D, nb, tiles=tiles) if tiles is not None else hip.PairPlan(ctx, D, nb)
-    ctx.dev_pair_entropy(plan, d_codes.ptr, F, F, d_types.ptr, d_ext.ptr, S, [0], True, False,
-                         d_s1.ptr, d_s2.ptr, d_mi.ptr)
+    ctx.dev_pair_entropy(plan, d_codes.ptr + f0, F, nf, d_types.ptr, d_ext.ptr, S, [0], True,
+                         False, d_s1.ptr, d_s2.ptr, d_mi.ptr)
     ctx.device_sync()
     dt = time.perf_counter() - t0
     plan.close()
@@ -409,6 +411,8 @@ def screen_measure(ctx, hip, n_train, D):
     full_mist, full_tree = _mist(ctx, D, iu, ju, mi, s1.sum())
     t_plain = t_pass + time.perf_counter() - t0
 
+    trees = {}
+
     def screened(cut):
         ctx.device_sync()
         t0 = time.perf_counter()
@@ -423,6 +427,7 @@ def screen_measure(ctx, hip, n_train, D):
         i, j = _tile_pairs(tiles, D, R)
         mst, tree = _mist(ctx, D, i, j, mi_s[_tri(i, j, D)], s1.sum())
         t_m = time.perf_counter() - t1
+        trees[cut] = tree
         tot = t_sel + t_p + t_m
         return {"cutoff": cut, "tiles": int(len(tiles)), "pairs_computed": len(i) / P,
                 "seconds": tot, "select_s": t_sel, "pass_s": t_p, "mst_s": t_m,
@@ -432,6 +437,36 @@ def screen_measure(ctx, hip, n_train, D):
 
     at_spec = screened(1e-3)
     at_tuned = screened(chosen)
+    # How reproducible the plain tree is: the plain trees of the two halves of
+    # the frames, compared with each other and with the full tree.  Edge
+    # agreement below this level is the data's sampling noise (near-equal MI
+    # among exchangeable DOFs), not the screen's.
+    half = F // 2
+    mi_a, s1_a, _ = _pass(ctx, hip, d_codes, d_types, d_ext, D, F, nb, S // 2, nf=half)
+    mi_b, s1_b, _ = _pass(ctx, hip, d_codes, d_types, d_ext, D, F, nb, S // 2, f0=half, nf=half)
+    _, tree_a = _mist(ctx, D, iu, ju, mi_a, s1_a.sum())
+    _, tree_b = _mist(ctx, D, iu, ju, mi_b, s1_b.sum())
+    del mi_a, mi_b
+    n_tree = max(1, len(full_tree))
+    types = d_types.numpy()
+    tree_s = trees[chosen]
+
+    def by_class(tree):
+        # plain-tree edges with a torsion end, and those between bonds/angles
+        tor = [e for e in full_tree if types[e[0]] == 3 or types[e[1]] == 3]
+        ba = [e for e in full_tree if types[e[0]] != 3 and types[e[1]] != 3]
+        return {"torsion_edges": len(tor),
+                "torsion_agree": sum(e in tree for e in tor) / max(1, len(tor)),
+                "bond_angle_edges": len(ba),
+                "bond_angle_agree": sum(e in tree for e in ba) / max(1, len(ba))}
+
+    reproducibility = {
+        "halves_agree": len(tree_a & tree_b) / n_tree,
+        "first_half_vs_full": len(tree_a & full_tree) / n_tree,
+        "second_half_vs_full": len(tree_b & full_tree) / n_tree,
+        "screened_by_class": by_class(tree_s),
+        "first_half_by_class": by_class(tree_a),
+    }
     for a in (d_codes, d_ext, d_types):
         a.free()
     return {
@@ -442,6 +477,7 @@ def screen_measure(ctx, hip, n_train, D):
                   f"MIST; trained on {n_train - n_hold} + {n_hold} held-back 1,000-DOF systems",
         "n_sub": n_sub, "topk_partners": topk,
         "plain": {"seconds": t_plain, "pass_s": t_pass, "mist": full_mist, "pairs": P},
+        "plain_tree_reproducibility": reproducibility,
         "screen_at_1e-3": at_spec,
         "screen_tuned": at_tuned,
         "cutoff_tuning": tune,

@@ -113,12 +113,23 @@ __global__ __launch_bounds__(256) void mst_colmax_kernel(MstArgs a, const unsign
                                                          int groups) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= a.n_node) return;
-    unsigned long long m = 0ull;
-    for (int g = 0; g < groups; ++g) {
-        const unsigned long long v = part[(size_t)g * a.n_node + c];
-        m = v > m ? v : m;
+    // four rows per iteration in independent maxima, so their loads overlap
+    unsigned long long m[4] = {0ull, 0ull, 0ull, 0ull};
+    int g = 0;
+    for (; g + 4 <= groups; g += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const unsigned long long v = part[(size_t)(g + u) * a.n_node + c];
+            m[u] = v > m[u] ? v : m[u];
+        }
     }
-    a.best_w[c] = m;
+    for (; g < groups; ++g) {
+        const unsigned long long v = part[(size_t)g * a.n_node + c];
+        m[0] = v > m[0] ? v : m[0];
+    }
+    const unsigned long long m01 = m[0] > m[1] ? m[0] : m[1];
+    const unsigned long long m23 = m[2] > m[3] ? m[2] : m[3];
+    a.best_w[c] = m01 > m23 ? m01 : m23;
 }
 
 __global__ void mst_argmin_kernel(MstArgs a) {
@@ -227,9 +238,10 @@ hipError_t run_mst_round(const MstArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(mst_reset_kernel, gv, dim3(256), 0, st, a);
     if (a.n_edge) {
         if (a.part && a.n_node <= kMstLdsNodes) {
-            // fewer workgroups than edge blocks when the graph is small
-            const uint64_t blocks = (a.n_edge + 1023) / 1024;
-            const int groups = blocks < (uint64_t)kMstGroups ? (int)blocks : kMstGroups;
+            // at least 16 edges per thread, so that small graphs leave few
+            // table rows for the column pass
+            const uint64_t want = (a.n_edge + 16383) / 16384;
+            const int groups = want < (uint64_t)kMstGroups ? (int)want : kMstGroups;
             const size_t lds = sizeof(unsigned long long) * a.n_node;
             if (lds > 65536) {
                 const hipError_t e = hipFuncSetAttribute(
