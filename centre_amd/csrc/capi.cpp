@@ -652,6 +652,9 @@ int batch_launch(centre_ctx* ctx, const PairKernelArgs& base, const std::vector<
     a.sys_tiles = dper.as<int>();
     a.sys_time = dtime.as<unsigned long long>() + 1;
     a.t0 = dtime.as<unsigned long long>();
+    uint64_t ss_max = 1;
+    for (const BatchSystem& b : bs) ss_max = std::max<uint64_t>(ss_max, b.ss);
+    a.kfix = pair_fixed_k((double)base.nsteps * (double)ss_max);
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
     if (w8) HIPCHK(launch_pair_batch8(a, grid, ctx->stream));
     else HIPCHK(launch_pair_persistent(a, grid, ctx->stream));

@@ -138,7 +138,11 @@ struct PairKernelArgs {
     const int* sys_tiles;         // [n_sys] tiles per system
     unsigned long long* sys_time; // [n_sys] completion stamp (100 MHz)
     unsigned long long* t0;       // launch stamp
+    // eight-wave tile: c log c sums in units of 2^-kfix (pair_fixed_k of the
+    // largest frame count a step of the launch reaches)
+    int kfix;
 };
+int pair_fixed_k(double n_max);
 
 
 // Launchers (return hipError_t of the launch).
@@ -305,8 +309,11 @@ constexpr int kOffB3 = kOffW3 + kScreenHidden;
 constexpr int kOffMu = kOffB3 + 1;
 constexpr int kOffSd = kOffMu + kScreenFeat;
 constexpr int kScreenParams = kOffSd + kScreenFeat;
+// Bins of the subsample MI feature: the codes merged to min(nb, 8) bins.
+constexpr int kScreenMiBins = 8;
 hipError_t launch_subsample(const uint8_t* codes, uint64_t stride, uint32_t D, uint64_t step,
-                            uint32_t n_sub, uint8_t* out, hipStream_t st);
+                            uint32_t n_sub, uint8_t* out, uint8_t* coarse, int nb, int nbc,
+                            hipStream_t st);
 hipError_t launch_dof_moments(const uint8_t* sub, uint32_t D, uint32_t n_sub, int nb, float* X,
                               float* Y, hipStream_t st);
 hipError_t launch_corr(const float* X, const float* Y, uint32_t D, uint32_t n, const int2* blocks,

@@ -11,7 +11,8 @@
 // maximum -- hooks every component to its choice (a 2-cycle through one
 // edge is broken towards the lower label), and relabels by pointer jumping.
 // Rounds stop when no cross edge remains, so disconnected graphs (A-MIST
-// neighbour lists) give a forest, as in the reference.
+// neighbour lists) give a forest, as in the reference.  Edges whose weight is
+// NaN are left out (a screened pair pass leaves the pairs it skipped NaN).
 #include "centre_internal.h"
 
 namespace {
@@ -70,12 +71,15 @@ __global__ void mst_max_kernel(MstArgs a) {
     const bool ok = e < a.n_edge;
     uint32_t cu = 0, cv = 0;
     unsigned long long k = 0;
+    bool num = false;
     if (ok) {
         cu = a.comp[a.src[e]];
         cv = a.comp[a.dst[e]];
-        k = order_key(a.w[e]);
+        const double w = a.w[e];
+        num = !isnan(w);
+        k = order_key(w);
     }
-    const bool act = ok && cu != cv;
+    const bool act = ok && num && cu != cv;
     max_into(a.best_w, cu, act, k);
     max_into(a.best_w, cv, act, k);
 }
@@ -95,12 +99,15 @@ __global__ __launch_bounds__(1024) void mst_max_lds_kernel(MstArgs a, unsigned l
         const bool ok = e < a.n_edge;
         uint32_t cu = 0, cv = 0;
         unsigned long long k = 0;
+        bool num = false;
         if (ok) {
             cu = a.comp[a.src[e]];
             cv = a.comp[a.dst[e]];
-            k = order_key(a.w[e]);
+            const double w = a.w[e];
+            num = !isnan(w);
+            k = order_key(w);
         }
-        const bool act = ok && cu != cv;
+        const bool act = ok && num && cu != cv;
         max_into(tab, cu, act, k);
         max_into(tab, cv, act, k);
     }
@@ -136,8 +143,9 @@ __global__ void mst_argmin_kernel(MstArgs a) {
     const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= a.n_edge) return;
     const uint32_t cu = a.comp[a.src[e]], cv = a.comp[a.dst[e]];
-    if (cu == cv) return;
-    const unsigned long long k = order_key(a.w[e]);
+    const double w = a.w[e];
+    if (cu == cv || isnan(w)) return;
+    const unsigned long long k = order_key(w);
     if (k == a.best_w[cu]) atomicMin(&a.best_e[cu], (unsigned int)e);
     if (k == a.best_w[cv]) atomicMin(&a.best_e[cv], (unsigned int)e);
 }

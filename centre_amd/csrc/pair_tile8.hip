@@ -18,7 +18,8 @@ namespace {
 // their own wave); the others only multiply.
 template <bool kNz>
 __device__ __forceinline__ void epi_block8(const v16i& blk, int brow, int gcol, int lane,
-                                           const Smem& sm, int nb, int R, int nstr, int ntab) {
+                                           const Smem& sm, int nb, int R, int nstr, int ntab,
+                                           int kfix) {
     // the lane holds column gcol and rows row0 + 8*(r/4) + r%4, r = 0..15,
     // row0 = brow + 4*(lane/32) in the 32-row block starting at brow; with
     // two waves per SIMD the accumulators are ordinary VGPRs and are read in
@@ -47,16 +48,17 @@ __device__ __forceinline__ void epi_block8(const v16i& blk, int brow, int gcol, 
     int mx = 0;
 #pragma unroll
     for (int r = 0; r < 16; ++r) mx = max(mx, blk[r]);
-    double tot = 0.0, hi = 0.0;
+    // sums in the table's fixed point (units of 2^-kfix)
+    unsigned long long tot = 0ull, hi = 0ull;
     int ntot = 0, nhi = 0;
     if (__builtin_amdgcn_ballot_w64(mx >= ntab) == 0) {
         // every count of the block is in the table (wave-uniform); the 16
         // lookups are issued together and summed as a tree, so that their
         // latencies overlap
         if (one) {
-            double v[16];
+            unsigned long long v[16];
 #pragma unroll
-            for (int r = 0; r < 16; ++r) v[r] = sm.ctab[blk[r]];
+            for (int r = 0; r < 16; ++r) v[r] = sm.ctabq[blk[r]];
 #pragma unroll
             for (int r = 0; r < 8; ++r) v[r] = v[2 * r] + v[2 * r + 1];
 #pragma unroll
@@ -70,9 +72,9 @@ __device__ __forceinline__ void epi_block8(const v16i& blk, int brow, int gcol, 
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const bool h = 8 * (r >> 2) + (r & 3) >= kk;
-                const double v = sm.ctab[blk[r]];
+                const unsigned long long v = sm.ctabq[blk[r]];
                 tot += v;
-                hi += h ? v : 0.0;
+                hi += h ? v : 0ull;
                 ntot += min(blk[r], 1);
                 nhi += h ? min(blk[r], 1) : 0;
             }
@@ -81,22 +83,22 @@ __device__ __forceinline__ void epi_block8(const v16i& blk, int brow, int gcol, 
             // 8q..8q+3 in the first half-wave, 8q+4..8q+7 in the second, so
             // only the group holding bl is split, and per lane only there:
             // it adds the suffix of its four values from the first row >= bl
-            double v[16];
+            unsigned long long v[16];
 #pragma unroll
-            for (int r = 0; r < 16; ++r) v[r] = sm.ctab[blk[r]];
-            double g[4];
-            hi = -0.0;
+            for (int r = 0; r < 16; ++r) v[r] = sm.ctabq[blk[r]];
+            unsigned long long g[4];
+            hi = 0ull;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const double s3 = v[4 * q + 3];
-                const double s2 = s3 + v[4 * q + 2];
-                const double s1 = s2 + v[4 * q + 1];
+                const unsigned long long s3 = v[4 * q + 3];
+                const unsigned long long s2 = s3 + v[4 * q + 2];
+                const unsigned long long s1 = s2 + v[4 * q + 1];
                 g[q] = s1 + v[4 * q];
                 if (bl <= 8 * q) {
                     hi += g[q];
                 } else if (bl < 8 * q + 8) {
                     const int t = tl - 8 * q;   // rows of the group before bl
-                    double part = t == 3 ? s3 : 0.0;
+                    unsigned long long part = t == 3 ? s3 : 0ull;
                     part = t == 2 ? s2 : part;
                     part = t == 1 ? s1 : part;
                     part = t <= 0 ? g[q] : part;
@@ -111,10 +113,11 @@ __device__ __forceinline__ void epi_block8(const v16i& blk, int brow, int gcol, 
         for (int r = 0; r < 16; ++r) {
             const int c = blk[r];
             const bool h = 8 * (r >> 2) + (r & 3) >= split;
-            const double v = c < ntab ? sm.ctab[c]
-                                      : (double)c * log_pos_lds((double)c, sm.logt, sm.invt, sm.invt + 256);
+            const unsigned long long v =
+                c < ntab ? sm.ctabq[c]
+                         : to_fixed((double)c * log_pos_lds((double)c, sm.logt, sm.invt, sm.invt + 256), kfix);
             tot += v;
-            hi += h ? v : 0.0;
+            hi += h ? v : 0ull;
             ntot += min(c, 1);
             nhi += h ? min(c, 1) : 0;
         }
@@ -163,6 +166,21 @@ __device__ __forceinline__ double dpp_sum8(double v) {
     return v;
 }
 
+template <int kCtl>
+__device__ __forceinline__ unsigned long long dpp_q(unsigned long long u) {
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, kCtl, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), kCtl, 0xF, 0xF, true);
+    return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+}
+
+// dpp_sum8 on 64-bit integers
+__device__ __forceinline__ unsigned long long dpp_sum8_q(unsigned long long v) {
+    v += dpp_q<0x111>(v);
+    v += dpp_q<0x112>(v);
+    v += dpp_q<0x114>(v);
+    return v;
+}
+
 __device__ __forceinline__ int dpp_sum8_i(int v) {
     v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, true);
     v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, true);
@@ -172,7 +190,8 @@ __device__ __forceinline__ int dpp_sum8_i(int v) {
 
 template <bool kNz>
 __device__ __forceinline__ void epi_block_small(const v16i& blk, int brow, int gcol, int lane,
-                                                const Smem& sm, int R, int nstr, int ntab) {
+                                                const Smem& sm, int R, int nstr, int ntab,
+                                                int kfix) {
     const double* const coef = sm.invt + 256;
     int mx = 0;
 #pragma unroll
@@ -183,18 +202,19 @@ __device__ __forceinline__ void epi_block_small(const v16i& blk, int brow, int g
     const bool last = (lane & 7) == 7;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        double v[4];
+        unsigned long long v[4];
         if (!big) {
 #pragma unroll
-            for (int m = 0; m < 4; ++m) v[m] = sm.ctab[blk[4 * q + m]];
+            for (int m = 0; m < 4; ++m) v[m] = sm.ctabq[blk[4 * q + m]];
         } else {
 #pragma unroll
             for (int m = 0; m < 4; ++m) {
                 const int c = blk[4 * q + m];
-                v[m] = c < ntab ? sm.ctab[c] : (double)c * log_pos_lds((double)c, sm.logt, sm.invt, coef);
+                v[m] = c < ntab ? sm.ctabq[c]
+                                : to_fixed((double)c * log_pos_lds((double)c, sm.logt, sm.invt, coef), kfix);
             }
         }
-        const double g = dpp_sum8((v[0] + v[1]) + (v[2] + v[3]));
+        const unsigned long long g = dpp_sum8_q((v[0] + v[1]) + (v[2] + v[3]));
         const int rd = (brow >> 3) + q;
         if (last) atomicAdd(&sm.slotT[(rd * R + cd) * nstr + stripe], g);
         if constexpr (kNz) {
@@ -230,7 +250,7 @@ __host__ __device__ inline Lay8 lay8(int R, bool small, bool nz) {
     off = (off + 15) & ~(size_t)15;
     L.nstripe = small ? 2 : 16;
     L.T = off;
-    off += sizeof(double) * R * R * L.nstripe;
+    off += sizeof(unsigned long long) * R * R * L.nstripe;
     L.N = off;
     if (nz) off += sizeof(int) * R * R * L.nstripe;
     off = (off + 15) & ~(size_t)15;
@@ -252,14 +272,14 @@ __host__ __device__ inline Lay8 lay8(int R, bool small, bool nz) {
 // its tile loop (computed inside the loop they would be hoisted out of it
 // and held in registers across every tile).
 template <bool kNz, bool kSmall>
-__device__ __forceinline__ void tile8_tables(int R, int nb) {
+__device__ __forceinline__ void tile8_tables(int R, int nb, int kfix) {
     extern __shared__ __align__(16) uint8_t smem_raw[];
     const Lay8 lay = lay8(R, kSmall, kNz);
     uint8_t* const row2dof = smem_raw + lay.map;
     uint8_t* const col2dof = smem_raw + lay.map + 256;
     double* const logt = reinterpret_cast<double*>(smem_raw + lay.log);
     double* const invt = logt + 256;
-    double* const ctab = reinterpret_cast<double*>(smem_raw + lay.tab);
+    unsigned long long* const ctab = reinterpret_cast<unsigned long long*>(smem_raw + lay.tab);
     const int tid = threadIdx.x;
     if (tid < 256) {
         const int d = tid / (kSmall ? 8 : nb);
@@ -274,7 +294,8 @@ __device__ __forceinline__ void tile8_tables(int R, int nb) {
             invt[256 + tid] = cf[tid];
         }
     }
-    for (int c = tid; c < lay.ntab; c += 512) ctab[c] = c > 1 ? (double)c * log((double)c) : 0.0;
+    for (int c = tid; c < lay.ntab; c += 512)
+        ctab[c] = c > 1 ? to_fixed((double)c * log((double)c), kfix) : 0ull;
 }
 
 // kTables: also write the tile-independent LDS contents (tile8_tables);
@@ -298,11 +319,12 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     sm.cl = nullptr;
     sm.row2dof = smem_raw + lay.map;
     sm.col2dof = smem_raw + lay.map + 256;
-    sm.slotT = reinterpret_cast<double*>(smem_raw + lay.T);
+    sm.slotT = reinterpret_cast<unsigned long long*>(smem_raw + lay.T);
     sm.slotN = reinterpret_cast<int*>(smem_raw + lay.N);
     sm.logt = reinterpret_cast<double*>(smem_raw + lay.log);
     sm.invt = sm.logt + 256;
-    sm.ctab = reinterpret_cast<double*>(smem_raw + lay.tab);
+    sm.ctab = nullptr;
+    sm.ctabq = reinterpret_cast<unsigned long long*>(smem_raw + lay.tab);
     uint8_t* const trash = smem_raw + lay.map + 512;
 
     int tid = threadIdx.x;
@@ -320,9 +342,9 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
 
     for (int i = tid; i < kBuf8 * 2 * kOpBytes / 16; i += 512)
         reinterpret_cast<uint4*>(sm.oh)[i] = make_uint4(0, 0, 0, 0);
-    if constexpr (kTables) tile8_tables<kNz, kSmall>(R, nb);
+    if constexpr (kTables) tile8_tables<kNz, kSmall>(R, nb, a.kfix);
     for (int i = tid; i < R * R * nstr; i += 512) {
-        sm.slotT[i] = 0.0;
+        sm.slotT[i] = 0ull;
         if constexpr (kNz) sm.slotN[i] = 0;
     }
     if (tid == 0) *reinterpret_cast<int*>(trash) = *reinterpret_cast<volatile int*>(a.flag);
@@ -550,8 +572,10 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     };
     // Step-end epilogue, all waves: each block's c log c and nonzero bins
     // into the per-pair LDS slots.
-    auto epilogue = [&]() __attribute__((always_inline)) {
+    auto epilogue = [&](int s) __attribute__((always_inline)) {
         __syncthreads();   // the pending finalize has read and cleared the slots
+        (void)s;
+        const int kfix = a.kfix;
         unsigned long long need = ~0ull;
         if constexpr (kSmall) {
             const unsigned long long m = *blkmask;
@@ -565,11 +589,11 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
                 if constexpr (kSmall) {
                     if ((need >> (8 * (rowblk(i) >> 5) + (colblk(j) >> 5))) & 1)
                         epi_block_small<kNz>(acc[i][j], rowblk(i), colblk(j) + (lane & 31), lane, sm,
-                                             R, nstr, ntab);
+                                             R, nstr, ntab, kfix);
                 }
                 else
                     epi_block8<kNz>(acc[i][j], rowblk(i), colblk(j) + (lane & 31), lane,
-                                    sm, nb, R, nstr, ntab);
+                                    sm, nb, R, nstr, ntab, kfix);
                 __builtin_amdgcn_sched_barrier(0);
             }
     };
@@ -581,6 +605,7 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
         const double* const coef = sm.invt + 256;
         const double n = (double)(fs + 1) * (double)ss;
         const double logn = log_pos_lds(n, sm.logt, sm.invt, coef);
+        const int kfix = a.kfix;
         asm volatile("" ::: "memory");   // read the output pointers here
         double* const s2 = kBatch ? sdp->s2 : a.s2;
         double* const mi = kBatch ? sdp->mi : a.mi;
@@ -594,17 +619,18 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
                                     ? -1
                                     : (a.tilemap ? (long long)a.tilemap[(size_t)tix * R * R + pq]
                                                  : tri_index(gi, gj, D));
-            double T = 0.0;
+            unsigned long long Tq = 0ull;
             int nz = 0;
             for (int k = 0; k < nstr; ++k) {
-                T += sm.slotT[pq * nstr + k];
-                sm.slotT[pq * nstr + k] = 0.0;
+                Tq += sm.slotT[pq * nstr + k];
+                sm.slotT[pq * nstr + k] = 0ull;
                 if constexpr (kNz) {
                     nz += sm.slotN[pq * nstr + k];
                     sm.slotN[pq * nstr + k] = 0;
                 }
             }
             if (p >= 0) {
+                const double T = from_fixed(Tq, kfix);
                 const double L = dof_L[gi * S + fs] + dof_L[gj * S + fs];
                 const double area = dof_w[gi] * dof_w[gj];
                 const double ml = (logn + log_pos_lds(area, sm.logt, sm.invt, coef)) - T / n + L / n;
@@ -690,7 +716,7 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
             // the step's last chunk: the epilogue needs the registers, so the
             // next chunk's fragments are read after the barrier
             const uint32_t nbase = chunk(std::false_type{});
-            epilogue();
+            epilogue(s);
             fin = s;
             __syncthreads();
             if (s + 1 < S) read_all(nbase);
@@ -719,7 +745,7 @@ __global__ __launch_bounds__(512) void pair_batch8_kernel(PairKernelArgs a, uint
     extern __shared__ __align__(16) uint8_t smem_raw[];
     int* const s_t = reinterpret_cast<int*>(smem_raw + lds_next);
     if (blockIdx.x == 0 && threadIdx.x == 0 && a.t0) *a.t0 = wall_clock64();
-    tile8_tables<kNz, kSmall>(a.R, a.nb);
+    tile8_tables<kNz, kSmall>(a.R, a.nb, a.kfix);
     for (;;) {
         __syncthreads();   // the previous tile is done with LDS and s_t
         if (threadIdx.x == 0) *s_t = atomicAdd(a.work, 1);
@@ -750,9 +776,14 @@ bool pair_tile8_supported(int R, int nb) {
     return small ? R == 32 : (R <= 8 && 2 * R * kGroups <= 512);
 }
 
-hipError_t launch_pair_tiles8(const PairKernelArgs& a, int n_tiles, hipStream_t st) {
+int pair_fixed_k(double n_max) { return fixed_k(n_max, std::log(n_max)); }
+
+hipError_t launch_pair_tiles8(const PairKernelArgs& a_in, int n_tiles, hipStream_t st) {
     if (n_tiles <= 0) return hipSuccess;
-    if (!pair_tile8_supported(a.R, a.nb)) return hipErrorInvalidValue;
+    if (!pair_tile8_supported(a_in.R, a_in.nb)) return hipErrorInvalidValue;
+    // the fixed point of the slot sums, from the last step's frame count
+    PairKernelArgs a = a_in;
+    a.kfix = pair_fixed_k((double)a.nsteps * (double)a.ss);
     const bool small = pair_tile8_small(a.nb);
     // nonzero bins are counted only for the Miller-Madow estimator
     const bool nz = a.slot_mm >= 0 || a.mi_from_mm;

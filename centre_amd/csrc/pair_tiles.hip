@@ -83,7 +83,7 @@ __device__ __forceinline__ void tile_body(const PairKernelArgs& a_in, int tix) {
     sm.cl = reinterpret_cast<uint32_t*>(smem_raw + lay.cl);
     sm.row2dof = smem_raw + omap;
     sm.col2dof = smem_raw + omap + 256;
-    sm.slotT = reinterpret_cast<double*>(smem_raw + lay.T);
+    sm.slotT = reinterpret_cast<unsigned long long*>(smem_raw + lay.T);
     sm.slotN = reinterpret_cast<int*>(smem_raw + lay.N);
     sm.logt = reinterpret_cast<double*>(smem_raw + lay.log);
     sm.invt = sm.logt + 256;
@@ -111,7 +111,7 @@ __device__ __forceinline__ void tile_body(const PairKernelArgs& a_in, int tix) {
             sm.col2dof[i] = (uint8_t)(d < nc ? d : 0xFF);
         }
         for (int i = tid; i < R * R * nstr; i += 256) {
-            sm.slotT[i] = 0.0;
+            sm.slotT[i] = 0ull;
             sm.slotN[i] = 0;
         }
         {
@@ -292,6 +292,10 @@ __device__ __forceinline__ void tile_body(const PairKernelArgs& a_in, int tix) {
         // One 32x32 block: the lane holds column (lane & 31) and rows
         // 8*(r/4) + 4*(lane/32) + r%4 for r = 0..15.
         const bool two_dof = nb >= 28 && !kDump;
+        // frames of the step and the fixed-point scale of its slot sums
+        const double n = (double)(s + 1) * (double)a.ss;
+        const double logn = log(n);
+        const int kfix = fixed_k(n, logn);
         auto epi = [&](const v16i& blk, const int i, const int j) __attribute__((always_inline)) {
             if (!(mask & (1u << (i * 4 + j)))) return;   // wave-uniform
             const int gcol = 128 * wc + 32 * j + (lane & 31);
@@ -356,12 +360,12 @@ __device__ __forceinline__ void tile_body(const PairKernelArgs& a_in, int tix) {
                 if (!ok) return;
                 const int stripe = lane % nstr;
                 if (ntot > nhi) {
-                    atomicAdd(&sm.slotT[(dlo * R + cd) * nstr + stripe], tot - hi);
+                    atomicAdd(&sm.slotT[(dlo * R + cd) * nstr + stripe], to_fixed(tot - hi, kfix));
                     atomicAdd(&sm.slotN[(dlo * R + cd) * nstr + stripe], ntot - nhi);
                 }
                 const int dhi = sm.row2dof[row0 + 27];
                 if (nhi && dhi != 0xFF && dhi != dlo) {
-                    atomicAdd(&sm.slotT[(dhi * R + cd) * nstr + stripe], hi);
+                    atomicAdd(&sm.slotT[(dhi * R + cd) * nstr + stripe], to_fixed(hi, kfix));
                     atomicAdd(&sm.slotN[(dhi * R + cd) * nstr + stripe], nhi);
                 }
                 return;
@@ -380,7 +384,7 @@ __device__ __forceinline__ void tile_body(const PairKernelArgs& a_in, int tix) {
                     if (fused && rd != 0xFF && cnt > 0) {
                         const double v = cnt < ntab ? sm.ctab[cnt]
                                                     : (double)cnt * log_count(cnt, sm.logt, sm.invt);
-                        atomicAdd(&sm.slotT[(rd * R + cd) * nstr + lane % nstr], v);
+                        atomicAdd(&sm.slotT[(rd * R + cd) * nstr + lane % nstr], to_fixed(v, kfix));
                         atomicAdd(&sm.slotN[(rd * R + cd) * nstr + lane % nstr], 1);
                     }
                     if constexpr (kDump) {
@@ -411,8 +415,6 @@ __device__ __forceinline__ void tile_body(const PairKernelArgs& a_in, int tix) {
         CENTRE_EPI(3, 0) CENTRE_EPI(3, 1) CENTRE_EPI(3, 2) CENTRE_EPI(3, 3)
 #undef CENTRE_EPI
         __syncthreads();
-        const double n = (double)(s + 1) * (double)a.ss;
-        const double logn = log(n);
         for (int q = fused ? tid : nr * nc; q < nr * nc; q += 256) {
             const int rd = q / nc, cd = q - rd * nc;
             const long long gi = r0 + rd, gj = c0 + cd;
@@ -420,12 +422,13 @@ __device__ __forceinline__ void tile_body(const PairKernelArgs& a_in, int tix) {
                 const long long p = a.tilemap ? (long long)a.tilemap[(size_t)tix * R * R + rd * R + cd]
                                               : tri_index(gi, gj, D);
                 if (p >= 0) {
-                    double T = 0.0;
+                    unsigned long long Tq = 0ull;
                     int nz = 0;
                     for (int k = 0; k < nstr; ++k) {
-                        T += sm.slotT[(rd * R + cd) * nstr + k];
+                        Tq += sm.slotT[(rd * R + cd) * nstr + k];
                         nz += sm.slotN[(rd * R + cd) * nstr + k];
                     }
+                    const double T = from_fixed(Tq, kfix);
                     const double L = a.dof_L[gi * S + s] + a.dof_L[gj * S + s];
                     const double area = a.dof_w[gi] * a.dof_w[gj];
                     const double ml = (logn + log(area)) - T / n + L / n;
@@ -439,7 +442,7 @@ __device__ __forceinline__ void tile_body(const PairKernelArgs& a_in, int tix) {
                 }
             }
             for (int k = 0; k < nstr; ++k) {
-                sm.slotT[(rd * R + cd) * nstr + k] = 0.0;
+                sm.slotT[(rd * R + cd) * nstr + k] = 0ull;
                 sm.slotN[(rd * R + cd) * nstr + k] = 0;
             }
         }

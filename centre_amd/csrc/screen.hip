@@ -9,7 +9,10 @@
 //   2     circular correlation of the bin angles theta = 2 pi (code + 1/2) / nb
 //         on a frame subsample (Jammalamadaka-SenGupta)
 //   3     linear (Pearson) correlation of the codes on the subsample
-//   4     MI on the subsample (ML plug-in, same bins; computed by the pair kernel)
+//   4     MI on the subsample (ML plug-in) with the bins merged to
+//         nbc = min(nb, 8): code * nbc / nb; computed by the pair kernel's
+//         small-bin tile (plug-in bias (nbc - 1)^2 / (2 n_sub) instead of
+//         (nb - 1)^2 / (2 n_sub))
 //   5, 6  DOF types of i and j (1 bond, 2 angle, 3 torsion, scaled by 1/3)
 //   7     1 when i and j have the same type
 // Network: x = (f - mu) / sd, h1 = relu(W1 x + b1), h2 = relu(W2 h1 + b2),
@@ -18,12 +21,19 @@
 
 namespace {
 
+// Frames k * step of every DOF; with coarse != nullptr also the merged codes
+// code * nbc / nb.
 __global__ __launch_bounds__(256) void subsample_kernel(const uint8_t* __restrict__ codes,
                                                         uint64_t stride, uint64_t step,
-                                                        uint32_t n_sub, uint8_t* __restrict__ out) {
+                                                        uint32_t n_sub, uint8_t* __restrict__ out,
+                                                        uint8_t* __restrict__ coarse, int nb,
+                                                        int nbc) {
     const uint32_t d = blockIdx.y;
     const uint32_t k = blockIdx.x * 256 + threadIdx.x;
-    if (k < n_sub) out[(uint64_t)d * n_sub + k] = codes[(uint64_t)d * stride + (uint64_t)k * step];
+    if (k >= n_sub) return;
+    const uint8_t c = codes[(uint64_t)d * stride + (uint64_t)k * step];
+    out[(uint64_t)d * n_sub + k] = c;
+    if (coarse) coarse[(uint64_t)d * n_sub + k] = (uint8_t)((int)c * nbc / nb);
 }
 
 // Per DOF over the subsample: standardised linear values X and circular
@@ -389,9 +399,10 @@ __global__ void fill_f32(float* p, uint64_t n, float v) {
 }  // namespace
 
 hipError_t launch_subsample(const uint8_t* codes, uint64_t stride, uint32_t D, uint64_t step,
-                            uint32_t n_sub, uint8_t* out, hipStream_t st) {
+                            uint32_t n_sub, uint8_t* out, uint8_t* coarse, int nb, int nbc,
+                            hipStream_t st) {
     hipLaunchKernelGGL(subsample_kernel, dim3((n_sub + 255) / 256, D), dim3(256), 0, st, codes,
-                       stride, step, n_sub, out);
+                       stride, step, n_sub, out, coarse, nb, nbc);
     return hipGetLastError();
 }
 

@@ -58,21 +58,29 @@ extern "C" int centre_screen_features(centre_ctx* ctx, const uint8_t* d_codes, u
     if (int r = centre_dev_dof_entropy(ctx, d_codes, code_stride, n_dof, n_frame, d_types,
                                        d_extrema, nbins, 1, &ml, 1, 1, 0, s1.as<double>(), nullptr))
         return r;
-    // evenly strided frame subsample
-    Buf sub;
+    // evenly strided frame subsample, and its codes merged to nbc bins
+    const int nbc = std::min(nbins, kScreenMiBins);
+    Buf sub, coarse;
     SCHK(sub.alloc(D * n_sub));
+    SCHK(coarse.alloc(D * n_sub));
     SCHK(launch_subsample(d_codes, code_stride, n_dof, n_frame / n_sub, n_sub, sub.as<uint8_t>(),
-                          ctx->stream));
-    // MI on the subsample through the pair kernel (one step, ML)
-    Buf s2, mi;
+                          coarse.as<uint8_t>(), nbins, nbc, ctx->stream));
+    // MI of the merged codes through the pair kernel (one step, ML, no
+    // Jacobian: widths and Jacobians cancel in MI, unit widths keep S2 finite)
+    Buf s2, mi, unit;
     SCHK(s2.alloc(sizeof(double) * P));
     SCHK(mi.alloc(sizeof(double) * P));
+    SCHK(unit.alloc(sizeof(double) * 4 * D));
     {
+        std::vector<double> ext(4 * D, 0.0);
+        for (uint64_t d = 0; d < D; ++d) ext[4 * d + 1] = (double)nbc;
+        SCHK(hipMemcpyAsync(unit.p, ext.data(), sizeof(double) * ext.size(),
+                            hipMemcpyHostToDevice, ctx->stream));
         centre_pair_plan* plan = nullptr;
-        if (int r = centre_pair_plan_create(ctx, n_dof, nbins, nullptr, 0, &plan)) return r;
-        const int r = centre_dev_pair_entropy(ctx, plan, sub.as<uint8_t>(), n_sub, n_sub, d_types,
-                                              d_extrema, 1, &ml, 1, 1, 0, nullptr,
-                                              s2.as<double>(), mi.as<double>(), nullptr);
+        if (int r = centre_pair_plan_create(ctx, n_dof, nbc, nullptr, 0, &plan)) return r;
+        const int r = centre_dev_pair_entropy(ctx, plan, coarse.as<uint8_t>(), n_sub, n_sub,
+                                              d_types, unit.as<double>(), 1, &ml, 1, 0, 0,
+                                              nullptr, s2.as<double>(), mi.as<double>(), nullptr);
         centre_pair_plan_destroy(plan);
         if (r) return r;
     }

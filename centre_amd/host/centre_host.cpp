@@ -722,16 +722,39 @@ void Host::run_entropy() {
     times["pair_kernels"] = kms / 1000.0;
 }
 
-std::vector<double> contri_of(const std::string& path, int est, int sch, int step,
-                              std::vector<uint32_t>& ids, int& order) {
+// One entropy-contribution file read whole: ids [records][order] and
+// contri_S [records][estimators][schemes][steps].  The scorer takes every
+// (estimator, scheme, step) slice from it, so each file is opened and read
+// once (a NetCDF-4 file's `id` is stored one record per chunk, which makes a
+// read per slice as slow as the number of records).
+struct Contri {
+    std::vector<uint32_t> ids;
+    std::vector<double> S;
+    std::vector<uint64_t> shape;
+    int order = 0;
+};
+
+const Contri& contri_file(std::map<std::string, Contri>& cache, const std::string& path) {
+    auto it = cache.find(path);
+    if (it != cache.end()) return it->second;
     auto r = ncf::open(path);
-    auto shp = r->shape("contri_S");
-    order = (int)r->shape("id")[1];
-    ids = r->read_u32("id");
-    std::vector<double> v(shp[0]);
-    if (shp[0] == 0) return v;
-    return r->read_f64("contri_S", {0, (uint64_t)est, (uint64_t)sch, (uint64_t)step},
-                       {shp[0], 1, 1, 1});
+    Contri c;
+    c.shape = r->shape("contri_S");
+    c.order = (int)r->shape("id")[1];
+    c.ids = r->read_u32("id");
+    if (c.shape[0]) c.S = r->read_f64("contri_S");
+    return cache.emplace(path, std::move(c)).first->second;
+}
+
+std::vector<double> contri_of(std::map<std::string, Contri>& cache, const std::string& path,
+                              int est, int sch, int step, std::vector<uint32_t>& ids, int& order) {
+    const Contri& c = contri_file(cache, path);
+    ids = c.ids;
+    order = c.order;
+    const uint64_t n = c.shape[0], ne = c.shape[1], ns = c.shape[2], S = c.shape[3];
+    std::vector<double> v(n);
+    for (uint64_t r = 0; r < n; ++r) v[r] = c.S[((r * ne + est) * ns + sch) * S + step];
+    return v;
 }
 
 // EntropyScorer::run (EntropyScorer.cpp:7-875)
@@ -746,6 +769,7 @@ void Host::run_scoring() {
         return n;
     }();
     const uint64_t step_size = F_used / S;
+    std::map<std::string, Contri> cache;
     for (size_t e = 0; e < in.estimators.size(); ++e) {
         for (size_t sc = 0; sc < schemes.size(); ++sc) {
             const std::string suffix = std::string(EST_NAME[in.estimators[e]]) + "_convergence_b-" +
@@ -772,7 +796,7 @@ void Host::run_scoring() {
                     const std::string p = out(std::string("entcontri_") + ONE_TAG[ty] + "-1d.nc");
                     if (!file_exists(p)) continue;
                     int order;
-                    S1v[ty] = contri_of(p, (int)e, (int)sc, st, id1[ty], order);
+                    S1v[ty] = contri_of(cache, p, (int)e, (int)sc, st, id1[ty], order);
                     for (size_t k = 0; k < id1[ty].size(); ++k) idx1[ty][id1[ty][k]] = (uint32_t)k;
                     for (double v : S1v[ty]) sum1[ty] += v;
                 }
@@ -787,7 +811,7 @@ void Host::run_scoring() {
                     const std::string p = out(std::string("entcontri_") + KINDS[k].tag + "-2d.nc");
                     if (!file_exists(p)) continue;
                     int order;
-                    auto S2 = contri_of(p, (int)e, (int)sc, st, ik[k], order);
+                    auto S2 = contri_of(cache, p, (int)e, (int)sc, st, ik[k], order);
                     mik[k].resize(S2.size());
                     for (size_t r = 0; r < S2.size(); ++r) {
                         const uint32_t a = idx1[KINDS[k].a].at(ik[k][2 * r]);

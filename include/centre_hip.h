@@ -203,7 +203,10 @@ int centre_pair_entropy(centre_ctx *ctx, const uint8_t *codes,
  * Replaces boruvkaMST (Centre.cpp:5-143) as called by EntropyScorer::run
  * (EntropyScorer.cpp:632-741): maximum spanning forest under the total order
  * (weight descending, edge index ascending).  tree_edge_idx receives the
- * forest's edge indices sorted by that order; *n_tree their count. */
+ * forest's edge indices sorted by that order; *n_tree their count.  Edges
+ * whose weight is NaN are left out (no counterpart: the reference has no
+ * missing pairs; a screened pass leaves the pairs it skipped NaN, so its MI
+ * array in triangle order is the edge list as it stands). */
 int centre_dev_mst(centre_ctx *ctx, const uint32_t *d_src,
                    const uint32_t *d_dst, const double *d_w, uint64_t n_edge,
                    uint32_t n_node, uint32_t *tree_edge_idx,
@@ -286,8 +289,10 @@ int centre_batch_pair_entropy(centre_ctx *ctx, const centre_batch_system *sys,
  *
  * centre_screen_features: d_feat[P][8] (device, FP32) for all P = D(D-1)/2
  * pairs i < j in triangle order: S1_i, S1_j (ML, all n_frame frames),
- * circular and linear correlation and ML MI on n_sub evenly strided frames
- * (n_sub a multiple of 4), type_i / 3, type_j / 3, same-type flag.
+ * circular and linear correlation on n_sub evenly strided frames (n_sub a
+ * multiple of 4), ML MI on the same frames with the bins merged to
+ * min(nbins, 8) (code * nbc / nbins), type_i / 3, type_j / 3, same-type
+ * flag.
  * centre_mlp_forward: d_out[n] = predicted log(MI).
  * centre_mlp_loss_grad: d_grad = gradient of sum 0.5 (y - t)^2 over the n
  * samples (FP32, accumulated with atomics), *loss = that sum.

@@ -10,7 +10,8 @@ gradient.
 Features of pair (i < j), as the device computes them:
   S1_i, S1_j (ML over all frames, one step, Jacobian on), circular and linear
   correlation of the codes on an evenly strided subsample, ML MI on the
-  subsample, type_i / 3, type_j / 3, same-type flag.
+  subsample with the bins merged to nbc = min(nb, 8) (code * nbc // nb),
+  type_i / 3, type_j / 3, same-type flag.
 """
 from __future__ import annotations
 
@@ -77,8 +78,12 @@ def features(codes, types, ext, nb, n_sub):
     sub = np.ascontiguousarray(codes[:, ::step][:, :n_sub])
     iu, ju = np.triu_indices(D, 1)
     pairs = np.stack([iu, ju], 1).astype(np.uint32)
-    s2 = _o.pair_entropies(sub, types, ext, pairs, 1, schemes=(nb,))[:, 0, 0, 0]
-    s1s = _o.dof_entropies(sub, types, ext, 1, schemes=(nb,))[:, 0, 0, 0]
+    nbc = min(nb, 8)
+    merged = (sub.astype(np.int64) * nbc // nb).astype(np.uint8)
+    unit = np.tile([0.0, float(nbc), 0.0, 0.0], (D, 1))
+    s2 = _o.pair_entropies(merged, types, unit, pairs, 1, schemes=(nbc,),
+                           jacobian=False)[:, 0, 0, 0]
+    s1s = _o.dof_entropies(merged, types, unit, 1, schemes=(nbc,), jacobian=False)[:, 0, 0, 0]
     mi = s1s[iu] + s1s[ju] - s2
     c = sub.astype(np.float64)
     xc = c - c.mean(1, keepdims=True)

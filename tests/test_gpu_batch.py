@@ -67,13 +67,15 @@ def test_batch_matches_single_system_path(nb, ests):
         assert rused == shape if shape else rused in (R, R - 1)
         assert tot > 0 and np.all(sys_ms > 0) and np.all(sys_ms <= tot * 1.05 + 0.05)
         for (a1, a2, am), o in zip(want, outs):
-            # S1 comes from the same sequential per-DOF sums: identical.  Pair
-            # sums are FP64 LDS atomics whose order varies from run to run and
-            # between the batch kernel and the eight-wave tile kernel, so S2
-            # and MI agree to a few ulp of the O(1) terms summed.
+            # S1 comes from the same sequential per-DOF sums: identical.  The
+            # pair sums are exact integer sums of c log c held in a fixed
+            # point of 2^-K, K set by the largest frame count of the launch;
+            # a batch takes K from its largest system, so a smaller system's
+            # table entries are rounded at a coarser K than in its own call:
+            # at most 2,500 x 2^-K apart in T, ~1e-13 of S2 for these sizes.
             np.testing.assert_array_equal(o[0].numpy(), a1)
-            np.testing.assert_allclose(o[1].numpy(), a2, rtol=1e-12, atol=1e-14)
-            np.testing.assert_allclose(o[2].numpy(), am, rtol=1e-12, atol=1e-14)
+            np.testing.assert_allclose(o[1].numpy(), a2, rtol=1e-12, atol=1e-12)
+            np.testing.assert_allclose(o[2].numpy(), am, rtol=1e-12, atol=1e-12)
     ctx.close()
 
 

@@ -187,6 +187,31 @@ def test_small_bin_tiles_vs_oracle(ctx, nb, D, F, S, ests):
     assert rel_err(s2, w2[:, :, 0, :]) < REL
 
 
+@pytest.mark.parametrize("nb,D", [(50, 160), (20, 300), (6, 600)])
+def test_pair_stage_is_deterministic(ctx, nb, D):
+    """The same inputs give bit-identical S2 and MI on every call: the
+    per-pair c log c sums are integers in LDS (fixed point), so the order in
+    which lanes and waves add them does not matter.  Eight-wave tile
+    (nb 50), four-wave tile (nb 20) and small-bin tile (nb 6), each with
+    hundreds of concurrently running tiles; checked against the oracle too."""
+    rng = np.random.default_rng(nb)
+    F, S = 12000, 6
+    codes = rng.integers(0, nb, size=(D, F), dtype=np.uint8)
+    codes[1::2] = (codes[0::2][: len(codes[1::2])] + rng.integers(0, 2, size=F)) % nb
+    types = np.array([1, 2, 3] * D, np.uint8)[:D]
+    ext = np.zeros((D, 4))
+    ext[:, 0] = rng.uniform(0.5, 1.0, D)
+    ext[:, 1] = ext[:, 0] + rng.uniform(0.2, 1.5, D)
+    runs = [ctx.pair_entropy(codes, types, ext, nb, S, ests=(0, 1)) for _ in range(4)]
+    for s1, s2, mi in runs[1:]:
+        assert np.array_equal(s2.view(np.uint64), runs[0][1].view(np.uint64))
+        assert np.array_equal(mi.view(np.uint64), runs[0][2].view(np.uint64))
+    iu, ju = np.triu_indices(D, 1)
+    pick = np.linspace(0, len(iu) - 1, 60).astype(np.int64)
+    w2 = O.pair_entropies(codes, types, ext, np.stack([iu[pick], ju[pick]], 1), S, [nb], (0, 1))
+    assert rel_err(runs[0][1][pick], w2[:, :, 0, :]) < REL
+
+
 @pytest.mark.parametrize("ests", [(0, 1), (0,)])
 @pytest.mark.parametrize("nb,D,F,S", [(50, 20, 4000, 4), (42, 18, 2500, 5), (36, 14, 3333, 3),
                                       (50, 23, 6002, 2), (29, 17, 1500, 2), (31, 16, 2000, 3),
@@ -389,6 +414,31 @@ def test_mst_more_nodes_than_the_lds_table(ctx):
     want = O.boruvka(n, src, dst, w)
     assert sorted(got.tolist()) == sorted(want.tolist())
     assert got.tolist() == kruskal_set(n, src.tolist(), dst.tolist(), w.tolist())
+
+
+def test_mst_nan_weights_are_absent_edges(ctx):
+    """Edges of NaN weight (pairs a screened pass skipped) are left out: the
+    forest equals the oracle's Boruvka over the other edges, in the original
+    edge indices, on a complete graph and on a sparse one that the NaN edges
+    would otherwise connect."""
+    rng = np.random.default_rng(23)
+    for n, dense in ((700, True), (5000, False)):
+        if dense:
+            src, dst = np.triu_indices(n, 1)
+        else:
+            src = rng.integers(0, n, 40000)
+            dst = rng.integers(0, n, 40000)
+            keep = src != dst
+            src, dst = src[keep], dst[keep]
+        w = rng.integers(0, 60, len(src)).astype(float) / 7.0
+        w[rng.random(len(src)) < 0.6] = np.nan
+        got = ctx.mst(n, src, dst, w)
+        ok = np.flatnonzero(~np.isnan(w))
+        want = ok[O.boruvka(n, src[ok], dst[ok], w[ok])]
+        assert not np.isnan(w[got]).any()
+        assert sorted(got.tolist()) == sorted(want.tolist())
+        assert got.tolist() == ok[kruskal_set(n, src[ok].tolist(), dst[ok].tolist(),
+                                               w[ok].tolist())].tolist()
 
 
 def test_mst_endpoint_out_of_range_is_an_error(ctx):

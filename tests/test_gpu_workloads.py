@@ -138,7 +138,7 @@ def _mist(ctx, D, i, j, w, s1sum):
 
 def test_screen_workload_6000_dof():
     nb, S, F, D = 50, 10, 100000, 6000
-    n_train, n_hold, n_sub, topk = 6, 2, 2000, 2
+    n_train, n_hold, n_sub, topk = 6, 2, 8000, 2
     P = D * (D - 1) // 2
     R = int(hip.lib().centre_pair_dofs_per_tile(nb))
     with hip.Context(0) as ctx:

@@ -64,8 +64,11 @@ def test_oracle_features_closed_forms():
     assert f[0, 2] == pytest.approx(1.0, abs=1e-12)          # identical -> circular r = 1
     assert abs(f[2, 3]) < 0.1 and abs(f[2, 2]) < 0.1         # independent -> small
     # MI of a DOF with itself on the subsample = its (ML, no Jacobian) entropy
+    # over the bins merged to 8 (code * 8 // 10: bins 0-1 and 5-6 merge)
     sub = a[::4][:1000]
-    p = np.bincount(sub, minlength=nb) / 1000.0
+    merged = sub.astype(int) * 8 // nb
+    assert sorted(set((np.arange(nb) * 8 // nb).tolist())) == list(range(8))
+    p = np.bincount(merged, minlength=8) / 1000.0
     h = -sum(v * np.log(v) for v in p if v > 0)
     assert f[0, 4] == pytest.approx(h, rel=1e-10)
     assert f[0, 7] == 1.0 and f[2, 7] == 0.0
