@@ -529,6 +529,7 @@ def batch_workload(args, hip):
                             mi=ctx.alloc((P, S), np.float64)))
     t_gen = time.perf_counter() - t0
     n_pairs = sum(s["P"] for s in systems)
+    int8_ops = float(sum(2.0 * nb * nb * s["F"] * s["P"] for s in systems))
     # one launch per system (the plain path, system after system)
     ctx.device_sync()
     t0 = time.perf_counter()
@@ -583,6 +584,10 @@ def batch_workload(args, hip):
                                   "kernel_ms_sum": seq_kernel_ms},
         "speedup_vs_per_system": t_seq / t_batch,
         "kernel_speedup_vs_per_system": seq_kernel_ms / kern_ms,
+        # int8 roofline of the batch kernel: 2 nb^2 F ops per pair, every system
+        "roofline": {"bound": "mfma", "ops": int8_ops,
+                     "achieved": int8_ops / (kern_ms * 1e-3) / 1e12, "peak": 5000.0,
+                     "unit": "TFLOP/s", "frac": int8_ops / (kern_ms * 1e-3) / 5.0e15},
         "results_match_per_system_path": bool(same),
         "mist_all_systems_s": t_mist,
         "generate_and_discretise_s": t_gen,
