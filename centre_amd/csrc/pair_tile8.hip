@@ -371,12 +371,18 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     const int nch = (int)((ss + kChunk - 1) / kChunk);
     const int G = S * nch;
     const int nlive = (diag ? 1 : 2) * R * kGroups;
-    // this lane's slot: DOF ld, frames k4..k4+3 of each chunk.  The first
-    // waves stage one slot per lane (lanes past nlive repeat live slots of
-    // their own wave); spreading the slots over all eight waves instead
-    // (nlive/8 lanes each) ran 20% slower.
-    const bool stager = kSmall ? wave * 64 < (diag ? 1 : 2) * R * 4 : wave * 64 < nlive;
-    const int q = (wave * 64 + lane) % nlive;
+    // this lane's slot: DOF ld, frames k4..k4+3 of each chunk.  Waves run on
+    // SIMD wave % 4.  The stagers are waves 1, 2, 3 (and 0 when nlive > 192),
+    // one slot per lane (lanes past nlive repeat live slots); SIMD 0 keeps
+    // two multiply-only waves, and one of them, wave 0 (else wave 4),
+    // finalizes each step's sums.  Against stagers 0, 1, 2 with wave 7
+    // finalizing: 24.9 against 25.8 ms on the headline tiles; two stagers on
+    // one SIMD (waves 0, 2, 4) took 32.1 ms, and spreading the slots over all
+    // eight waves (nlive/8 lanes each) ran 20% slower.
+    const int swave = kSmall ? wave : (wave < 4 ? (wave + 3) & 3 : wave);
+    const bool stager = kSmall ? wave * 64 < (diag ? 1 : 2) * R * 4 : swave * 64 < nlive;
+    const int q = (swave * 64 + lane) % nlive;
+    const int fwave = nlive > 192 ? 4 : 0;   // finalizer, never a stager
     const int ld = q / kGroups, k4 = (q - ld * kGroups) * 4;
     const bool isA = ld < R;
     // rows past the system's last DOF (a partial last block) read the last
@@ -597,7 +603,7 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
                 __builtin_amdgcn_sched_barrier(0);
             }
     };
-    // Slot sums of step fs -> S2 / MI, by wave 7 (never a stager; on the
+    // Slot sums of step fs -> S2 / MI, by wave fwave (never a stager; on the
     // small-bin tile by every wave) one iteration after the epilogue, so
     // that it overlaps the next chunk; the iteration barrier orders it after
     // the atomics, and the next epilogue's opening barrier after it.
@@ -701,7 +707,7 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
             if constexpr (kSmall) {
                 if (fin >= 0) finalize(fin);
             } else if constexpr (!kStg) {
-                if (fin >= 0 && wave == 7) finalize(fin);
+                if (fin >= 0 && wave == fwave) finalize(fin);
             }
             fin = -1;
             advance(ps, pc, nch);
@@ -724,7 +730,7 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     };
     if (stager) run(std::true_type{});
     else run(std::false_type{});
-    if (fin >= 0 && (kSmall || wave == 7)) finalize(fin);
+    if (fin >= 0 && (kSmall || wave == fwave)) finalize(fin);
 }
 
 template <bool kNz, bool kSmall>
