@@ -55,8 +55,8 @@ def run_both(ctx, D, F, nb, S, ests, stride, pairs):
 
 # (nb, D, F, S, ests, code row stride, explicit pairs): all-pairs plans on the
 # eight-wave tile, a misaligned stride (step-aligned copy in the scratch), an
-# explicit list (canonical outputs in the scratch), and a small-nb plan with
-# few tiles (frame-split dumps in the scratch)
+# explicit list (canonical outputs in the scratch), and a small-nb plan on the
+# small-bin tile
 CASES = [
     (50, 64, 20000, 4, (0, 1), 20000, False),
     (50, 23, 9001, 3, (1,), 9003, False),
