@@ -18,18 +18,31 @@
  *                                      all 75 S1 and 2533 S2 records x 4
  *                                      steps, ML + Jacobian) and the
  *                                      example-2 stored counts; MM/CS/JS by
- *                                      closed forms (uniform and delta
- *                                      distributions) in tests.
+ *                                      closed forms: uniform and delta
+ *                                      distributions, and 40-digit
+ *                                      evaluations of Chao-Shen with
+ *                                      singletons (1-D, 2-D with the row-bin
+ *                                      Jacobian, prefix steps), 2-D
+ *                                      James-Stein (target 1/B) and 2-D
+ *                                      Miller-Madow, each also showing the
+ *                                      plausible slip fails.
  *   or_bnd/ang_real2int              : closed form, invariants and the
  *                                      example-1 stored extrema ranges.
  *   or_dih_real2int                  : hand-built arcs with known gaps.
  *                                      On real data: parity unpinned (the
  *                                      reference ships no BAT inputs).
  *   or_bessel_i                      : Abramowitz & Stegun tables (scipy).
- *   or_vmkde                         : planted multi-modal data with known
- *                                      maxima; stored minima only bound
- *                                      plausibility.  Parity unpinned on
- *                                      real data.
+ *   or_vmkde                         : the grid against the analytic von
+ *                                      Mises derivatives (one frame), their
+ *                                      mean (several frames) and its
+ *                                      invariants (periodicity, zero
+ *                                      integrals, second derivative as the
+ *                                      derivative of the first); the states
+ *                                      against planted multi-modal data and
+ *                                      a mirrored mixture with exact minima;
+ *                                      stored minima only bound
+ *                                      plausibility.  No stored trajectory
+ *                                      reaches it with real data.
  *   or_boruvka                       : Kruskal under the same total order,
  *                                      golden tree CSVs (example-1/2).
  */
