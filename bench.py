@@ -5,8 +5,8 @@ Workload (the headline configuration): 600 DOF (200 bonds, 200 angles,
 histogram discretisation with 50 bins, 10 convergence steps, ML estimator
 with the Jacobian, all 179,700 DOF pairs.  One step = one pass of the pair
 stage from device-resident uint8 codes to per-pair S2 (10 steps each) and MI
-in HBM: marginal counts and S1, the one-hot int8 MFMA joint counts with the
-fused FP64 entropy epilogue, and MI.  Discretisation and MIST run once
+in HBM: marginal counts and S1, the one-hot MFMA joint counts (FP4 e2m1
+operands, exact f32 counts) with the fused FP64 entropy epilogue, and MI.  Discretisation and MIST run once
 outside the timed region and are reported as separate stage times.
 
 Prints ONE JSON line (rank 0).  --gpus N > 1 is refused: the library runs
@@ -25,6 +25,16 @@ import numpy as np
 
 
 METRIC = "pair entropies/sec @100k frames, 50 bins; screened vs plain at <=0.1% S error"
+
+# Dense MFMA peaks of the MI355X (TFLOP/s, or TOP/s for int8).
+PEAK = {"fp4": 10100.0, "int8": 5000.0}
+
+
+def mfma_form(n_max):
+    """Operand format of the pair tiles for a launch whose largest joint count
+    is n_max (steps x frames a step): FP4 while f32 accumulation is exact
+    (n_max <= 2^24), else int8 (the library's default rule)."""
+    return "fp4" if n_max <= (1 << 24) else "int8"
 
 
 def parse():
@@ -223,7 +233,8 @@ def main():
     kernel_ms = statistics.mean(kms)
     ops = 2.0 * nb * nb * (S * (F // S)) * P
     achieved = ops / (kernel_ms / 1000.0) / 1e12
-    peak = 5000.0  # MI355X dense int8 MFMA, TOP/s
+    form = mfma_form(S * (F // S))
+    peak = PEAK[form]
 
     # ---- CPU baseline: the oracle's per-pair loop on a bounded sample -------
     cpu = None
@@ -261,7 +272,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int8",
+        "dtype": form,
         "data": "synthetic (seeded Philox BAT generator, latent-coupled)",
         "config": {"workload": f"full pairwise MIE pair stage, {D} DOF "
                                f"({nbd}B/{nan}A/{ndh}T) x {F} frames, {nb} bins, "
@@ -529,7 +540,8 @@ def batch_workload(args, hip):
                             mi=ctx.alloc((P, S), np.float64)))
     t_gen = time.perf_counter() - t0
     n_pairs = sum(s["P"] for s in systems)
-    int8_ops = float(sum(2.0 * nb * nb * s["F"] * s["P"] for s in systems))
+    mm_ops = float(sum(2.0 * nb * nb * s["F"] * s["P"] for s in systems))
+    form = mfma_form(S * max(F // S for _, F in dims))
     # one launch per system (the plain path, system after system)
     ctx.device_sync()
     t0 = time.perf_counter()
@@ -584,10 +596,11 @@ def batch_workload(args, hip):
                                   "kernel_ms_sum": seq_kernel_ms},
         "speedup_vs_per_system": t_seq / t_batch,
         "kernel_speedup_vs_per_system": seq_kernel_ms / kern_ms,
-        # int8 roofline of the batch kernel: 2 nb^2 F ops per pair, every system
-        "roofline": {"bound": "mfma", "ops": int8_ops,
-                     "achieved": int8_ops / (kern_ms * 1e-3) / 1e12, "peak": 5000.0,
-                     "unit": "TFLOP/s", "frac": int8_ops / (kern_ms * 1e-3) / 5.0e15},
+        # MFMA roofline of the batch kernel: 2 nb^2 F ops per pair, every system
+        "dtype": form,
+        "roofline": {"bound": "mfma", "ops": mm_ops,
+                     "achieved": mm_ops / (kern_ms * 1e-3) / 1e12, "peak": PEAK[form],
+                     "unit": "TFLOP/s", "frac": mm_ops / (kern_ms * 1e-3) / 1e12 / PEAK[form]},
         "results_match_per_system_path": bool(same),
         "mist_all_systems_s": t_mist,
         "generate_and_discretise_s": t_gen,
@@ -677,7 +690,7 @@ def vmkde_workload(args, hip):
         "value": P / tp, "unit": "neighbour pair entropies/s", "higher_is_better": True,
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": (td + tp + tm) * 1e3, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64 (KDE, entropies); int8 one-hot counts",
+        "dtype": "f64 (KDE, entropies); fp4 one-hot counts",
         "data": "synthetic (seeded torsion mixtures, latent-coupled)",
         "config": {"workload": "3,000 torsions x 50,000 frames, vmKDE kappa 10, 64-interval grid, "
                                "nmaxconf 5, neighbours within 30 DOF, 10 steps, ML",
@@ -806,7 +819,7 @@ def host_workload(args, hip):
     out = {
         "metric": "pair entropies/sec end to end through the C++ host, file I/O included",
         "value": P / t_run, "unit": "pair entropies/s", "higher_is_better": True,
-        "n_gpus": 1, "scaling": "weak", "vs_baseline": None, "dtype": "int8",
+        "n_gpus": 1, "scaling": "weak", "vs_baseline": None, "dtype": mfma_form(F),
         "data": "synthetic (seeded Philox BAT generator, latent-coupled), CDF-5 BAT file",
         "config": {"workload": f"centre_hip -i centre.cfg: discretise + entropy + MIE/MIST, {D} DOF "
                                f"({nbd}B/{nan}A/{ndh}T) x {F} frames, {nb} bins, {S} steps, ML",

@@ -153,6 +153,13 @@ int centre_sync(centre_ctx* ctx) {
 
 void* centre_stream(centre_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
+extern "C" int centre_set_mfma_format(centre_ctx* ctx, int format) {
+    if (!ctx) return -3;
+    if (format != 0 && format != 1) return fail(ctx, -3, "mfma format must be 0 (FP4 where exact) or 1 (int8)");
+    ctx->int8_only = format;
+    return 0;
+}
+
 float centre_last_pair_kernel_ms(const centre_ctx* ctx) {
     return ctx ? ctx->last_pair_ms : 0.f;
 }
@@ -655,6 +662,7 @@ int batch_launch(centre_ctx* ctx, const PairKernelArgs& base, const std::vector<
     uint64_t ss_max = 1;
     for (const BatchSystem& b : bs) ss_max = std::max<uint64_t>(ss_max, b.ss);
     a.kfix = pair_fixed_k((double)base.nsteps * (double)ss_max);
+    a.int8_only = ctx->int8_only || !pair_f4_exact((double)base.nsteps * (double)ss_max);
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
     if (w8) HIPCHK(launch_pair_batch8(a, grid, ctx->stream));
     else HIPCHK(launch_pair_persistent(a, grid, ctx->stream));
@@ -897,6 +905,7 @@ int run_pair_fused(centre_ctx* ctx, const centre_pair_plan* plan, const uint8_t*
     }
     const bool all = plan->all_pairs;
     PairKernelArgs a{};
+    a.int8_only = ctx->int8_only;
     if (int r = stage_codes(ctx, d_codes, code_stride, D, ss, nsteps, ws.packed, &a.codes,
                             &a.stride, &a.sstride))
         return r;

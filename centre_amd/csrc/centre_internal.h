@@ -28,6 +28,9 @@ struct centre_ctx {
     // device address of the library's static scratch array on this device
     // (the allocation-free pair-stage entry point carves it)
     void* noalloc = nullptr;
+    // eight-wave pair tile operands (centre_set_mfma_format): 0 = FP4 where
+    // the counts stay exact, 1 = int8 always
+    int int8_only = 0;
 };
 
 struct centre_pair_plan {
@@ -141,8 +144,13 @@ struct PairKernelArgs {
     // eight-wave tile: c log c sums in units of 2^-kfix (pair_fixed_k of the
     // largest frame count a step of the launch reaches)
     int kfix;
+    // eight-wave tile: 1 = int8 one-hot operands; 0 = FP4 operands when
+    // pair_f4_exact holds for the launch's largest frame count, else int8
+    int int8_only;
 };
 int pair_fixed_k(double n_max);
+// FP4 one-hot counts accumulate in f32: exact while no count exceeds 2^24
+bool pair_f4_exact(double n_max);
 
 
 // Launchers (return hipError_t of the launch).

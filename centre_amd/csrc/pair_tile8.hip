@@ -7,6 +7,33 @@
 
 namespace {
 
+typedef int v8i __attribute__((ext_vector_type(8)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+// FP4 form of the tile (kF4): one-hot operands as e2m1 nibbles (0x2 = 1.0,
+// 0x0 = 0), two frames a byte (the even frame in the low nibble), multiplied
+// by v_mfma_scale_f32_32x32x64_f8f6f4 with both block scales 2^0 (E8M0 127):
+// a 64-frame chunk is one MFMA a block (the int8 form takes two, each in the
+// same issue time), read from the first 32 bytes of each LDS row.  Every product is 0 or 1 and the f32 accumulator
+// holds integers, exact up to 2^24, so the counts equal the int8 path's while
+// a pair's total frame count is at most 2^24 (the launch checks it).  A lane
+// holds 32 consecutive k of its row or column (k = 32 * (lane / 32) + t, t in
+// its 16 bytes low nibble first); the operands are laid out alike, so only the
+// pairing of A's and B's k matters, and it is the same frame on both sides.
+__device__ __forceinline__ v16f mfma_f4(const v4i& a, const v4i& b, const v16f& c) {
+    const v8i a8 = {a[0], a[1], a[2], a[3], 0, 0, 0, 0};
+    const v8i b8 = {b[0], b[1], b[2], b[3], 0, 0, 0, 0};
+    return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a8, b8, c, 4, 4, 0, 127, 0, 127);
+}
+
+__device__ __forceinline__ v16i counts_of(const v16i& c) { return c; }
+__device__ __forceinline__ v16i counts_of(const v16f& c) {
+    v16i r;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) r[k] = (int)c[k];
+    return r;
+}
+
 // ---- eight-wave kernel for full tiles --------------------------------------------
 // Tiles with R DOFs on both sides (diagonal tiles included: both operands
 // are then the same rows, and only pairs i < j are kept) and 4-byte aligned
@@ -299,8 +326,8 @@ __device__ __forceinline__ void tile8_tables(int R, int nb, int kfix) {
 }
 
 // kTables: also write the tile-independent LDS contents (tile8_tables);
-// false when the caller wrote them already.
-template <bool kBatch, bool kNz, bool kSmall, bool kTables = true>
+// false when the caller wrote them already.  kF4: the FP4 form (mfma_f4).
+template <bool kBatch, bool kNz, bool kSmall, bool kF4, bool kTables = true>
 __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) {
     extern __shared__ __align__(16) uint8_t smem_raw[];
     const PairKernelArgs& a = a_in;
@@ -368,10 +395,11 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
         }
     }
 
-    const int nch = (int)((ss + kChunk - 1) / kChunk);
+    constexpr int kCh = kChunk;   // frames a chunk
+    const int nch = (int)((ss + kCh - 1) / kCh);
     const int G = S * nch;
     const int nlive = (diag ? 1 : 2) * R * kGroups;
-    // this lane's slot: DOF ld, frames k4..k4+3 of each chunk.  Waves run on
+    // this lane's slot: DOF ld, frames kfr..kfr+3 of each chunk.  Waves run on
     // SIMD wave % 4.  The stagers are waves 1, 2, 3 (and 0 when nlive > 192),
     // one slot per lane (lanes past nlive repeat live slots); SIMD 0 keeps
     // two multiply-only waves, and one of them, wave 0 (else wave 4),
@@ -383,14 +411,17 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     const bool stager = kSmall ? wave * 64 < (diag ? 1 : 2) * R * 4 : swave * 64 < nlive;
     const int q = (swave * 64 + lane) % nlive;
     const int fwave = nlive > 192 ? 4 : 0;   // finalizer, never a stager
-    const int ld = q / kGroups, k4 = (q - ld * kGroups) * 4;
+    // kfr: the slot's first frame in the chunk; kb: its byte offset in its
+    // LDS rows (the FP4 form holds the 4 frames in 2 bytes)
+    const int ld = q / kGroups, kfr = (q - ld * kGroups) * 4;
+    const int kb = kF4 ? kfr / 2 : kfr;
     const bool isA = ld < R;
     // rows past the system's last DOF (a partial last block) read the last
     // row; their pairs are never written
     const long long grow = isA ? r0 + ld : c0 + (ld - R);
-    const uint8_t* const src = codes + (uint64_t)(grow < D ? grow : D - 1) * cstride + k4;
-    const uint32_t dst = (uint32_t)((isA ? 0 : kOpBytes) + (isA ? ld : ld - R) * nb * kLdsRow + k4);
-    const uint32_t trash_off = (uint32_t)(lay.map + 512 + 16 + k4);
+    const uint8_t* const src = codes + (uint64_t)(grow < D ? grow : D - 1) * cstride + kfr;
+    const uint32_t dst = (uint32_t)((isA ? 0 : kOpBytes) + (isA ? ld : ld - R) * nb * kLdsRow + kb);
+    const uint32_t trash_off = (uint32_t)(lay.map + 512 + 16 + kb);
     // what this slot set in the buffers staged one, two and three chunks
     // before the next (the buffer staged next is the one staged three ago)
     uint32_t set3 = 0u, set2 = 0u, set1 = 0u;
@@ -400,41 +431,41 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     // the system's last DOF read row 0 and stage no frames.
     const int nlive_s = (diag ? 1 : 2) * R * 4;
     const int qs = (wave * 64 + lane) % nlive_s;
-    const int sd = qs >> 2, g16 = (qs & 3) * 16;
+    const int sd = qs >> 2, g16 = (qs & 3) * 16, gb = kF4 ? g16 / 2 : g16;
     const bool sA = sd < R;
     const long long srow = sA ? r0 + sd : c0 + (sd - R);
     const bool srow_ok = srow < D;
     const uint8_t* const src_s = codes + (uint64_t)(srow_ok ? srow : 0) * cstride + g16;
-    const uint32_t dst_s = (uint32_t)((sA ? 0 : kOpBytes) + (sA ? sd : sd - R) * 8 * kLdsRow + g16);
+    const uint32_t dst_s = (uint32_t)((sA ? 0 : kOpBytes) + (sA ? sd : sd - R) * 8 * kLdsRow + gb);
     const bool stager_s = wave * 64 < nlive_s;   // wave-uniform
     int ls = 0, lc = 0;   // next chunk to load
     auto advance = [](int& s, int& c, int n) __attribute__((always_inline)) {
         if (++c == n) { c = 0; ++s; }
     };
-    const int nv_last = (int)(ss - (uint64_t)(nch - 1) * kChunk);   // a step's last chunk
+    const int nv_last = (int)(ss - (uint64_t)(nch - 1) * kCh);   // a step's last chunk
     auto nvalid_of = [&](int s, int c) __attribute__((always_inline)) -> int {
-        return s < S ? (c + 1 < nch ? kChunk : nv_last) : 0;
+        return s < S ? (c + 1 < nch ? kCh : nv_last) : 0;
     };
     // this slot's code dword of the next chunk to load (past the last chunk
     // a dword every row has, unused)
     auto load1 = [&]() __attribute__((always_inline)) -> uint32_t {
-        const uint64_t f0 = (uint64_t)ls * sstride + (uint64_t)lc * kChunk;
+        const uint64_t f0 = (uint64_t)ls * sstride + (uint64_t)lc * kCh;
         const int nv = nvalid_of(ls, lc);
         uint32_t v;
-        if (nv >= kChunk)   // wave-uniform: a full chunk, every lane's dword is in the step
+        if (nv >= kCh)   // wave-uniform: a full chunk, every lane's dword is in the step
             v = *reinterpret_cast<const uint32_t*>(src + f0);
         else
-            v = *reinterpret_cast<const uint32_t*>(k4 < nv ? src + f0 : codes);
+            v = *reinterpret_cast<const uint32_t*>(kfr < nv ? src + f0 : codes);
         advance(ls, lc, nch);
         return v;
     };
     // small-bin tile: the 16 code bytes of the next chunk to load (past the
     // last chunk or the step's valid frames, bytes of the first row, unused)
     auto load4 = [&]() __attribute__((always_inline)) -> uint4 {
-        const uint64_t f0 = (uint64_t)ls * sstride + (uint64_t)lc * kChunk;
+        const uint64_t f0 = (uint64_t)ls * sstride + (uint64_t)lc * kCh;
         const int nv = nvalid_of(ls, lc);
         uint32_t v[4];
-        if (nv >= kChunk) {   // wave-uniform: every dword is in the step
+        if (nv >= kCh) {   // wave-uniform: every dword is in the step
 #pragma unroll
             for (int d = 0; d < 4; ++d) v[d] = *reinterpret_cast<const uint32_t*>(src_s + f0 + 4 * d);
         } else {
@@ -450,6 +481,13 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     // permute selects byte `code` of an 8-byte table holding 1 at the bin,
     // and selector 12 (an invalid frame) reads 0.  Rows of bins past nb stay
     // zero, every other row is overwritten whole, so nothing is cleared.
+    //
+    // FP4 form: the item's 16 frames in 8 bytes a bin, two a byte.  Its even
+    // and its odd frames are first gathered into selector dwords (frames 8m,
+    // 8m+2, 8m+4, 8m+6 and 8m+1, ..., 8m+7 of code dwords 2m, 2m+1); a bin's
+    // byte j of dword m is then the permute of the even selector through a
+    // table holding 0x02 at the bin, or-ed with that of the odd selector
+    // through one holding 0x20: frames 8m+2j (low nibble) and 8m+2j+1.
     auto stage4 = [&](int b, int nv, uint4 w) __attribute__((always_inline)) {
         const int nvk = (srow_ok ? nv : 0) - g16;
         uint32_t x[4] = {w.x, w.y, w.z, w.w};
@@ -462,17 +500,37 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
             }
         }
         const uint32_t base = (uint32_t)(b * 2 * kOpBytes) + dst_s;
+        if constexpr (kF4) {
+            uint32_t ev[2], od[2];
 #pragma unroll
-        for (int bin = 0; bin < 8; ++bin) {
-            if (bin >= nb) break;   // uniform
-            const uint32_t lo = bin < 4 ? 1u << (8 * bin) : 0u;
-            const uint32_t hi = bin < 4 ? 0u : 1u << (8 * (bin - 4));
-            uint4 o;
-            o.x = __builtin_amdgcn_perm(hi, lo, x[0]);
-            o.y = __builtin_amdgcn_perm(hi, lo, x[1]);
-            o.z = __builtin_amdgcn_perm(hi, lo, x[2]);
-            o.w = __builtin_amdgcn_perm(hi, lo, x[3]);
-            *reinterpret_cast<uint4*>(smem_raw + base + bin * kLdsRow) = o;
+            for (int m = 0; m < 2; ++m) {
+                ev[m] = __builtin_amdgcn_perm(x[2 * m + 1], x[2 * m], 0x06040200u);
+                od[m] = __builtin_amdgcn_perm(x[2 * m + 1], x[2 * m], 0x07050301u);
+            }
+#pragma unroll
+            for (int bin = 0; bin < 8; ++bin) {
+                if (bin >= nb) break;   // uniform
+                const uint32_t lo = bin < 4 ? 2u << (8 * bin) : 0u;
+                const uint32_t hi = bin < 4 ? 0u : 2u << (8 * (bin - 4));
+                uint32_t o[2];
+#pragma unroll
+                for (int m = 0; m < 2; ++m)
+                    o[m] = __builtin_amdgcn_perm(hi, lo, ev[m]) | __builtin_amdgcn_perm(hi << 4, lo << 4, od[m]);
+                *reinterpret_cast<uint2*>(smem_raw + base + bin * kLdsRow) = make_uint2(o[0], o[1]);
+            }
+        } else {
+#pragma unroll
+            for (int bin = 0; bin < 8; ++bin) {
+                if (bin >= nb) break;   // uniform
+                const uint32_t lo = bin < 4 ? 1u << (8 * bin) : 0u;
+                const uint32_t hi = bin < 4 ? 0u : 1u << (8 * (bin - 4));
+                uint4 o;
+                o.x = __builtin_amdgcn_perm(hi, lo, x[0]);
+                o.y = __builtin_amdgcn_perm(hi, lo, x[1]);
+                o.z = __builtin_amdgcn_perm(hi, lo, x[2]);
+                o.w = __builtin_amdgcn_perm(hi, lo, x[3]);
+                *reinterpret_cast<uint4*>(smem_raw + base + bin * kLdsRow) = o;
+            }
         }
     };
     // chunk with nv valid frames (codes w) into buffer b: clear what this
@@ -490,13 +548,46 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
             for (int e = 0; e < 4; ++e) smem_raw[rowoff + ((w >> (8 * e)) & 0xFFu) * kLdsRow + e] = 1;
             return w;
         }
-        const int nvk = nv - k4;
+        const int nvk = nv - kfr;
         const uint32_t vmask = nvk >= 4 ? 0xFFFFFFFFu : (nvk <= 0 ? 0u : 0xFFFFFFFFu >> (8 * (4 - nvk)));
         const uint32_t wv = w & vmask;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const uint32_t o = nvk > e ? rowoff : trash_off;
             smem_raw[o + ((wv >> (8 * e)) & 0xFFu) * kLdsRow + e] = 1;
+        }
+        return wv;
+    };
+    // FP4 form of stage: the slot's 4 frames in 2 bytes of each row, frame
+    // 2e in the low and 2e+1 in the high nibble of byte e.  Clearing writes 0
+    // to byte e of both old rows; setting writes 0x02 to row c0 and 0x20 to
+    // row c1, or 0x22 to both (one byte) when the two codes are equal.  The
+    // slot owns these bytes of every row, so whole-byte writes lose nothing.
+    auto stage_f4 = [&](int b, int nv, uint32_t w, uint32_t old) __attribute__((always_inline)) -> uint32_t {
+        const uint32_t rowoff = (uint32_t)(b * 2 * kOpBytes) + dst;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) smem_raw[rowoff + ((old >> (8 * i)) & 0xFFu) * kLdsRow + (i >> 1)] = 0;
+        if (nv >= kCh) {
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const uint32_t c0 = (w >> (16 * e)) & 0xFFu, c1 = (w >> (16 * e + 8)) & 0xFFu;
+                const bool same = c0 == c1;
+                smem_raw[rowoff + c0 * kLdsRow + e] = same ? 0x22 : 0x02;
+                smem_raw[rowoff + c1 * kLdsRow + e] = same ? 0x22 : 0x20;
+            }
+            return w;
+        }
+        // a partial chunk: frames past nv go to the trash row
+        const int nvk = nv - kfr;
+        const uint32_t vmask = nvk >= 4 ? 0xFFFFFFFFu : (nvk <= 0 ? 0u : 0xFFFFFFFFu >> (8 * (4 - nvk)));
+        const uint32_t wv = w & vmask;
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const uint32_t c0 = (wv >> (16 * e)) & 0xFFu, c1 = (wv >> (16 * e + 8)) & 0xFFu;
+            const bool v0 = nvk > 2 * e, v1 = nvk > 2 * e + 1;   // v1 implies v0
+            const bool same = c0 == c1;
+            smem_raw[(v0 ? rowoff : trash_off) + c0 * kLdsRow + e] = v1 && same ? 0x22 : 0x02;
+            smem_raw[(v1 ? rowoff : trash_off) + c1 * kLdsRow + e] = v0 && same ? 0x22 : 0x20;
         }
         return wv;
     };
@@ -510,15 +601,18 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
         if constexpr (kSmall) {
             stage4(b, nv, w);
             return 0u;
+        } else if constexpr (kF4) {
+            return stage_f4(b, nv, w, old);
         } else {
             return stage(b, nv, w, old);
         }
     };
-    v16i acc[4][2];
+    using AccT = std::conditional_t<kF4, v16f, v16i>;
+    AccT acc[4][2];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = v16i{};
+        for (int j = 0; j < 2; ++j) acc[i][j] = AccT{};
     // fragment i of the row operand / j of the column operand, 32 frames kk.
     // The wave's 4 row blocks and 2 column blocks of 32: contiguous (rows
     // 128wr.., columns 64wc..), or on the small-bin tile interleaved (rows
@@ -548,12 +642,14 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
         for (int i = 0; i < 4; ++i) fa0[i] = rdA(base, 0, i);
 #pragma unroll
         for (int j = 0; j < 2; ++j) fb0[j] = rdB(base, 0, j);
+        if constexpr (!kF4) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) fa1[i] = rdA(base, 1, i);
+            for (int i = 0; i < 4; ++i) fa1[i] = rdA(base, 1, i);
 #pragma unroll
-        for (int j = 0; j < 2; ++j) fb1[j] = rdB(base, 1, j);
+            for (int j = 0; j < 2; ++j) fb1[j] = rdB(base, 1, j);
+        }
     };
-    // The MFMAs of 32 frames (kk); with kPref each fragment register is
+    // The MFMAs of 32 frames (kk; the FP4 form's one kk is all 64); with kPref each fragment register is
     // reloaded with chunk g+1's fragment (buffer nbase) as soon as its last
     // MFMA of chunk g has issued, so the next chunk's LDS reads overlap this
     // chunk's MFMAs and need no second register set.
@@ -562,8 +658,13 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
         constexpr bool kPref = decltype(pref)::value;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            acc[i][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i], bf[0], acc[i][0], 0, 0, 0);
-            acc[i][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i], bf[1], acc[i][1], 0, 0, 0);
+            if constexpr (kF4) {
+                acc[i][0] = mfma_f4(af[i], bf[0], acc[i][0]);
+                acc[i][1] = mfma_f4(af[i], bf[1], acc[i][1]);
+            } else {
+                acc[i][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i], bf[0], acc[i][0], 0, 0, 0);
+                acc[i][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i], bf[1], acc[i][1], 0, 0, 0);
+            }
             if constexpr (kPref) {
                 __builtin_amdgcn_sched_barrier(0);
                 af[i] = rdA(nbase, kk, i);
@@ -594,11 +695,11 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
             for (int j = 0; j < 2; ++j) {
                 if constexpr (kSmall) {
                     if ((need >> (8 * (rowblk(i) >> 5) + (colblk(j) >> 5))) & 1)
-                        epi_block_small<kNz>(acc[i][j], rowblk(i), colblk(j) + (lane & 31), lane, sm,
+                        epi_block_small<kNz>(counts_of(acc[i][j]), rowblk(i), colblk(j) + (lane & 31), lane, sm,
                                              R, nstr, ntab, kfix);
                 }
                 else
-                    epi_block8<kNz>(acc[i][j], rowblk(i), colblk(j) + (lane & 31), lane,
+                    epi_block8<kNz>(counts_of(acc[i][j]), rowblk(i), colblk(j) + (lane & 31), lane,
                                     sm, nb, R, nstr, ntab, kfix);
                 __builtin_amdgcn_sched_barrier(0);
             }
@@ -701,7 +802,7 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
                 w0 = w1;
             }
             mfma_kk(pref, fa0, fb0, nbase, 0);
-            mfma_kk(pref, fa1, fb1, nbase, 1);
+            if constexpr (!kF4) mfma_kk(pref, fa1, fb1, nbase, 1);
             if constexpr (kStg) w1 = w2;
             // the small-bin tile's 1,024 pairs: every wave, two pairs a lane
             if constexpr (kSmall) {
@@ -733,9 +834,9 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     if (fin >= 0 && (kSmall || wave == fwave)) finalize(fin);
 }
 
-template <bool kNz, bool kSmall>
+template <bool kNz, bool kSmall, bool kF4>
 __global__ __launch_bounds__(512) void pair_tile8_kernel(PairKernelArgs a) {
-    tile8_body<false, kNz, kSmall>(a, blockIdx.x);
+    tile8_body<false, kNz, kSmall, kF4>(a, blockIdx.x);
 }
 
 // Batch form of the eight-wave tile, persistent: one workgroup per CU takes
@@ -746,7 +847,7 @@ __global__ __launch_bounds__(512) void pair_tile8_kernel(PairKernelArgs a) {
 // stamps the system's completion (100 MHz wall clock).
 // Persistent batch kernel: the tile-independent LDS tables are written once,
 // then each tile of the loop writes only its operand buffers and pair slots.
-template <bool kNz, bool kSmall>
+template <bool kNz, bool kSmall, bool kF4>
 __global__ __launch_bounds__(512) void pair_batch8_kernel(PairKernelArgs a, uint32_t lds_next) {
     extern __shared__ __align__(16) uint8_t smem_raw[];
     int* const s_t = reinterpret_cast<int*>(smem_raw + lds_next);
@@ -758,7 +859,7 @@ __global__ __launch_bounds__(512) void pair_batch8_kernel(PairKernelArgs a, uint
         __syncthreads();
         const int t = __builtin_amdgcn_readfirstlane(*s_t);
         if (t >= a.n_work) break;
-        tile8_body<true, kNz, kSmall, false>(a, t);
+        tile8_body<true, kNz, kSmall, kF4, false>(a, t);
         if (threadIdx.x == 0 && a.sys_count) {
             const int sys = a.tile_sys[t];
             __threadfence();
@@ -784,55 +885,67 @@ bool pair_tile8_supported(int R, int nb) {
 
 int pair_fixed_k(double n_max) { return fixed_k(n_max, std::log(n_max)); }
 
+bool pair_f4_exact(double n_max) { return n_max <= 16777216.0; }
+
+namespace {
+
+// the eight-wave kernel of a launch: batch or not, nonzero-bin slots,
+// small-bin tile, FP4 operands
+template <bool kNz, bool kSmall, bool kF4>
+const void* tile8_fn(bool batch) {
+    return batch ? reinterpret_cast<const void*>(&pair_batch8_kernel<kNz, kSmall, kF4>)
+                 : reinterpret_cast<const void*>(&pair_tile8_kernel<kNz, kSmall, kF4>);
+}
+
+const void* tile8_pick(bool batch, bool nz, bool small, bool f4) {
+    static const void* (*const tab[8])(bool) = {
+        tile8_fn<false, false, false>, tile8_fn<false, false, true>,
+        tile8_fn<false, true, false>,  tile8_fn<false, true, true>,
+        tile8_fn<true, false, false>,  tile8_fn<true, false, true>,
+        tile8_fn<true, true, false>,   tile8_fn<true, true, true>};
+    return tab[(nz ? 4 : 0) + (small ? 2 : 0) + (f4 ? 1 : 0)](batch);
+}
+
+}  // namespace
+
 hipError_t launch_pair_tiles8(const PairKernelArgs& a_in, int n_tiles, hipStream_t st) {
     if (n_tiles <= 0) return hipSuccess;
     if (!pair_tile8_supported(a_in.R, a_in.nb)) return hipErrorInvalidValue;
     // the fixed point of the slot sums, from the last step's frame count
     PairKernelArgs a = a_in;
-    a.kfix = pair_fixed_k((double)a.nsteps * (double)a.ss);
+    const double n_max = (double)a.nsteps * (double)a.ss;
+    a.kfix = pair_fixed_k(n_max);
+    const bool f4 = !a.int8_only && pair_f4_exact(n_max);
     const bool small = pair_tile8_small(a.nb);
     // nonzero bins are counted only for the Miller-Madow estimator
     const bool nz = a.slot_mm >= 0 || a.mi_from_mm;
     const size_t lds = lay8(a.R, small, nz).total;
-    const void* fn;
-    if (small) fn = nz ? reinterpret_cast<const void*>(&pair_tile8_kernel<true, true>)
-                       : reinterpret_cast<const void*>(&pair_tile8_kernel<false, true>);
-    else fn = nz ? reinterpret_cast<const void*>(&pair_tile8_kernel<true, false>)
-                 : reinterpret_cast<const void*>(&pair_tile8_kernel<false, false>);
+    const void* fn = tile8_pick(false, nz, small, f4);
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    if (small) {
-        if (nz) hipLaunchKernelGGL((pair_tile8_kernel<true, true>), dim3(n_tiles), dim3(512), lds, st, a);
-        else hipLaunchKernelGGL((pair_tile8_kernel<false, true>), dim3(n_tiles), dim3(512), lds, st, a);
-    } else {
-        if (nz) hipLaunchKernelGGL((pair_tile8_kernel<true, false>), dim3(n_tiles), dim3(512), lds, st, a);
-        else hipLaunchKernelGGL((pair_tile8_kernel<false, false>), dim3(n_tiles), dim3(512), lds, st, a);
-    }
+    void* args[] = {&a};
+    e = hipLaunchKernel(fn, dim3(n_tiles), dim3(512), args, lds, st);
+    if (e != hipSuccess) return e;
     return hipGetLastError();
 }
 
-hipError_t launch_pair_batch8(const PairKernelArgs& a, int grid, hipStream_t st) {
-    if (a.n_work <= 0 || grid <= 0) return hipSuccess;
-    if (!pair_tile8_supported(a.R, a.nb)) return hipErrorInvalidValue;
+// the caller sets a.int8_only when the batch's largest frame count is past
+// pair_f4_exact
+hipError_t launch_pair_batch8(const PairKernelArgs& a_in, int grid, hipStream_t st) {
+    if (a_in.n_work <= 0 || grid <= 0) return hipSuccess;
+    if (!pair_tile8_supported(a_in.R, a_in.nb)) return hipErrorInvalidValue;
+    PairKernelArgs a = a_in;
     const bool small = pair_tile8_small(a.nb);
     const bool nz = a.slot_mm >= 0 || a.mi_from_mm;
     const size_t body = lay8(a.R, small, nz).total;
-    const uint32_t next = (uint32_t)((body + 15) & ~(size_t)15);
+    uint32_t next = (uint32_t)((body + 15) & ~(size_t)15);
     const size_t lds = next + 16;
     if (lds > kLdsCap) return hipErrorInvalidValue;
-    const void* fn;
-    if (small) fn = nz ? reinterpret_cast<const void*>(&pair_batch8_kernel<true, true>)
-                       : reinterpret_cast<const void*>(&pair_batch8_kernel<false, true>);
-    else fn = nz ? reinterpret_cast<const void*>(&pair_batch8_kernel<true, false>)
-                 : reinterpret_cast<const void*>(&pair_batch8_kernel<false, false>);
+    const void* fn = tile8_pick(true, nz, small, !a.int8_only);
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    if (small) {
-        if (nz) hipLaunchKernelGGL((pair_batch8_kernel<true, true>), dim3(grid), dim3(512), lds, st, a, next);
-        else hipLaunchKernelGGL((pair_batch8_kernel<false, true>), dim3(grid), dim3(512), lds, st, a, next);
-    } else {
-        if (nz) hipLaunchKernelGGL((pair_batch8_kernel<true, false>), dim3(grid), dim3(512), lds, st, a, next);
-        else hipLaunchKernelGGL((pair_batch8_kernel<false, false>), dim3(grid), dim3(512), lds, st, a, next);
-    }
+    void* args[] = {&a, &next};
+    e = hipLaunchKernel(fn, dim3(grid), dim3(512), args, lds, st);
+    if (e != hipSuccess) return e;
     return hipGetLastError();
 }

@@ -27,6 +27,7 @@ _SIGS = {
     "centre_sync": ([P], I32),
     "centre_stream": ([P], P),
     "centre_last_pair_kernel_ms": ([P], C.c_float),
+    "centre_set_mfma_format": ([P, I32], I32),
     "centre_batch_pair_entropy": ([P, P, I32, I32, I32, P, I32, I32, I32, I32, I32, P, P, P, P],
                                   I32),
     "centre_screen_n_params": ([], I32),
@@ -235,6 +236,10 @@ class Context:
 
     def last_pair_kernel_ms(self) -> float:
         return float(self._L.centre_last_pair_kernel_ms(self.h))
+
+    def set_mfma_format(self, fmt: int) -> None:
+        """Pair-tile operands: 0 = FP4 where the counts stay exact, 1 = int8."""
+        self.check(self._L.centre_set_mfma_format(self.h, int(fmt)))
 
     # ---- host-buffer entry points ------------------------------------------
     def discretize_hist(self, bat: np.ndarray, dof_type: int, schemes,

@@ -49,6 +49,14 @@ int centre_sync(centre_ctx *ctx);
 void *centre_stream(centre_ctx *ctx);
 /* Milliseconds of the last pair-stage kernel launch (HIP events). */
 float centre_last_pair_kernel_ms(const centre_ctx *ctx);
+/* Operand format of the pair stage's one-hot MFMA tiles.  No reference
+ * counterpart (the reference counts on the CPU).  0 (the default): FP4 (e2m1)
+ * one-hot operands, two frames a byte, whenever the largest joint count a
+ * call can reach (steps x frames a step) is at most 2^24, where the f32
+ * accumulation of 0/1 products is exact; int8 operands beyond.  1: int8
+ * operands always.  The counts, and so every result, are the same either
+ * way.  Returns 0, or -3 for another value. */
+int centre_set_mfma_format(centre_ctx *ctx, int format);
 
 /* ---- discretizer ----------------------------------------------------------
  * Replaces Real2Int::bndReal2Int / angReal2Int / dihReal2Int

@@ -30,6 +30,15 @@ def ctx():
     c.close()
 
 
+@pytest.fixture(params=[0, 1], ids=["fp4", "int8"])
+def fctx(ctx, request):
+    """The module context with the pair tiles' one-hot operands in FP4
+    (the default wherever counts stay below 2^24) or int8."""
+    ctx.set_mfma_format(request.param)
+    yield ctx
+    ctx.set_mfma_format(0)
+
+
 # ---------------------------------------------------------------------------
 # reference goldens: example-1 codes -> entcontri S1/S2 (30 bins, 4 steps)
 TYPES = {"b": 1, "a": 2, "t": 3}
@@ -160,7 +169,7 @@ def test_ragged_shapes_vs_oracle(ctx, nb, D, F, S):
 @pytest.mark.parametrize("ests", [(0, 1), (0,)])
 @pytest.mark.parametrize("nb,D,F,S", [(5, 70, 3001, 3), (8, 64, 2048, 2), (2, 33, 999, 1),
                                       (3, 100, 5000, 5), (7, 40, 130, 2), (1, 35, 700, 2)])
-def test_small_bin_tiles_vs_oracle(ctx, nb, D, F, S, ests):
+def test_small_bin_tiles_vs_oracle(fctx, nb, D, F, S, ests):
     """nb <= 8 runs every tile, those of a partial last block included, on
     the eight-wave small-bin kernel (32 DOFs a side, 8 one-hot rows per DOF
     built by byte permutes); D past and below a block multiple, steps of
@@ -174,6 +183,7 @@ def test_small_bin_tiles_vs_oracle(ctx, nb, D, F, S, ests):
     ext[:, 0] = rng.uniform(0.5, 1.0, D)
     ext[:, 1] = ext[:, 0] + rng.uniform(0.2, 1.5, D)
     iu, ju = np.triu_indices(D, 1)
+    ctx = fctx
     print(f"small-bin tiles nb={nb} D={D} F={F} S={S} ests={ests}", flush=True)
     s1, s2, mi = ctx.pair_entropy(codes, types, ext, nb, S, ests=ests)
     w2 = O.pair_entropies(codes, types, ext, np.stack([iu, ju], 1), S, [nb], ests)
@@ -217,7 +227,7 @@ def test_pair_stage_is_deterministic(ctx, nb, D):
                                       (50, 23, 6002, 2), (29, 17, 1500, 2), (31, 16, 2000, 3),
                                       (33, 15, 1000, 2), (40, 13, 1200, 2), (45, 11, 1100, 2),
                                       (130, 4, 700, 2)])
-def test_full_tiles_vs_oracle(ctx, nb, D, F, S, ests):
+def test_full_tiles_vs_oracle(fctx, nb, D, F, S, ests):
     """Tiles with R DOFs on both sides (diagonal ones included) run on the
     eight-wave kernel for nb >= 29; shapes with whole blocks only, a
     partial last block, and step lengths that are and are not a multiple of
@@ -234,6 +244,7 @@ def test_full_tiles_vs_oracle(ctx, nb, D, F, S, ests):
     ext[:, 0] = rng.uniform(0.5, 1.0, D)
     ext[:, 1] = ext[:, 0] + rng.uniform(0.2, 1.5, D)
     iu, ju = np.triu_indices(D, 1)
+    ctx = fctx
     print(f"full tiles nb={nb} D={D} F={F} S={S} ests={ests}", flush=True)
     s1, s2, mi = ctx.pair_entropy(codes, types, ext, nb, S, ests=ests)
     w2 = O.pair_entropies(codes, types, ext, np.stack([iu, ju], 1), S, [nb], ests)
@@ -246,6 +257,52 @@ def test_full_tiles_vs_oracle(ctx, nb, D, F, S, ests):
     rev = ests[::-1]
     s1, s2, mi = ctx.pair_entropy(codes, types, ext, nb, S, pairs=pairs, ests=rev)
     w2 = O.pair_entropies(codes, types, ext, pairs, S, [nb], rev)
+    assert rel_err(s2, w2[:, :, 0, :]) < REL
+
+
+@pytest.mark.parametrize("nb,D,F,S", [(50, 160, 12000, 6), (36, 70, 9001, 4), (6, 300, 10000, 5),
+                                      (3, 96, 777, 3)])
+def test_mfma_formats_bit_identical(ctx, nb, D, F, S):
+    """FP4 and int8 one-hot operands give the same counts, so S2 and MI
+    (fixed-point sums of the same counts) are bit-identical: eight-wave
+    tile (nb 50, 36) and small-bin tile (nb 6, 3), partial chunks, ML + MM."""
+    rng = np.random.default_rng(nb * 5 + D)
+    codes = rng.integers(0, nb, size=(D, F), dtype=np.uint8)
+    codes[1::2] = (codes[0::2][: len(codes[1::2])] + rng.integers(0, 2, size=F)) % nb
+    codes[3::7] = 0
+    types = rng.integers(1, 4, size=D).astype(np.uint8)
+    ext = np.zeros((D, 4))
+    ext[:, 0] = rng.uniform(0.5, 1.0, D)
+    ext[:, 1] = ext[:, 0] + rng.uniform(0.2, 1.5, D)
+    print(f"formats nb={nb} D={D} F={F} S={S}", flush=True)
+    out = []
+    try:
+        for fmt in (0, 1):
+            ctx.set_mfma_format(fmt)
+            out.append(ctx.pair_entropy(codes, types, ext, nb, S, ests=(0, 1)))
+    finally:
+        ctx.set_mfma_format(0)
+    assert np.array_equal(out[0][1].view(np.uint64), out[1][1].view(np.uint64))
+    assert np.array_equal(out[0][2].view(np.uint64), out[1][2].view(np.uint64))
+
+
+def test_counts_past_the_fp4_limit(ctx):
+    """One step of 18,500,003 frames with joint bin (0, 0) holding about 99%
+    of them: the count passes 2^24, where f32 accumulation of FP4 products
+    would round, so the launch takes int8 operands; S2 against the oracle."""
+    nb, D, F = 50, 5, 18_500_003
+    rng = np.random.default_rng(24)
+    codes = np.zeros((D, F), np.uint8)
+    hot = rng.random((D, F)) < 0.005
+    codes[hot] = rng.integers(0, nb, size=int(hot.sum()), dtype=np.uint8)
+    types = np.array([1, 2, 3, 1, 2], np.uint8)
+    ext = np.zeros((D, 4))
+    ext[:, 0] = 1.0
+    ext[:, 1] = 2.0
+    iu, ju = np.triu_indices(D, 1)
+    print(f"past the fp4 limit nb={nb} D={D} F={F}", flush=True)
+    s1, s2, mi = ctx.pair_entropy(codes, types, ext, nb, 1, ests=(0,))
+    w2 = O.pair_entropies(codes, types, ext, np.stack([iu, ju], 1), 1, [nb], (0,))
     assert rel_err(s2, w2[:, :, 0, :]) < REL
 
 
