@@ -14,12 +14,19 @@ typedef float v16f __attribute__((ext_vector_type(16)));
 // 0x0 = 0), two frames a byte (the even frame in the low nibble), multiplied
 // by v_mfma_scale_f32_32x32x64_f8f6f4 with both block scales 2^0 (E8M0 127):
 // a 64-frame chunk is one MFMA a block (the int8 form takes two, each in the
-// same issue time), read from the first 32 bytes of each LDS row.  Every product is 0 or 1 and the f32 accumulator
-// holds integers, exact up to 2^24, so the counts equal the int8 path's while
-// a pair's total frame count is at most 2^24 (the launch checks it).  A lane
+// same issue time), from LDS rows of 32 bytes (kLdsRowF4 apart).  Every
+// product is 0 or 1 and the f32 accumulator holds integers, exact up to 2^24,
+// so the counts equal the int8 path's while a pair's total frame count is at
+// most 2^24 (the launch checks it).  A lane
 // holds 32 consecutive k of its row or column (k = 32 * (lane / 32) + t, t in
 // its 16 bytes low nibble first); the operands are laid out alike, so only the
 // pairing of A's and B's k matters, and it is the same frame on both sides.
+// LDS row stride of the FP4 one-hot operands: 32 bytes of nibbles (64
+// frames) and 16 of padding, so that the 16 lanes of a 16-byte fragment read
+// cover distinct banks; the smaller buffers leave room for a larger c log c
+// table
+constexpr int kLdsRowF4 = 48;
+
 __device__ __forceinline__ v16f mfma_f4(const v4i& a, const v4i& b, const v16f& c) {
     const v8i a8 = {a[0], a[1], a[2], a[3], 0, 0, 0, 0};
     const v8i b8 = {b[0], b[1], b[2], b[3], 0, 0, 0, 0};
@@ -260,18 +267,19 @@ __device__ __forceinline__ void epi_block_small(const v16i& blk, int brow, int g
 // trash row, the per-pair slots, the log tables and as large a c log c table
 // as the rest leaves room for.
 constexpr int kBuf8 = 3;
+constexpr int kTabSmall = 16384;   // c log c table cap of the small-bin tile
 
 struct Lay8 {
-    size_t map, T, N, log, tab, total;
+    size_t map, T, N, log, dofc, tmap, tab, total;
     int nstripe, ntab;
 };
 
 // nz: nonzero-bin slots (Miller-Madow) are kept; without them the c log c
 // table takes their room.  The small-bin tile's map area also holds the
 // 64-bit mask of the 32x32 blocks that contain a pair to evaluate.
-__host__ __device__ inline Lay8 lay8(int R, bool small, bool nz) {
+__host__ __device__ inline Lay8 lay8(int R, bool small, bool nz, bool f4) {
     Lay8 L;
-    size_t off = (size_t)kBuf8 * 2 * kOpBytes;
+    size_t off = (size_t)kBuf8 * 2 * kTileRows * (f4 ? kLdsRowF4 : kLdsRow);
     L.map = off;
     off += 512 + kLdsRow + 16;
     off = (off + 15) & ~(size_t)15;
@@ -283,11 +291,21 @@ __host__ __device__ inline Lay8 lay8(int R, bool small, bool nz) {
     off = (off + 15) & ~(size_t)15;
     L.log = off;   // log(1 + j/256), 1/(1 + j/256), then 8 series coefficients
     off += (2 * 256 + 8) * sizeof(double);
+    // small-bin tile: its finalize inputs (the big-bin tile's finalize reads
+    // them from global memory; LDS addresses held for it cost registers)
+    L.dofc = off;   // the tile's 2R DOFs: bin width, then L and S1 of the step
+    if (small) off += 3 * 64 * sizeof(double);
+    L.tmap = off;   // the tile's canonical pair indices (explicit plans)
+    if (small) off += sizeof(int) * R * R;
+    off = (off + 15) & ~(size_t)15;
     L.tab = off;
     // 16 bytes stay free past the layout: the persistent batch kernel's
     // tile counter word
     const size_t room = off + 16 < kLdsCap ? (kLdsCap - off - 16) / sizeof(double) : 0;
-    L.ntab = (int)((room < (size_t)kTab ? room : (size_t)kTab) / 256 * 256);
+    // the small-bin tile's counts are large (few bins), so its table takes
+    // all the room there is
+    const size_t cap = small ? (size_t)kTabSmall : (size_t)kTab;
+    L.ntab = (int)((room < cap ? room : cap) / 256 * 256);
     off += sizeof(double) * L.ntab;
     L.total = off;
     return L;
@@ -298,10 +316,10 @@ __host__ __device__ inline Lay8 lay8(int R, bool small, bool nz) {
 // the c log c table.  The persistent batch kernel writes them once before
 // its tile loop (computed inside the loop they would be hoisted out of it
 // and held in registers across every tile).
-template <bool kNz, bool kSmall>
+template <bool kNz, bool kSmall, bool kF4>
 __device__ __forceinline__ void tile8_tables(int R, int nb, int kfix) {
     extern __shared__ __align__(16) uint8_t smem_raw[];
-    const Lay8 lay = lay8(R, kSmall, kNz);
+    const Lay8 lay = lay8(R, kSmall, kNz, kF4);
     uint8_t* const row2dof = smem_raw + lay.map;
     uint8_t* const col2dof = smem_raw + lay.map + 256;
     double* const logt = reinterpret_cast<double*>(smem_raw + lay.log);
@@ -321,8 +339,11 @@ __device__ __forceinline__ void tile8_tables(int R, int nb, int kfix) {
             invt[256 + tid] = cf[tid];
         }
     }
+    // c log c by the same log_pos_lds as the epilogue's counts past the
+    // table, so that a count's term does not depend on the table's size
+    __syncthreads();
     for (int c = tid; c < lay.ntab; c += 512)
-        ctab[c] = c > 1 ? to_fixed((double)c * log((double)c), kfix) : 0ull;
+        ctab[c] = c > 1 ? to_fixed((double)c * log_pos_lds((double)c, logt, invt, invt + 256), kfix) : 0ull;
 }
 
 // kTables: also write the tile-independent LDS contents (tile8_tables);
@@ -339,8 +360,11 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     const uint64_t ss = kBatch ? sdp->ss : a.ss;
     const uint64_t sstride = kBatch ? sdp->sstride : a.sstride;
     const int R = a.R, nb = a.nb, S = a.nsteps;
-    const Lay8 lay = lay8(R, kSmall, kNz);
+    const Lay8 lay = lay8(R, kSmall, kNz, kF4);
     const int nstr = lay.nstripe, ntab = lay.ntab;
+    // one-hot row stride and operand size of this form
+    constexpr int kRow = kF4 ? kLdsRowF4 : kLdsRow;
+    constexpr int kOpB = kTileRows * kRow;
     Smem sm;
     sm.oh = smem_raw;
     sm.cl = nullptr;
@@ -364,15 +388,31 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     const int2 tile = a.tiles[tix];
     const int r0 = tile.x * R, c0 = tile.y * R;
     const bool diag = tile.x == tile.y;
-    const size_t boff = diag ? 0 : kOpBytes;   // column operand
+    const size_t boff = diag ? 0 : kOpB;   // column operand
     const long long D = kBatch ? sdp->n_dof : a.n_dof;
 
-    for (int i = tid; i < kBuf8 * 2 * kOpBytes / 16; i += 512)
+    for (int i = tid; i < kBuf8 * 2 * kOpB / 16; i += 512)
         reinterpret_cast<uint4*>(sm.oh)[i] = make_uint4(0, 0, 0, 0);
-    if constexpr (kTables) tile8_tables<kNz, kSmall>(R, nb, a.kfix);
+    if constexpr (kTables) tile8_tables<kNz, kSmall, kF4>(R, nb, a.kfix);
     for (int i = tid; i < R * R * nstr; i += 512) {
         sm.slotT[i] = 0ull;
         if constexpr (kNz) sm.slotN[i] = 0;
+    }
+    // small-bin tile, per-tile finalize inputs: bin widths of the tile's
+    // DOFs and, for an explicit plan, its canonical pair indices (rows past
+    // the system's last DOF read the last one; their pairs are never written)
+    double* const dcw = reinterpret_cast<double*>(smem_raw + lay.dofc);
+    double* const dcL = dcw + 64;
+    double* const dcS = dcw + 128;
+    int* const tmap = reinterpret_cast<int*>(smem_raw + lay.tmap);
+    if constexpr (kSmall) {
+        const double* const dw = kBatch ? sdp->dof_w : a.dof_w;
+        for (int t = tid; t < 2 * R; t += 512) {
+            const long long g = t < R ? r0 + t : c0 + (t - R);
+            dcw[t] = dw[g < D ? g : D - 1];
+        }
+        if (a.tilemap)
+            for (int pq = tid; pq < R * R; pq += 512) tmap[pq] = a.tilemap[(size_t)tix * R * R + pq];
     }
     if (tid == 0) *reinterpret_cast<int*>(trash) = *reinterpret_cast<volatile int*>(a.flag);
     // small-bin tile: bit 8*rb + cb marks a 32x32 MFMA block (row DOFs
@@ -420,7 +460,7 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     // row; their pairs are never written
     const long long grow = isA ? r0 + ld : c0 + (ld - R);
     const uint8_t* const src = codes + (uint64_t)(grow < D ? grow : D - 1) * cstride + kfr;
-    const uint32_t dst = (uint32_t)((isA ? 0 : kOpBytes) + (isA ? ld : ld - R) * nb * kLdsRow + kb);
+    const uint32_t dst = (uint32_t)((isA ? 0 : kOpB) + (isA ? ld : ld - R) * nb * kRow + kb);
     const uint32_t trash_off = (uint32_t)(lay.map + 512 + 16 + kb);
     // what this slot set in the buffers staged one, two and three chunks
     // before the next (the buffer staged next is the one staged three ago)
@@ -436,7 +476,7 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     const long long srow = sA ? r0 + sd : c0 + (sd - R);
     const bool srow_ok = srow < D;
     const uint8_t* const src_s = codes + (uint64_t)(srow_ok ? srow : 0) * cstride + g16;
-    const uint32_t dst_s = (uint32_t)((sA ? 0 : kOpBytes) + (sA ? sd : sd - R) * 8 * kLdsRow + gb);
+    const uint32_t dst_s = (uint32_t)((sA ? 0 : kOpB) + (sA ? sd : sd - R) * 8 * kRow + gb);
     const bool stager_s = wave * 64 < nlive_s;   // wave-uniform
     int ls = 0, lc = 0;   // next chunk to load
     auto advance = [](int& s, int& c, int n) __attribute__((always_inline)) {
@@ -499,7 +539,7 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
                 x[d] = (x[d] & keep) | (0x0C0C0C0Cu & ~keep);
             }
         }
-        const uint32_t base = (uint32_t)(b * 2 * kOpBytes) + dst_s;
+        const uint32_t base = (uint32_t)(b * 2 * kOpB) + dst_s;
         if constexpr (kF4) {
             uint32_t ev[2], od[2];
 #pragma unroll
@@ -516,7 +556,7 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
 #pragma unroll
                 for (int m = 0; m < 2; ++m)
                     o[m] = __builtin_amdgcn_perm(hi, lo, ev[m]) | __builtin_amdgcn_perm(hi << 4, lo << 4, od[m]);
-                *reinterpret_cast<uint2*>(smem_raw + base + bin * kLdsRow) = make_uint2(o[0], o[1]);
+                *reinterpret_cast<uint2*>(smem_raw + base + bin * kRow) = make_uint2(o[0], o[1]);
             }
         } else {
 #pragma unroll
@@ -529,7 +569,7 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
                 o.y = __builtin_amdgcn_perm(hi, lo, x[1]);
                 o.z = __builtin_amdgcn_perm(hi, lo, x[2]);
                 o.w = __builtin_amdgcn_perm(hi, lo, x[3]);
-                *reinterpret_cast<uint4*>(smem_raw + base + bin * kLdsRow) = o;
+                *reinterpret_cast<uint4*>(smem_raw + base + bin * kRow) = o;
             }
         }
     };
@@ -540,12 +580,12 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     // go to the trash row.  Offsets stay integers so the stores compile to
     // ds_write.
     auto stage = [&](int b, int nv, uint32_t w, uint32_t old) __attribute__((always_inline)) -> uint32_t {
-        const uint32_t rowoff = (uint32_t)(b * 2 * kOpBytes) + dst;
+        const uint32_t rowoff = (uint32_t)(b * 2 * kOpB) + dst;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) smem_raw[rowoff + ((old >> (8 * e)) & 0xFFu) * kLdsRow + e] = 0;
+        for (int e = 0; e < 4; ++e) smem_raw[rowoff + ((old >> (8 * e)) & 0xFFu) * kRow + e] = 0;
         if (nv >= kChunk) {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) smem_raw[rowoff + ((w >> (8 * e)) & 0xFFu) * kLdsRow + e] = 1;
+            for (int e = 0; e < 4; ++e) smem_raw[rowoff + ((w >> (8 * e)) & 0xFFu) * kRow + e] = 1;
             return w;
         }
         const int nvk = nv - kfr;
@@ -554,7 +594,7 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const uint32_t o = nvk > e ? rowoff : trash_off;
-            smem_raw[o + ((wv >> (8 * e)) & 0xFFu) * kLdsRow + e] = 1;
+            smem_raw[o + ((wv >> (8 * e)) & 0xFFu) * kRow + e] = 1;
         }
         return wv;
     };
@@ -564,16 +604,16 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     // row c1, or 0x22 to both (one byte) when the two codes are equal.  The
     // slot owns these bytes of every row, so whole-byte writes lose nothing.
     auto stage_f4 = [&](int b, int nv, uint32_t w, uint32_t old) __attribute__((always_inline)) -> uint32_t {
-        const uint32_t rowoff = (uint32_t)(b * 2 * kOpBytes) + dst;
+        const uint32_t rowoff = (uint32_t)(b * 2 * kOpB) + dst;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) smem_raw[rowoff + ((old >> (8 * i)) & 0xFFu) * kLdsRow + (i >> 1)] = 0;
+        for (int i = 0; i < 4; ++i) smem_raw[rowoff + ((old >> (8 * i)) & 0xFFu) * kRow + (i >> 1)] = 0;
         if (nv >= kCh) {
 #pragma unroll
             for (int e = 0; e < 2; ++e) {
                 const uint32_t c0 = (w >> (16 * e)) & 0xFFu, c1 = (w >> (16 * e + 8)) & 0xFFu;
                 const bool same = c0 == c1;
-                smem_raw[rowoff + c0 * kLdsRow + e] = same ? 0x22 : 0x02;
-                smem_raw[rowoff + c1 * kLdsRow + e] = same ? 0x22 : 0x20;
+                smem_raw[rowoff + c0 * kRow + e] = same ? 0x22 : 0x02;
+                smem_raw[rowoff + c1 * kRow + e] = same ? 0x22 : 0x20;
             }
             return w;
         }
@@ -586,8 +626,8 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
             const uint32_t c0 = (wv >> (16 * e)) & 0xFFu, c1 = (wv >> (16 * e + 8)) & 0xFFu;
             const bool v0 = nvk > 2 * e, v1 = nvk > 2 * e + 1;   // v1 implies v0
             const bool same = c0 == c1;
-            smem_raw[(v0 ? rowoff : trash_off) + c0 * kLdsRow + e] = v1 && same ? 0x22 : 0x02;
-            smem_raw[(v1 ? rowoff : trash_off) + c1 * kLdsRow + e] = v0 && same ? 0x22 : 0x20;
+            smem_raw[(v0 ? rowoff : trash_off) + c0 * kRow + e] = v1 && same ? 0x22 : 0x02;
+            smem_raw[(v1 ? rowoff : trash_off) + c1 * kRow + e] = v0 && same ? 0x22 : 0x20;
         }
         return wv;
     };
@@ -618,7 +658,7 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     // 128wr.., columns 64wc..), or on the small-bin tile interleaved (rows
     // 32(wr + 2i), columns 32(wc + 4j)), so that the blocks whose epilogue
     // is skipped (a triangle of the tile) are spread over the waves.
-    const int lrow = (lane & 31) * kLdsRow + (lane >> 5) * 16;
+    const int lrow = (lane & 31) * kRow + (lane >> 5) * 16;
     auto rowblk = [&](int i) __attribute__((always_inline)) {
         return kSmall ? 32 * (wr + 2 * i) : 128 * wr + 32 * i;
     };
@@ -629,11 +669,11 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     // __shared__ array itself, so the reads compile to ds_read (a generic
     // pointer carried through the loop compiles to flat loads)
     auto rdA = [&](uint32_t bo, int kk, int i) __attribute__((always_inline)) -> v4i {
-        return *reinterpret_cast<const v4i*>(smem_raw + bo + rowblk(i) * kLdsRow + lrow +
+        return *reinterpret_cast<const v4i*>(smem_raw + bo + rowblk(i) * kRow + lrow +
                                              kk * 32);
     };
     auto rdB = [&](uint32_t bo, int kk, int j) __attribute__((always_inline)) -> v4i {
-        return *reinterpret_cast<const v4i*>(smem_raw + bo + boff + colblk(j) * kLdsRow +
+        return *reinterpret_cast<const v4i*>(smem_raw + bo + boff + colblk(j) * kRow +
                                              lrow + kk * 32);
     };
     v4i fa0[4], fb0[2], fa1[4], fb1[2];
@@ -681,7 +721,20 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     // into the per-pair LDS slots.
     auto epilogue = [&](int s) __attribute__((always_inline)) {
         __syncthreads();   // the pending finalize has read and cleared the slots
-        (void)s;
+        // small-bin tile: the step's L and S1 of the tile's DOFs for the
+        // finalize, loaded by a multiply-only wave before its blocks and
+        // stored to LDS after them, so that the blocks hide the load latency
+        const bool ldw = kSmall && wave == 7 && lane < 2 * R;
+        double eL = 0.0, eS = 0.0;
+        if (ldw) {
+            const double* const dL = kBatch ? sdp->dof_L : a.dof_L;
+            const double* const dS = kBatch ? sdp->dof_S1 : a.dof_S1;
+            const double* const mo = kBatch ? sdp->mi : a.mi;
+            const long long g0 = lane < R ? r0 + lane : c0 + (lane - R);
+            const long long g = g0 < D ? g0 : D - 1;
+            eL = dL[g * S + s];
+            if (mo) eS = dS[g * S + s];
+        }
         const int kfix = a.kfix;
         unsigned long long need = ~0ull;
         if constexpr (kSmall) {
@@ -703,6 +756,10 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
                                     sm, nb, R, nstr, ntab, kfix);
                 __builtin_amdgcn_sched_barrier(0);
             }
+        if (ldw) {
+            dcL[lane] = eL;
+            dcS[lane] = eS;
+        }
     };
     // Slot sums of step fs -> S2 / MI, by wave fwave (never a stager; on the
     // small-bin tile by every wave) one iteration after the epilogue, so
@@ -716,16 +773,19 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
         asm volatile("" ::: "memory");   // read the output pointers here
         double* const s2 = kBatch ? sdp->s2 : a.s2;
         double* const mi = kBatch ? sdp->mi : a.mi;
+        // small-bin tile: inputs from the LDS caches (bin widths and pair
+        // indices of the tile, L and S1 of the step), no global load
         const double* const dof_w = kBatch ? sdp->dof_w : a.dof_w;
         const double* const dof_L = kBatch ? sdp->dof_L : a.dof_L;
         const double* const dof_S1 = kBatch ? sdp->dof_S1 : a.dof_S1;
         for (int pq = kSmall ? tid : lane; pq < R * R; pq += kSmall ? 512 : 64) {
             const int rd = pq / R, cd = pq - rd * R;
             const long long gi = r0 + rd, gj = c0 + cd;
-            const long long p = gi >= gj || gj >= D
-                                    ? -1
-                                    : (a.tilemap ? (long long)a.tilemap[(size_t)tix * R * R + pq]
-                                                 : tri_index(gi, gj, D));
+            const long long p =
+                gi >= gj || gj >= D
+                    ? -1
+                    : (a.tilemap ? (long long)(kSmall ? tmap[pq] : a.tilemap[(size_t)tix * R * R + pq])
+                                 : tri_index(gi, gj, D));
             unsigned long long Tq = 0ull;
             int nz = 0;
             for (int k = 0; k < nstr; ++k) {
@@ -738,14 +798,15 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
             }
             if (p >= 0) {
                 const double T = from_fixed(Tq, kfix);
-                const double L = dof_L[gi * S + fs] + dof_L[gj * S + fs];
-                const double area = dof_w[gi] * dof_w[gj];
+                const double L = kSmall ? dcL[rd] + dcL[R + cd] : dof_L[gi * S + fs] + dof_L[gj * S + fs];
+                const double area = kSmall ? dcw[rd] * dcw[R + cd] : dof_w[gi] * dof_w[gj];
                 const double ml = (logn + log_pos_lds(area, sm.logt, sm.invt, coef)) - T / n + L / n;
                 const double mm = ml + (double)(unsigned)(nz - 1) / (2.0 * n);
                 if (a.slot_ml >= 0) s2[((size_t)p * a.nest + a.slot_ml) * S + fs] = ml;
                 if (a.slot_mm >= 0) s2[((size_t)p * a.nest + a.slot_mm) * S + fs] = mm;
                 if (mi) {
-                    const double s1 = dof_S1[gi * S + fs] + dof_S1[gj * S + fs];
+                    const double s1 =
+                        kSmall ? dcS[rd] + dcS[R + cd] : dof_S1[gi * S + fs] + dof_S1[gj * S + fs];
                     mi[(size_t)p * S + fs] = s1 - (a.mi_from_mm ? mm : ml);
                 }
             }
@@ -786,7 +847,7 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
         auto chunk = [&](auto pref) __attribute__((always_inline)) {
             const int bn = b == kBuf8 - 1 ? 0 : b + 1;
             const int bs = bn == kBuf8 - 1 ? 0 : bn + 1;
-            const uint32_t nbase = (uint32_t)(bn * 2 * kOpBytes);
+            const uint32_t nbase = (uint32_t)(bn * 2 * kOpB);
             CodeT w2{};
             if constexpr (kStg) w2 = loadc();   // chunk g+4
             // the staging goes first: its scattered byte writes (bank
@@ -852,7 +913,7 @@ __global__ __launch_bounds__(512) void pair_batch8_kernel(PairKernelArgs a, uint
     extern __shared__ __align__(16) uint8_t smem_raw[];
     int* const s_t = reinterpret_cast<int*>(smem_raw + lds_next);
     if (blockIdx.x == 0 && threadIdx.x == 0 && a.t0) *a.t0 = wall_clock64();
-    tile8_tables<kNz, kSmall>(a.R, a.nb, a.kfix);
+    tile8_tables<kNz, kSmall, kF4>(a.R, a.nb, a.kfix);
     for (;;) {
         __syncthreads();   // the previous tile is done with LDS and s_t
         if (threadIdx.x == 0) *s_t = atomicAdd(a.work, 1);
@@ -878,7 +939,7 @@ bool pair_tile8_small(int nb) { return nb >= 1 && nb <= 8; }
 
 bool pair_tile8_supported(int R, int nb) {
     const bool small = pair_tile8_small(nb);
-    const Lay8 L = lay8(R, small, true);
+    const Lay8 L = lay8(R, small, true, false);
     if (L.ntab < 256 || L.total > kLdsCap) return false;
     return small ? R == 32 : (R <= 8 && 2 * R * kGroups <= 512);
 }
@@ -919,7 +980,7 @@ hipError_t launch_pair_tiles8(const PairKernelArgs& a_in, int n_tiles, hipStream
     const bool small = pair_tile8_small(a.nb);
     // nonzero bins are counted only for the Miller-Madow estimator
     const bool nz = a.slot_mm >= 0 || a.mi_from_mm;
-    const size_t lds = lay8(a.R, small, nz).total;
+    const size_t lds = lay8(a.R, small, nz, f4).total;
     const void* fn = tile8_pick(false, nz, small, f4);
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
@@ -937,11 +998,12 @@ hipError_t launch_pair_batch8(const PairKernelArgs& a_in, int grid, hipStream_t 
     PairKernelArgs a = a_in;
     const bool small = pair_tile8_small(a.nb);
     const bool nz = a.slot_mm >= 0 || a.mi_from_mm;
-    const size_t body = lay8(a.R, small, nz).total;
+    const bool f4 = !a.int8_only;
+    const size_t body = lay8(a.R, small, nz, f4).total;
     uint32_t next = (uint32_t)((body + 15) & ~(size_t)15);
     const size_t lds = next + 16;
     if (lds > kLdsCap) return hipErrorInvalidValue;
-    const void* fn = tile8_pick(true, nz, small, !a.int8_only);
+    const void* fn = tile8_pick(true, nz, small, f4);
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     void* args[] = {&a, &next};
