@@ -236,6 +236,40 @@ def main():
     form = mfma_form(S * (F // S))
     peak = PEAK[form]
 
+    # ---- the same pass with int8 one-hot operands and int32 counts (after
+    # the timed region): reported beside the default FP4 form's figures, with
+    # a check that both give bit-identical S2 and MI ------------------------
+    int8_form = None
+    if form == "fp4":
+        d_s2b = ctx.alloc((P, 1, S), np.float64)
+        d_mib = ctx.alloc((P, S), np.float64)
+        ctx.set_mfma_format(1)
+        try:
+            def step8():
+                ctx.dev_pair_entropy(plan, d_codes.ptr, F, F, d_types.ptr, d_ext.ptr, S,
+                                     [0], True, False, d_s1.ptr, d_s2b.ptr, d_mib.ptr)
+            step8()
+            ctx.device_sync()
+            kms8 = []
+            t8 = time.perf_counter()
+            for _ in range(args.steps):
+                step8()
+                kms8.append(ctx.last_pair_kernel_ms())
+            ctx.device_sync()
+            t8 = time.perf_counter() - t8
+        finally:
+            ctx.set_mfma_format(0)
+        k8 = statistics.mean(kms8)
+        same = (np.array_equal(d_s2b.numpy().view(np.uint64), d_s2.numpy().view(np.uint64)) and
+                np.array_equal(d_mib.numpy().view(np.uint64), d_mi.numpy().view(np.uint64)))
+        int8_form = {"value": P / (t8 / args.steps), "ms_per_step": 1000.0 * t8 / args.steps,
+                     "kernel_ms": k8, "peak": PEAK["int8"], "unit": "TOP/s",
+                     "achieved": ops / (k8 / 1000.0) / 1e12,
+                     "frac": ops / (k8 / 1000.0) / 1e12 / PEAK["int8"],
+                     "s2_mi_bit_identical_to_fp4": bool(same)}
+        d_s2b.free()
+        d_mib.free()
+
     # ---- CPU baseline: the oracle's per-pair loop on a bounded sample -------
     cpu = None
     parity = None
@@ -283,6 +317,7 @@ def main():
                      "traffic": traffic_from_profile((D, F, nb, S) == (600, 100000, 50, 10)),
                      "kernel": "pair_tile8_kernel", "kernel_ms": kernel_ms,
                      "ops_per_launch": ops},
+        "int8_form": int8_form,
         "cpu_baseline": cpu,
         "screened_vs_plain": scr,
         "parity_sample": parity,
