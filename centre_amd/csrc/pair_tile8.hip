@@ -466,7 +466,8 @@ __device__ __forceinline__ void tile8_body(const PairKernelArgs& a_in, int tix) 
     // before the next (the buffer staged next is the one staged three ago)
     uint32_t set3 = 0u, set2 = 0u, set1 = 0u;
     // Small-bin tile: this lane's item is DOF sd of the two operands and
-    // frames g16..g16+15 of each chunk, one 16-byte row segment per bin;
+    // frames g16..g16+15 of each chunk, one row segment per bin (16 bytes;
+    // 8 in the FP4 form, at byte gb);
     // the first waves stage (256 items, 128 on a diagonal tile).  Rows past
     // the system's last DOF read row 0 and stage no frames.
     const int nlive_s = (diag ? 1 : 2) * R * 4;

@@ -134,8 +134,10 @@ int centre_pair_plan_tiles(const centre_pair_plan *plan);
 /* Replaces binData2D + entropy2D (Estimators.cpp:130-205, 391-651) over all
  * pairs of the plan (EntropyCalculator::run2d_xx / run2d_xy,
  * EntropyCalculator.cpp:281-1088), plus MI = S1_i + S1_j - S2_ij
- * (getMaxMI2D, Convergance.cpp:183-194).  Joint counts are exact int32 MFMA
- * contractions of one-hot codes; ML and MM are fused into the count pass.
+ * (getMaxMI2D, Convergance.cpp:183-194).  Joint counts are exact MFMA
+ * contractions of one-hot codes (FP4 operands with f32 counts while counts
+ * stay <= 2^24, else int8 with int32 counts; see centre_set_mfma_format);
+ * ML and MM are fused into the count pass.
  * With CS or JS among the estimators, or with d_counts, the counts of groups
  * of row blocks (at most 2 GiB at a time, or one row block) are materialised
  * and every estimator and MI is evaluated from them on the device, each
