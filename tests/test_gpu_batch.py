@@ -79,13 +79,18 @@ def test_batch_matches_single_system_path(nb, ests):
     ctx.close()
 
 
-def test_batch_vs_oracle_and_bad_shape():
+@pytest.mark.parametrize("fmt", [0, 1], ids=["fp4", "int8"])
+@pytest.mark.parametrize("nb", [50, 6])
+def test_batch_vs_oracle_and_bad_shape(nb, fmt):
     """Partial last blocks on the persistent eight-wave tile (D not a
-    multiple of the tile's DOFs) against the oracle directly; tile shapes
+    multiple of the tile's DOFs; the small-bin tile at nb 6) against the
+    oracle directly, with FP4 and with int8 one-hot operands; tile shapes
     outside 1..R are refused."""
     import oracle as O
     ctx = hip.Context(0)
-    nb, S = 50, 2
+    ctx.set_mfma_format(fmt)
+    S = 2
+    R = int(hip.lib().centre_pair_dofs_per_tile(nb))
     systems = [make_system(ctx, 0xCE27E + 300 + k, 4, 3, 4 + k, 4000 + 1000 * k, nb) for k in range(3)]
     outs, descs = [], []
     for s in systems:
@@ -95,7 +100,8 @@ def test_batch_vs_oracle_and_bad_shape():
         outs.append(o)
         descs.append(hip.BatchSystem(s["codes"].ptr, F, D, F, s["types"].ptr, s["ext"].ptr,
                                      o[0].ptr, o[1].ptr, o[2].ptr))
-    ctx.batch_pair_entropy(descs, nb, S, [0], order=1, tile_dofs=5)
+    print(f"batch vs oracle nb={nb} fmt={fmt}", flush=True)
+    ctx.batch_pair_entropy(descs, nb, S, [0], order=1, tile_dofs=R)
     for s, o in zip(systems, outs):
         D = s["D"]
         codes, types, ext = s["codes"].numpy(), s["types"].numpy(), s["ext"].numpy()
@@ -105,7 +111,8 @@ def test_batch_vs_oracle_and_bad_shape():
         s1 = O.dof_entropies(codes, types, ext, S, [nb])
         mi = s1[iu, 0, 0, :] + s1[ju, 0, 0, :] - want[:, 0, 0, :]
         np.testing.assert_allclose(o[2].numpy(), mi, rtol=1e-9, atol=1e-11)
-    for bad in (6, -1):
+    for bad in (R + 1, -1):
         with pytest.raises(RuntimeError):
             ctx.batch_pair_entropy(descs, nb, S, [0], order=1, tile_dofs=bad)
+    ctx.close()
     ctx.close()
