@@ -307,6 +307,10 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": form,
+        "dtype_note": ("one-hot MFMA operands are exactly 0 or 1 (e2m1 nibbles in the fp4 form); "
+                       "joint counts are exact integers (f32 below 2^24, int32 in the int8 form); "
+                       "entropy sums are FP64 and fixed-point integer; int8_form checks that S2 and "
+                       "MI are bit-identical between the two forms"),
         "data": "synthetic (seeded Philox BAT generator, latent-coupled)",
         "config": {"workload": f"full pairwise MIE pair stage, {D} DOF "
                                f"({nbd}B/{nan}A/{ndh}T) x {F} frames, {nb} bins, "
