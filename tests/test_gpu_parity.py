@@ -363,6 +363,16 @@ def test_code_out_of_range_is_an_error(ctx, bad, F):
     assert e.value.code == -4
 
 
+def test_mfma_format_argument(ctx):
+    """Only 0 (FP4 where exact) and 1 (int8) are operand formats; another
+    value is refused with -3 and leaves the context's format as it was."""
+    for bad in (-1, 2, 7):
+        with pytest.raises(hip.CentreError) as e:
+            ctx.set_mfma_format(bad)
+        assert e.value.code == -3
+    ctx.set_mfma_format(0)
+
+
 # ---------------------------------------------------------------------------
 def kruskal_set(n, src, dst, w):
     w = np.asarray(w, np.float64) + 0.0          # -0.0 and +0.0 tie
